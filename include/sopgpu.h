@@ -1,0 +1,201 @@
+/*
+ * sopgpu.h -- C ABI of the MI355X SOP Langevin engine.
+ *
+ * Plain pointers and sizes only; all device memory is owned by the engine.
+ * Every function returns 0 on success and a negative SOP_E* code on
+ * failure, with the message in sop_last_error().  The reference reports
+ * the same conditions with DIE() and exit(-1) (src/Util/wrapper.h:23,
+ * src/Util/wrapper.cpp:28-42); here they are per-call errors so a serving
+ * loop can fail one job and keep the others.
+ *
+ * Reference interfaces each group replaces (paths relative to
+ * BarsegovGroup/SOP-GPU):
+ *   engine / jobs   GSOP state + initGPU/initFF/runGPU, src/gsop.h:39-67,
+ *                   src/gsop.cu:45-163; Potential/Updater/Integrator
+ *                   plugins src/gsop.h:69-105
+ *   lists           CovalentPotential::buildMap covalent.cu:65-109,
+ *                   NativePotential::buildMap native.cu:58-107,
+ *                   PossiblepairList::update possiblepairlist.cu:41-48,
+ *                   PairList::update pairlist.cu:100-128
+ *   energies        SOPPotential::computeEnergy / sumEnergies gsop.cu:255-271
+ *   RNG             initRand / generateSeeds / rforce, src/Integrators/ht.cu
+ *   config runs     sop-gpu main(), src/sop-gpu.cpp:18-65
+ */
+#ifndef SOPGPU_H
+#define SOPGPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    SOP_OK = 0,
+    SOP_E_ARG = -1,      /* bad argument */
+    SOP_E_HIP = -2,      /* HIP runtime error */
+    SOP_E_MEM = -3,      /* HBM budget exhausted */
+    SOP_E_LIMIT = -4,    /* a topology limit (max_covalent, max_native, ...) exceeded */
+    SOP_E_OVERFLOW = -5, /* a pairlist outgrew its capacity on the device */
+    SOP_E_CONFIG = -6,   /* parameter / topology / coordinate file error */
+    SOP_E_STATE = -7,    /* call not valid in the job's current state */
+};
+
+/* Parameters of one system, named after the reference's PARAMETER()s
+ * (Appendix of each plugin header: covalent.h:12-14, native.h:11-12,
+ * pairs.h:12-15, pairlist.h:12-14, possiblepairlist.h:14-15,
+ * langevin.h:13-23, pulling.h:15-26, indentation.h:12-37, gsop.h:35,
+ * output_manager.h:19, dcd_manager.h:13).  Field order matches the
+ * oracle's or_params. */
+typedef struct sop_params {
+    float kspring_cov, R_limit;
+    float R_limit_bond;
+    float pairs_cutoff, el, a;
+    float pairlist_cutoff, pairs_threshold;
+    int pairs_freq, possiblepairs_freq;
+    int max_covalent, max_native, max_pairs, max_possiblePairs;
+    float timestep, zeta, temperature;
+    int minimization;
+    int seed; /* seed + firstrun, as initRand receives it (langevin.cu:23) */
+    int heating, tempFreq;
+    float initialT, deltaT;
+    int nav, outputtiming, dcdfreq;
+    int pulling;
+    float deltax, k_trans, fconst;
+    int pullFreq;
+    int pullDirectionVector; /* 0 endToEnd, 1 vector */
+    float pullVector[3];
+    int fixedEnd, pulledEnd;
+    int indentation;
+    float indChip[3], indTip[3], indDirection[3], indSurfR0[3], indSurfN[3];
+    float indTipR, indKs, indDeltaX;
+    int indMoveSurface, indFixTrans;
+    float indTipSigma, indSurfSigma, indTipEl, indSurfEl;
+    float indTipA, indTipB, indSurfA, indSurfB;
+    float indTipZeta;
+    int indOutputFreq;
+    long long indRetractionStep;
+} sop_params;
+
+/* Topology and coordinates of one system in .top order
+ * (src/IO/topio.cpp:36-80): bonds, native contacts (r0, eh) and the
+ * optional [ pairs ] section that seeds the first possible-pairs list. */
+typedef struct sop_system {
+    int n;
+    const float* coords4; /* n x (x,y,z,w) */
+    int nbonds;
+    const int* bond_i;
+    const int* bond_j;
+    const float* bond_r0;
+    int nnative;
+    const int* nat_i;
+    const int* nat_j;
+    const float* nat_r0;
+    const float* nat_eh;
+    int npairs;
+    const int* pair_i;
+    const int* pair_j;
+    int nfixed;
+    const int* fixed;
+    int npulled;
+    const int* pulled;
+} sop_system;
+
+typedef struct sop_engine sop_engine;
+
+const char* sop_last_error(void);
+const char* sop_version(void);
+
+/* Engine holding any number of jobs on one GPU.  hbm_budget bounds the
+ * arena all job arrays live in (0: 90% of the device's free memory). */
+int sop_engine_create(int device, unsigned long long hbm_budget, sop_engine** out);
+void sop_engine_destroy(sop_engine* e);
+
+/* Admit a job: builds the slot-major maps, seeds the RNG, uploads, and runs
+ * the pre-loop Verlet build of runGPU (gsop.cu:126).  numsteps follows the
+ * reference: the job runs numsteps/stride+1 blocks of stride steps. */
+int sop_engine_add_job(sop_engine* e, const sop_params* p, const sop_system* s, long long numsteps, int* job_id);
+int sop_engine_remove_job(sop_engine* e, int job_id);
+/* bytes the job would take in the arena */
+int sop_engine_job_bytes(const sop_params* p, const sop_system* s, unsigned long long* bytes);
+
+/* Launch shape: max_blocks caps the persistent step grid (0: one
+ * workgroup per 4 tiles); use_graph != 0 replays each block from a
+ * captured hipGraph. */
+int sop_engine_set_tuning(sop_engine* e, int max_blocks, int use_graph);
+/* Pick max_blocks by timing candidate grids on the admitted jobs. */
+int sop_engine_autotune(sop_engine* e, int* max_blocks_out);
+
+/* Batched stepping: every unfinished job advances one block of L steps
+ * (L = gcd of the jobs' strides) per call, all jobs in one step kernel
+ * per step. */
+int sop_engine_run_blocks(sop_engine* e, int nblocks);
+/* Plain stepping: run one job alone to its end, one launch per kernel,
+ * in runGPU's order (the one-job-per-launch baseline). */
+int sop_engine_run_job(sop_engine* e, int job_id);
+int sop_engine_block_steps(sop_engine* e, int* L);
+int sop_engine_synchronize(sop_engine* e);
+
+int sop_engine_job_status(sop_engine* e, int job_id, long long* step, long long* end_step, int* overflow);
+int sop_engine_active_jobs(sop_engine* e, int* count, int* max_n);
+
+/* Read back (synchronous). */
+int sop_engine_get_coords(sop_engine* e, int job_id, float* out4);
+int sop_engine_set_coords(sop_engine* e, int job_id, const float* in4);
+int sop_engine_get_temperature(sop_engine* e, int job_id, float* out);
+int sop_engine_get_rng(sop_engine* e, int job_id, uint32_t* out4);
+int sop_engine_get_ext(sop_engine* e, int job_id, float* out4);
+/* which: 0 covalent, 1 native, 2 Verlet pairs, 3 possible pairs.
+ * counts[n]; entries[cap*n] slot-major; *cap receives the capacity. */
+int sop_engine_list_cap(sop_engine* e, int job_id, int which, int* cap);
+int sop_engine_get_list(sop_engine* e, int job_id, int which, int* counts, int* entries);
+/* tip(3), chip(3), micaR(3), tip force average (4) */
+int sop_engine_get_indentation(sop_engine* e, int job_id, float* out13);
+
+/* Parity hooks: forces from the current coordinates and lists (no state
+ * change), per-bead energies (3n) and per-system sums (3) as
+ * SOPPotential::sumEnergies forms them. */
+int sop_engine_compute_forces(sop_engine* e, int job_id, float* out4);
+int sop_engine_compute_energies(sop_engine* e, int job_id, float* per_bead3n, float* sums3);
+/* Force a list rebuild now (mask: 1 possible, 2 Verlet). */
+int sop_engine_rebuild_lists(sop_engine* e, int job_id, int mask);
+
+/* Timing of the step kernel with HIP events on the engine's stream:
+ * while enabled, each step-kernel launch is bracketed by events. */
+int sop_engine_profile(sop_engine* e, int enable);
+int sop_engine_profile_read(sop_engine* e, long long* launches, double* total_ms);
+/* Average list lengths over admitted jobs' beads (covalent, native,
+ * Verlet, possible) and the bead count, for the roofline byte count. */
+int sop_engine_list_stats(sop_engine* e, double* avg4, long long* beads);
+
+/* RNG (src/Integrators/ht.cu): seed table on the host with the reference's
+ * ran2 stream, and a device probe that advances `count` draws per bead and
+ * returns the uint32 outputs (gaussian=0) or Gaussian triples (gaussian=1). */
+int sop_generate_seeds(int seed, int n, uint32_t* out4);
+int sop_rng_probe(int device, const uint32_t* seeds4, int n, int count, int gaussian, float* out, uint32_t* state_out4);
+
+/* Synthetic systems: a seeded chain of n beads (3.8 A bonds, compact
+ * globule) with native contacts |i-j|>2 and r<8 A, eh in [eh_lo, eh_hi].
+ * Arrays are allocated by the library; free with sop_free_system. */
+typedef struct sop_synth {
+    int n;
+    float* coords4;
+    int nbonds;
+    int* bond_i;
+    int* bond_j;
+    float* bond_r0;
+    int nnative;
+    int* nat_i;
+    int* nat_j;
+    float* nat_r0;
+    float* nat_eh;
+} sop_synth;
+int sop_synth_chain(int n, unsigned long long seed, float density, float eh_lo, float eh_hi, sop_synth* out);
+/* nchains chains of n beads each placed on a sphere of radius R,
+ * inter-chain contacts added for r < 8 A. */
+int sop_synth_shell(int nchains, int n, unsigned long long seed, float radius, float eh_lo, float eh_hi, sop_synth* out);
+void sop_free_system(sop_synth* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,247 @@
+// capi.cpp -- extern "C" entry points (include/sopgpu.h).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/sopgpu.h"
+#include "engine.h"
+#include "hostutil.h"
+#include "kernels.h"
+
+namespace sop {
+const char* last_error();
+}
+
+struct sop_engine {
+    sop::Engine* eng;
+};
+
+#define ENG(e)                                                   \
+    if (!(e) || !(e)->eng) return sop::fail(SOP_E_ARG, "null engine"); \
+    sop::Engine& E = *(e)->eng
+
+extern "C" {
+
+const char* sop_last_error(void) { return sop::last_error(); }
+const char* sop_version(void) { return "sop-mi355x 0.1 (gfx950)"; }
+
+int sop_engine_create(int device, unsigned long long hbm_budget, sop_engine** out) {
+    if (!out) return sop::fail(SOP_E_ARG, "null output");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return sop::fail(SOP_E_HIP, "no HIP device available");
+    if (device < 0 || device >= ndev) return sop::fail(SOP_E_ARG, "device index out of range");
+    if (hbm_budget == 0) {
+        size_t fr = 0, tot = 0;
+        if (hipSetDevice(device) != hipSuccess || hipMemGetInfo(&fr, &tot) != hipSuccess)
+            return sop::fail(SOP_E_HIP, "hipMemGetInfo failed");
+        hbm_budget = (unsigned long long)(0.9 * (double)fr);
+    }
+    try {
+        auto* e = new sop_engine;
+        e->eng = new sop::Engine(device, (size_t)hbm_budget);
+        *out = e;
+        return 0;
+    } catch (const sop::Error& err) {
+        return sop::fail(err.code, err.msg);
+    } catch (const std::bad_alloc&) {
+        return sop::fail(SOP_E_MEM, "host allocation failed");
+    }
+}
+
+void sop_engine_destroy(sop_engine* e) {
+    if (!e) return;
+    delete e->eng;
+    delete e;
+}
+
+int sop_engine_add_job(sop_engine* e, const sop_params* p, const sop_system* s, long long numsteps, int* job_id) {
+    ENG(e);
+    if (!p || !s || !job_id) return sop::fail(SOP_E_ARG, "null argument");
+    try {
+        return E.add_job(*p, *s, numsteps, job_id);
+    } catch (const std::bad_alloc&) {
+        return sop::fail(SOP_E_MEM, "host allocation failed");
+    }
+}
+
+int sop_engine_remove_job(sop_engine* e, int job_id) {
+    ENG(e);
+    return E.remove_job(job_id);
+}
+
+int sop_engine_job_bytes(const sop_params* p, const sop_system* s, unsigned long long* bytes) {
+    if (!p || !s || !bytes) return sop::fail(SOP_E_ARG, "null argument");
+    return sop::Engine::job_bytes(*p, *s, bytes);
+}
+
+int sop_engine_set_tuning(sop_engine* e, int max_blocks, int use_graph) {
+    ENG(e);
+    E.set_tuning(max_blocks, use_graph);
+    return 0;
+}
+
+int sop_engine_autotune(sop_engine* e, int* max_blocks_out) {
+    ENG(e);
+    int b = 0;
+    int rc = E.autotune(&b);
+    if (max_blocks_out) *max_blocks_out = b;
+    return rc;
+}
+
+int sop_engine_run_blocks(sop_engine* e, int nblocks) {
+    ENG(e);
+    if (nblocks < 0) return sop::fail(SOP_E_ARG, "nblocks < 0");
+    return E.run_blocks(nblocks);
+}
+
+int sop_engine_run_job(sop_engine* e, int job_id) {
+    ENG(e);
+    return E.run_job(job_id);
+}
+
+int sop_engine_block_steps(sop_engine* e, int* L) {
+    ENG(e);
+    if (!L) return sop::fail(SOP_E_ARG, "null argument");
+    *L = E.block_steps();
+    return 0;
+}
+
+int sop_engine_synchronize(sop_engine* e) {
+    ENG(e);
+    return E.synchronize();
+}
+
+int sop_engine_job_status(sop_engine* e, int job_id, long long* step, long long* end_step, int* overflow) {
+    ENG(e);
+    return E.job_status(job_id, step, end_step, overflow);
+}
+
+int sop_engine_active_jobs(sop_engine* e, int* count, int* max_n) {
+    ENG(e);
+    int c = 0, m = 0;
+    for (auto& j : E.jobs())
+        if (j.step < j.end_step) {
+            ++c;
+            if (j.n > m) m = j.n;
+        }
+    if (count) *count = c;
+    if (max_n) *max_n = m;
+    return 0;
+}
+
+int sop_engine_get_coords(sop_engine* e, int job_id, float* out4) {
+    ENG(e);
+    if (!out4) return sop::fail(SOP_E_ARG, "null output");
+    return E.get_coords(job_id, out4);
+}
+
+int sop_engine_set_coords(sop_engine* e, int job_id, const float* in4) {
+    ENG(e);
+    if (!in4) return sop::fail(SOP_E_ARG, "null input");
+    return E.set_coords(job_id, in4);
+}
+
+int sop_engine_get_temperature(sop_engine* e, int job_id, float* out) {
+    ENG(e);
+    return E.get_array(job_id, 0, out);
+}
+
+int sop_engine_get_rng(sop_engine* e, int job_id, uint32_t* out4) {
+    ENG(e);
+    return E.get_array(job_id, 1, out4);
+}
+
+int sop_engine_get_ext(sop_engine* e, int job_id, float* out4) {
+    ENG(e);
+    return E.get_array(job_id, 2, out4);
+}
+
+int sop_engine_list_cap(sop_engine* e, int job_id, int which, int* cap) {
+    ENG(e);
+    if (!cap) return sop::fail(SOP_E_ARG, "null output");
+    return E.list_cap(job_id, which, cap);
+}
+
+int sop_engine_get_list(sop_engine* e, int job_id, int which, int* counts, int* entries) {
+    ENG(e);
+    if (!counts || !entries) return sop::fail(SOP_E_ARG, "null output");
+    return E.get_list(job_id, which, counts, entries);
+}
+
+int sop_engine_get_indentation(sop_engine* e, int job_id, float* out13) {
+    ENG(e);
+    if (!out13) return sop::fail(SOP_E_ARG, "null output");
+    return E.get_indentation(job_id, out13);
+}
+
+int sop_engine_compute_forces(sop_engine* e, int job_id, float* out4) {
+    ENG(e);
+    if (!out4) return sop::fail(SOP_E_ARG, "null output");
+    return E.compute_forces(job_id, out4);
+}
+
+int sop_engine_compute_energies(sop_engine* e, int job_id, float* per_bead3n, float* sums3) {
+    ENG(e);
+    return E.compute_energies(job_id, per_bead3n, sums3);
+}
+
+int sop_engine_rebuild_lists(sop_engine* e, int job_id, int mask) {
+    ENG(e);
+    return E.rebuild_lists(job_id, mask);
+}
+
+int sop_engine_profile(sop_engine* e, int enable) {
+    ENG(e);
+    E.profile(enable != 0);
+    return 0;
+}
+
+int sop_engine_profile_read(sop_engine* e, long long* launches, double* total_ms) {
+    ENG(e);
+    long long l = 0;
+    double ms = 0.0;
+    E.profile_read(&l, &ms);
+    if (launches) *launches = l;
+    if (total_ms) *total_ms = ms;
+    return 0;
+}
+
+int sop_engine_list_stats(sop_engine* e, double* avg4, long long* beads) {
+    ENG(e);
+    if (!avg4 || !beads) return sop::fail(SOP_E_ARG, "null output");
+    return E.list_stats(avg4, beads);
+}
+
+int sop_generate_seeds(int seed, int n, uint32_t* out4) {
+    if (n < 0 || (n > 0 && !out4)) return sop::fail(SOP_E_ARG, "bad arguments");
+    sop::generate_seeds(seed, n, out4);
+    return 0;
+}
+
+int sop_rng_probe(int device, const uint32_t* seeds4, int n, int count, int gaussian, float* out, uint32_t* state_out4) {
+    if (n <= 0 || count <= 0 || !seeds4 || !out) return sop::fail(SOP_E_ARG, "bad arguments");
+    if (hipSetDevice(device) != hipSuccess) return sop::fail(SOP_E_HIP, "hipSetDevice failed");
+    uint4* d_state = nullptr;
+    float* d_out = nullptr;
+    const size_t per = gaussian ? 3 : 1;
+    const size_t outbytes = (size_t)n * count * per * 4;
+    if (hipMalloc(&d_state, (size_t)n * 16) != hipSuccess || hipMalloc(&d_out, outbytes) != hipSuccess) {
+        (void)hipFree(d_state);
+        return sop::fail(SOP_E_HIP, "hipMalloc failed");
+    }
+    int rc = 0;
+    hipError_t err = hipMemcpy(d_state, seeds4, (size_t)n * 16, hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = sop::launch_rng_probe(d_state, n, count, gaussian, d_out, nullptr);
+    if (err == hipSuccess) err = hipDeviceSynchronize();
+    if (err == hipSuccess) err = hipMemcpy(out, d_out, outbytes, hipMemcpyDeviceToHost);
+    if (err == hipSuccess && state_out4) err = hipMemcpy(state_out4, d_state, (size_t)n * 16, hipMemcpyDeviceToHost);
+    if (err != hipSuccess) rc = sop::fail(SOP_E_HIP, std::string("rng probe: ") + hipGetErrorString(err));
+    (void)hipFree(d_state);
+    (void)hipFree(d_out);
+    return rc;
+}
+
+}  // extern "C"

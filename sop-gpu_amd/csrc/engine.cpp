@@ -1,0 +1,970 @@
+// engine.cpp -- job admission, HBM arena and the block loop.
+#include "engine.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+
+#include "hostutil.h"
+#include "kernels.h"
+
+namespace sop {
+
+static thread_local std::string g_error;
+static thread_local int g_code = 0;
+
+void set_error(int code, const std::string& msg) {
+    g_code = code;
+    g_error = msg;
+}
+int fail(int code, const std::string& msg) {
+    set_error(code, msg);
+    return code;
+}
+const char* last_error() { return g_error.c_str(); }
+
+// ---------------------------------------------------------------------------
+// Arena
+// ---------------------------------------------------------------------------
+static constexpr size_t kAlign = 256;
+static constexpr size_t kChunk = size_t(1) << 30;
+
+Arena::~Arena() {
+    for (auto& c : chunks_) (void)hipFree(c.base);
+}
+
+void* Arena::alloc(size_t bytes) {
+    bytes = (bytes + kAlign - 1) / kAlign * kAlign;
+    if (bytes == 0) bytes = kAlign;
+    for (size_t ci = 0; ci < chunks_.size(); ++ci) {
+        auto& c = chunks_[ci];
+        for (auto it = c.free.begin(); it != c.free.end(); ++it) {
+            if (it->second >= bytes) {
+                size_t off = it->first, len = it->second;
+                c.free.erase(it);
+                if (len > bytes) c.free[off + bytes] = len - bytes;
+                void* p = c.base + off;
+                live_[p] = {(int)ci, bytes};
+                in_use_ += bytes;
+                return p;
+            }
+        }
+    }
+    size_t want = std::max(bytes, kChunk);
+    if (reserved_ + want > budget_) want = bytes;
+    if (reserved_ + want > budget_) return nullptr;
+    char* base = nullptr;
+    if (hipMalloc(&base, want) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    Chunk c;
+    c.base = base;
+    c.size = want;
+    if (want > bytes) c.free[bytes] = want - bytes;
+    chunks_.push_back(c);
+    reserved_ += want;
+    void* p = base;
+    live_[p] = {(int)chunks_.size() - 1, bytes};
+    in_use_ += bytes;
+    return p;
+}
+
+void Arena::release(void* p) {
+    auto it = live_.find(p);
+    if (it == live_.end()) return;
+    auto& c = chunks_[it->second.first];
+    size_t off = (char*)p - c.base, len = it->second.second;
+    in_use_ -= len;
+    live_.erase(it);
+    auto nx = c.free.lower_bound(off);
+    if (nx != c.free.end() && off + len == nx->first) {
+        len += nx->second;
+        nx = c.free.erase(nx);
+    }
+    if (nx != c.free.begin()) {
+        auto pv = std::prev(nx);
+        if (pv->first + pv->second == off) {
+            pv->second += len;
+            return;
+        }
+    }
+    c.free[off] = len;
+}
+
+// ---------------------------------------------------------------------------
+// Engine
+// ---------------------------------------------------------------------------
+Engine::Engine(int device, size_t budget) : device_(device), arena_(budget) {
+    if (hipSetDevice(device) != hipSuccess) throw Error{SOP_E_HIP, "hipSetDevice failed"};
+    if (hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking) != hipSuccess)
+        throw Error{SOP_E_HIP, "hipStreamCreate failed"};
+}
+
+Engine::~Engine() {
+    (void)hipSetDevice(device_);
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    drop_graph();
+    for (auto& p : ev_pool_) {
+        (void)hipEventDestroy(p.first);
+        (void)hipEventDestroy(p.second);
+    }
+    for (auto& j : jobs_)
+        for (void* b : j.blocks) arena_.release(b);
+    (void)hipFree(d_jobs_);
+    (void)hipFree(d_state_);
+    (void)hipFree(d_tiles_);
+    (void)hipFree(d_tiles1_);
+    (void)hipFree(d_scratch4_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+int Engine::check(hipError_t e, const char* what) {
+    if (e == hipSuccess) return 0;
+    return fail(SOP_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+HostJob* Engine::find(int id) {
+    for (auto& j : jobs_)
+        if (j.id == id) return &j;
+    return nullptr;
+}
+
+int Engine::slot_of(int id) const {
+    for (size_t i = 0; i < jobs_.size(); ++i)
+        if (jobs_[i].id == id) return (int)i;
+    return -1;
+}
+
+// Host-side construction of one job's maps, as the reference's
+// constructors build them (covalent.cu:65-109, native.cu:58-107,
+// pairlist.cu:41-57, pulling.cu:36-146, indentation.cu:84-200).
+struct BuiltJob {
+    int n = 0;
+    int cov_cap = 1, nat_cap = 1, pair_cap = 1, pp_cap = 1, excl_cap = 1;
+    std::vector<int> cov_cnt, cov_j, nat_cnt, nat_j, pp_cnt, pp_j, excl_cnt, excl_j, pulled;
+    std::vector<float> cov_r0;
+    std::vector<float2> nat_par;
+    std::vector<float4> coord, ext;
+    std::vector<uint4> rng;
+    DevJob dj{};
+    DevJobState st{};
+    long long end_step = 0;
+};
+
+static int build_job(const sop_params& p, const sop_system& s, long long numsteps, BuiltJob& b) {
+    const int n = s.n;
+    if (n <= 0 || !s.coords4) return fail(SOP_E_ARG, "system has no beads");
+    if (p.timestep <= 0.0f || p.zeta <= 0.0f) return fail(SOP_E_ARG, "timestep and zeta must be positive");
+    if (p.pairs_freq <= 0 || p.possiblepairs_freq <= 0 || p.nav <= 0 || p.outputtiming <= 0 || p.dcdfreq <= 0)
+        return fail(SOP_E_ARG, "update frequencies must be positive");
+    if (p.max_covalent <= 0 || p.max_native <= 0 || p.max_pairs <= 0 || p.max_possiblePairs <= 0)
+        return fail(SOP_E_ARG, "list limits must be positive");
+    b.n = n;
+    auto in_range = [n](int i) { return i >= 0 && i < n; };
+    // covalent
+    b.cov_cnt.assign(n, 0);
+    std::vector<std::vector<std::pair<int, float>>> cov(n);
+    for (int k = 0; k < s.nbonds; ++k) {
+        int i = s.bond_i[k], j = s.bond_j[k];
+        if (!in_range(i) || !in_range(j)) return fail(SOP_E_ARG, "bond index out of range");
+        cov[i].push_back({j, s.bond_r0[k]});
+        cov[j].push_back({i, s.bond_r0[k]});
+        if ((int)cov[i].size() > p.max_covalent || (int)cov[j].size() > p.max_covalent)
+            return fail(SOP_E_LIMIT, "Maximum number of covalent bonds exceeded the limit of " +
+                                         std::to_string(p.max_covalent) + ".");
+    }
+    for (int i = 0; i < n; ++i) b.cov_cap = std::max(b.cov_cap, (int)cov[i].size());
+    b.cov_j.assign((size_t)b.cov_cap * n, 0);
+    b.cov_r0.assign((size_t)b.cov_cap * n, 0.0f);
+    for (int i = 0; i < n; ++i) {
+        b.cov_cnt[i] = (int)cov[i].size();
+        for (size_t k = 0; k < cov[i].size(); ++k) {
+            b.cov_j[k * n + i] = cov[i][k].first;
+            b.cov_r0[k * n + i] = cov[i][k].second;
+        }
+    }
+    // native: r0^2 and -12 eh / r0^2 formed in double, stored as float
+    b.nat_cnt.assign(n, 0);
+    std::vector<std::vector<std::pair<int, float2>>> nat(n);
+    for (int k = 0; k < s.nnative; ++k) {
+        int i = s.nat_i[k], j = s.nat_j[k];
+        if (!in_range(i) || !in_range(j)) return fail(SOP_E_ARG, "native contact index out of range");
+        float r0 = s.nat_r0[k], eh = s.nat_eh[k];
+        float2 par;
+        par.x = r0 * r0;
+        par.y = (float)(-12.0 * (double)eh / (double)(r0 * r0));
+        nat[i].push_back({j, par});
+        nat[j].push_back({i, par});
+        if ((int)nat[i].size() > p.max_native || (int)nat[j].size() > p.max_native)
+            return fail(SOP_E_LIMIT, "ERROR: Maximum number of native contacts exceeded the limit of " +
+                                         std::to_string(p.max_native) + ".");
+    }
+    for (int i = 0; i < n; ++i) b.nat_cap = std::max(b.nat_cap, (int)nat[i].size());
+    b.nat_j.assign((size_t)b.nat_cap * n, 0);
+    b.nat_par.assign((size_t)b.nat_cap * n, float2{0.0f, 0.0f});
+    for (int i = 0; i < n; ++i) {
+        b.nat_cnt[i] = (int)nat[i].size();
+        for (size_t k = 0; k < nat[i].size(); ++k) {
+            b.nat_j[k * n + i] = nat[i][k].first;
+            b.nat_par[k * n + i] = nat[i][k].second;
+        }
+    }
+    // exclusions for the possible-pairs build: sorted unique partners
+    b.excl_cnt.assign(n, 0);
+    std::vector<std::vector<int>> ex(n);
+    for (int i = 0; i < n; ++i) {
+        for (auto& c : cov[i]) ex[i].push_back(c.first);
+        for (auto& c : nat[i]) ex[i].push_back(c.first);
+        std::sort(ex[i].begin(), ex[i].end());
+        ex[i].erase(std::unique(ex[i].begin(), ex[i].end()), ex[i].end());
+        b.excl_cap = std::max(b.excl_cap, (int)ex[i].size());
+    }
+    b.excl_j.assign((size_t)b.excl_cap * n, 0);
+    for (int i = 0; i < n; ++i) {
+        b.excl_cnt[i] = (int)ex[i].size();
+        for (size_t k = 0; k < ex[i].size(); ++k) b.excl_j[k * n + i] = ex[i][k];
+    }
+    // possible list capacity: no bead has more than n-1 partners
+    const int nm1 = std::max(1, n - 1);
+    b.pp_cap = std::min(p.max_possiblePairs, nm1);
+    b.pair_cap = std::min(p.max_pairs, nm1);
+    b.pp_cnt.assign(n, 0);
+    std::vector<std::vector<int>> pp(n);
+    for (int k = 0; k < s.npairs; ++k) {
+        int i = s.pair_i[k], j = s.pair_j[k];
+        if (!in_range(i) || !in_range(j)) return fail(SOP_E_ARG, "pair index out of range");
+        pp[i].push_back(j);
+        pp[j].push_back(i);
+        if ((int)pp[i].size() > p.max_possiblePairs || (int)pp[j].size() > p.max_possiblePairs)
+            return fail(SOP_E_LIMIT, "ERROR: Maximum number of possible pairs exceeded the limit of " +
+                                         std::to_string(p.max_possiblePairs) + ".");
+    }
+    for (int i = 0; i < n; ++i) b.pp_cap = std::max(b.pp_cap, (int)pp[i].size());
+    b.pp_j.assign((size_t)b.pp_cap * n, 0);
+    for (int i = 0; i < n; ++i) {
+        b.pp_cnt[i] = (int)pp[i].size();
+        for (size_t k = 0; k < pp[i].size(); ++k) b.pp_j[k * n + i] = pp[i][k];
+    }
+    // coordinates, w = 0 (gsop.cu:196)
+    b.coord.resize(n);
+    for (int i = 0; i < n; ++i)
+        b.coord[i] = make_float4(s.coords4[4 * i], s.coords4[4 * i + 1], s.coords4[4 * i + 2], 0.0f);
+    // RNG seeds (langevin.cu:23-24)
+    b.rng.resize(n);
+    generate_seeds(p.seed, n, reinterpret_cast<uint32_t*>(b.rng.data()));
+
+    DevJob& J = b.dj;
+    DevJobState& S = b.st;
+    J.n = n;
+    J.cov_cap = b.cov_cap;
+    J.nat_cap = b.nat_cap;
+    J.pair_cap = b.pair_cap;
+    J.pp_cap = b.pp_cap;
+    J.excl_cap = b.excl_cap;
+    J.ks = p.kspring_cov;
+    J.R2 = p.R_limit * p.R_limit;
+    J.a2 = p.a * p.a;
+    J.m6 = -6.0f * p.el / J.a2;
+    J.rc2 = p.pairs_cutoff * p.pairs_cutoff;
+    J.el = p.el;
+    J.pl_cut = p.pairlist_cutoff;
+    J.pp_thr = p.pairs_threshold;
+    J.h = p.timestep;
+    J.zeta = p.zeta;
+    J.hOverZeta = p.timestep / p.zeta;
+    J.tempNorm = (float)((double)p.zeta / (6.0 * (double)p.timestep));
+    J.nav = p.nav;
+    J.pairs_freq = p.pairs_freq;
+    J.pp_freq = p.possiblepairs_freq;
+    if (p.minimization) J.flags |= JF_MINIMIZATION;
+    if (p.heating) {
+        if (p.tempFreq <= 0) return fail(SOP_E_ARG, "tempFreq must be positive");
+        J.flags |= JF_HEATING;
+        J.tempFreq = p.tempFreq;
+        J.initialT = p.initialT;
+        J.deltaT = p.deltaT;
+    }
+    {
+        const float temp = p.heating ? p.initialT : p.temperature;
+        S.var = sqrtf(2.0f * temp * p.zeta / p.timestep);
+    }
+    // pulling (pulling.cu:36-146)
+    b.ext.assign(n, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    if (p.pulling) {
+        if (p.deltax == 0.0f && p.fconst == 0.0f)
+            return fail(SOP_E_CONFIG, "ERROR: Either 'deltax' or 'fconst' parameter should be specified to initiate pulling");
+        J.flags |= JF_PULLING;
+        if (!in_range(p.pulledEnd) || !in_range(p.fixedEnd)) return fail(SOP_E_ARG, "pulledEnd/fixedEnd out of range");
+        float v[3];
+        if (p.pullDirectionVector) {
+            v[0] = p.pullVector[0];
+            v[1] = p.pullVector[1];
+            v[2] = p.pullVector[2];
+        } else {
+            const float4 a = b.coord[p.pulledEnd], c = b.coord[p.fixedEnd];
+            float x = a.x - c.x, y = a.y - c.y, z = a.z - c.z;
+            float l = sqrtf(x * x + y * y + z * z);
+            v[0] = x / l;
+            v[1] = y / l;
+            v[2] = z / l;
+        }
+        for (int d = 0; d < 3; ++d) J.pullVec[d] = v[d];
+        const float4 pe = b.coord[p.pulledEnd];
+        J.cant0[0] = S.cant[0] = pe.x;
+        J.cant0[1] = S.cant[1] = pe.y;
+        J.cant0[2] = S.cant[2] = pe.z;
+        for (int k = 0; k < s.npulled; ++k) {
+            int i = s.pulled[k];
+            // the reference accepts i == N (pulling.cu:113); it would write
+            // past the system, so it is rejected here
+            if (!in_range(i)) return fail(SOP_E_CONFIG, "Invalid pulling bead #" + std::to_string(i));
+            b.ext[i] = make_float4(v[0] * p.fconst, v[1] * p.fconst, v[2] * p.fconst, 2.0f);
+            b.pulled.push_back(i);
+        }
+        for (int k = 0; k < s.nfixed; ++k) {
+            int i = s.fixed[k];
+            if (!in_range(i)) return fail(SOP_E_CONFIG, "Invalid fixed bead #" + std::to_string(i));
+            b.ext[i] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+        }
+        if (p.deltax != 0.0f) {
+            if (p.pullFreq <= 0) return fail(SOP_E_ARG, "pullFreq must be positive");
+            J.flags |= JF_PULL_UPDATER;
+            J.k_trans = p.k_trans;
+            J.deltax = p.deltax;
+            J.pullFreq = p.pullFreq;
+            J.pulledEnd = p.pulledEnd;
+        }
+    }
+    // indentation, continuous surface (indentation.cu:84-200)
+    if (p.indentation) {
+        J.flags |= JF_INDENTATION;
+        for (int d = 0; d < 3; ++d) {
+            S.chip[d] = J.chip0[d] = p.indChip[d];
+            S.tip[d] = p.indTip[d];
+            J.dir[d] = p.indDirection[d];
+            S.micaR[d] = J.micaR0[d] = p.indSurfR0[d];
+            J.micaN[d] = p.indSurfN[d];
+        }
+        S.V = p.indDeltaX;
+        S.dx = 0.0f;
+        J.tipR = p.indTipR;
+        J.indKs = p.indKs;
+        J.tipZeta = p.indTipZeta;
+        J.moveSurface = p.indMoveSurface;
+        J.fixTrans = p.indFixTrans;
+        J.retraction = p.indRetractionStep;
+        J.tipAp = 12.0f * p.indTipEl * powf(p.indTipSigma, 12.0f) * p.indTipA;
+        J.tipBp = 6.0f * p.indTipEl * powf(p.indTipSigma, 6.0f) * p.indTipB;
+        J.surfAp = 12.0f * p.indSurfEl * powf(p.indSurfSigma, 12.0f) * p.indSurfA;
+        J.surfBp = 6.0f * p.indSurfEl * powf(p.indSurfSigma, 6.0f) * p.indSurfB;
+        float norm = sqrtf(J.dir[0] * J.dir[0] + J.dir[1] * J.dir[1] + J.dir[2] * J.dir[2]);
+        for (int d = 0; d < 3; ++d) J.dir[d] /= norm;
+        norm = sqrtf(J.micaN[0] * J.micaN[0] + J.micaN[1] * J.micaN[1] + J.micaN[2] * J.micaN[2]);
+        for (int d = 0; d < 3; ++d) J.micaN[d] /= norm;
+    }
+    // stride = min(nav, every created updater's frequency) (gsop.cu:116-122)
+    int st = p.nav;
+    auto mn = [&st](int f) { st = std::min(st, f); };
+    if (p.indentation) mn(p.nav);
+    if (J.flags & JF_PULL_UPDATER) mn(p.pullFreq);
+    mn(p.possiblepairs_freq);
+    mn(p.outputtiming);
+    mn(p.pairs_freq);
+    mn(p.dcdfreq);
+    if (p.heating) mn(p.tempFreq);
+    J.stride = st;
+    if (numsteps < 0) return fail(SOP_E_ARG, "numsteps must be >= 0");
+    // the outer loop runs i = 0 .. numsteps/stride inclusive (gsop.cu:129)
+    b.end_step = (numsteps / st + 1) * (long long)st;
+    J.end_step = b.end_step;
+    S.step = 0;
+    return 0;
+}
+
+template <typename T>
+static size_t nbytes(const std::vector<T>& v) {
+    return v.size() * sizeof(T);
+}
+
+static unsigned long long round_up(unsigned long long x) { return (x + kAlign - 1) / kAlign * kAlign; }
+
+static unsigned long long built_bytes(const BuiltJob& b, bool pulling, bool indentation) {
+    const unsigned long long n = b.n;
+    unsigned long long t = 0;
+    t += 2 * round_up(n * 16);            // coord[2]
+    t += round_up(n * 16);                // rng
+    t += round_up(n * 4);                 // T
+    if (pulling) t += round_up(n * 16);   // ext
+    if (indentation) t += round_up(n * 16);
+    t += round_up(n * 4) * 5;             // five count arrays
+    t += round_up(n * b.cov_cap * 4) * 2; // cov j, r0
+    t += round_up(n * b.nat_cap * 4) + round_up(n * b.nat_cap * 8);
+    t += round_up(n * b.pair_cap * 4) + round_up(n * b.pp_cap * 4) + round_up(n * b.excl_cap * 4);
+    t += round_up(std::max<size_t>(1, b.pulled.size()) * 4);
+    t += round_up(n * 12);                // energies
+    return t;
+}
+
+int Engine::job_bytes(const sop_params& p, const sop_system& s, unsigned long long* bytes) {
+    BuiltJob b;
+    int rc = build_job(p, s, 0, b);
+    if (rc) return rc;
+    *bytes = built_bytes(b, p.pulling != 0, p.indentation != 0);
+    return 0;
+}
+
+int Engine::add_job(const sop_params& p, const sop_system& s, long long numsteps, int* id) {
+    if (hipSetDevice(device_) != hipSuccess) return fail(SOP_E_HIP, "hipSetDevice failed");
+    BuiltJob b;
+    int rc = build_job(p, s, numsteps, b);
+    if (rc) return rc;
+    HostJob hj;
+    hj.n = b.n;
+    hj.bytes = built_bytes(b, p.pulling != 0, p.indentation != 0);
+    bool oom = false;
+    auto A = [&](size_t bytes) -> void* {
+        void* q = arena_.alloc(bytes);
+        if (!q) oom = true;
+        else hj.blocks.push_back(q);
+        return q;
+    };
+    DevJob& J = b.dj;
+    const size_t n = b.n;
+    J.coord[0] = (float4*)A(n * 16);
+    J.coord[1] = (float4*)A(n * 16);
+    J.rng = (uint4*)A(n * 16);
+    J.T = (float*)A(n * 4);
+    J.ext = p.pulling ? (float4*)A(n * 16) : nullptr;
+    J.tipF = p.indentation ? (float4*)A(n * 16) : nullptr;
+    J.fextra = nullptr;
+    J.cov_cnt = (int*)A(n * 4);
+    J.cov_j = (int*)A(n * b.cov_cap * 4);
+    J.cov_r0 = (float*)A(n * b.cov_cap * 4);
+    J.nat_cnt = (int*)A(n * 4);
+    J.nat_j = (int*)A(n * b.nat_cap * 4);
+    J.nat_par = (float2*)A(n * b.nat_cap * 8);
+    J.pair_cnt = (int*)A(n * 4);
+    J.pair_j = (int*)A(n * b.pair_cap * 4);
+    J.pp_cnt = (int*)A(n * 4);
+    J.pp_j = (int*)A(n * b.pp_cap * 4);
+    J.excl_cnt = (int*)A(n * 4);
+    J.excl_j = (int*)A(n * b.excl_cap * 4);
+    J.pulled = (int*)A(std::max<size_t>(1, b.pulled.size()) * 4);
+    J.npulled = (int)b.pulled.size();
+    J.energies = (float*)A(n * 12);
+    if (oom) {
+        for (void* q : hj.blocks) arena_.release(q);
+        return fail(SOP_E_MEM, "HBM budget exhausted admitting a job of " + std::to_string(b.n) + " beads");
+    }
+    hipStream_t s0 = stream_;
+    int err = 0;
+    auto H2D = [&](void* dst, const void* src, size_t bytes) {
+        if (!err && bytes) err = check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s0), "upload");
+    };
+    H2D(J.coord[0], b.coord.data(), nbytes(b.coord));
+    H2D(J.coord[1], b.coord.data(), nbytes(b.coord));
+    H2D(J.rng, b.rng.data(), nbytes(b.rng));
+    std::vector<float> zeros(n * 4, 0.0f);
+    H2D(J.T, zeros.data(), n * 4);
+    if (J.ext) H2D(J.ext, b.ext.data(), nbytes(b.ext));
+    if (J.tipF) H2D(J.tipF, zeros.data(), n * 16);
+    H2D(J.cov_cnt, b.cov_cnt.data(), nbytes(b.cov_cnt));
+    H2D(J.cov_j, b.cov_j.data(), nbytes(b.cov_j));
+    H2D(J.cov_r0, b.cov_r0.data(), nbytes(b.cov_r0));
+    H2D(J.nat_cnt, b.nat_cnt.data(), nbytes(b.nat_cnt));
+    H2D(J.nat_j, b.nat_j.data(), nbytes(b.nat_j));
+    H2D(J.nat_par, b.nat_par.data(), nbytes(b.nat_par));
+    H2D(J.pair_cnt, zeros.data(), n * 4);
+    H2D(J.pp_cnt, b.pp_cnt.data(), nbytes(b.pp_cnt));
+    H2D(J.pp_j, b.pp_j.data(), nbytes(b.pp_j));
+    H2D(J.excl_cnt, b.excl_cnt.data(), nbytes(b.excl_cnt));
+    H2D(J.excl_j, b.excl_j.data(), nbytes(b.excl_j));
+    if (!b.pulled.empty()) H2D(J.pulled, b.pulled.data(), nbytes(b.pulled));
+    if (!err) err = check(hipStreamSynchronize(s0), "upload sync");
+    if (err) {
+        for (void* q : hj.blocks) arena_.release(q);
+        return err;
+    }
+    hj.id = next_id_++;
+    hj.dj = J;
+    hj.st0 = b.st;
+    hj.step = 0;
+    hj.end_step = b.end_step;
+    jobs_.push_back(hj);
+    tables_dirty_ = true;
+    drop_graph();
+    // pre-loop Verlet build of runGPU (gsop.cu:126): from the [ pairs ]
+    // section, at step 0
+    rc = sync_tables();
+    if (!rc) rc = rebuild_lists(hj.id, 2);
+    if (rc) {
+        remove_job(hj.id);
+        return rc;
+    }
+    *id = hj.id;
+    return 0;
+}
+
+int Engine::remove_job(int id) {
+    int slot = slot_of(id);
+    if (slot < 0) return fail(SOP_E_ARG, "no such job");
+    if (int rc = pull_states()) return rc;
+    for (void* b : jobs_[slot].blocks) arena_.release(b);
+    jobs_.erase(jobs_.begin() + slot);
+    tables_dirty_ = true;
+    drop_graph();
+    return 0;
+}
+
+// Copies the device state of the jobs that were on the device at the last
+// table upload back into their host records, so a table rebuild (which
+// moves jobs between slots) carries it over.
+int Engine::pull_states() {
+    if (synced_jobs_ <= 0 || !d_state_) return 0;
+    if (int rc = synchronize()) return rc;
+    std::vector<DevJobState> st(synced_jobs_);
+    if (int rc = check(hipMemcpy(st.data(), d_state_, st.size() * sizeof(DevJobState), hipMemcpyDeviceToHost),
+                       "state read"))
+        return rc;
+    for (int i = 0; i < synced_jobs_ && i < (int)jobs_.size(); ++i) jobs_[i].st0 = st[i];
+    synced_jobs_ = 0;
+    return 0;
+}
+
+// Uploads the job and tile tables when the job set changed.  Device state
+// of surviving jobs is carried over through st0 (remove_job refreshed it).
+int Engine::sync_tables() {
+    if (!tables_dirty_) return 0;
+    if (int rc = pull_states()) return rc;
+    const int nj = (int)jobs_.size();
+    if (nj > cap_jobs_) {
+        int cap = std::max(16, nj * 2);
+        (void)hipFree(d_jobs_);
+        (void)hipFree(d_state_);
+        d_jobs_ = nullptr;
+        d_state_ = nullptr;
+        if (int rc = check(hipMalloc(&d_jobs_, cap * sizeof(DevJob)), "job table")) return rc;
+        if (int rc = check(hipMalloc(&d_state_, cap * sizeof(DevJobState)), "state table")) return rc;
+        cap_jobs_ = cap;
+    }
+    std::vector<DevJob> dj(nj);
+    std::vector<DevJobState> ds(nj);
+    std::vector<int2> tiles;
+    int g = 0;
+    for (int k = 0; k < nj; ++k) {
+        dj[k] = jobs_[k].dj;
+        ds[k] = jobs_[k].st0;
+        ds[k].step = jobs_[k].step;
+        for (int t = 0; t < jobs_[k].n; t += 64) tiles.push_back(make_int2(k, t));
+        if (k == 0 || jobs_[k].dj.stride) g = std::gcd(g, jobs_[k].dj.stride);
+    }
+    L_ = g > 0 ? g : 1;
+    ntiles_ = (int)tiles.size();
+    if (ntiles_ > cap_tiles_) {
+        int cap = std::max(256, ntiles_ * 2);
+        (void)hipFree(d_tiles_);
+        d_tiles_ = nullptr;
+        if (int rc = check(hipMalloc(&d_tiles_, cap * sizeof(int2)), "tile table")) return rc;
+        cap_tiles_ = cap;
+    }
+    if (nj) {
+        if (int rc = check(hipMemcpyAsync(d_jobs_, dj.data(), nj * sizeof(DevJob), hipMemcpyHostToDevice, stream_),
+                           "job table upload"))
+            return rc;
+        if (int rc = check(hipMemcpyAsync(d_state_, ds.data(), nj * sizeof(DevJobState), hipMemcpyHostToDevice, stream_),
+                           "state upload"))
+            return rc;
+    }
+    if (ntiles_) {
+        if (int rc = check(hipMemcpyAsync(d_tiles_, tiles.data(), ntiles_ * sizeof(int2), hipMemcpyHostToDevice, stream_),
+                           "tile upload"))
+            return rc;
+    }
+    if (int rc = check(hipStreamSynchronize(stream_), "table sync")) return rc;
+    tables_dirty_ = false;
+    synced_jobs_ = nj;
+    return 0;
+}
+
+void Engine::drop_graph() {
+    if (graph_) {
+        (void)hipGraphExecDestroy(graph_);
+        graph_ = nullptr;
+    }
+}
+
+void Engine::set_tuning(int max_blocks, int use_graph) {
+    max_blocks_ = max_blocks < 0 ? 0 : max_blocks;
+    use_graph_ = use_graph != 0;
+    drop_graph();
+}
+
+void Engine::profile(bool on) {
+    if (!on) flush_profile();
+    profiling_ = on;
+    drop_graph();
+}
+
+void Engine::flush_profile() {
+    if (ev_used_ == 0) return;
+    (void)hipStreamSynchronize(stream_);
+    for (size_t k = 0; k < ev_used_; ++k) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, ev_pool_[k].first, ev_pool_[k].second) == hipSuccess) {
+            prof_ms_ += ms;
+            prof_launches_++;
+        }
+    }
+    ev_used_ = 0;
+}
+
+void Engine::profile_read(long long* launches, double* ms) {
+    flush_profile();
+    *launches = prof_launches_;
+    *ms = prof_ms_;
+    prof_launches_ = 0;
+    prof_ms_ = 0.0;
+}
+
+// One block: per-job updaters, list rebuilds, L steps, step advance.
+// slot_only >= 0 runs that job alone with its own stride (plain mode).
+int Engine::launch_block(int slot_only) {
+    const DevJob* jobs = d_jobs_;
+    DevJobState* st = d_state_;
+    const int2* tiles = d_tiles_;
+    int nj = (int)jobs_.size(), nt = ntiles_, L = L_;
+    if (slot_only >= 0) {
+        if (int rc = tiles_for_slot(slot_only, &nt)) return rc;
+        jobs = d_jobs_ + slot_only;
+        st = d_state_ + slot_only;
+        tiles = d_tiles1_;
+        nj = 1;
+        L = jobs_[slot_only].dj.stride;
+    }
+    if (int rc = check(launch_job_update(jobs, st, nj, stream_), "job update")) return rc;
+    if (int rc = check(launch_possible(jobs, st, tiles, nt, 0, stream_), "possible pairs")) return rc;
+    if (int rc = check(launch_verlet(jobs, st, tiles, nt, 0, stream_), "verlet")) return rc;
+    for (int k = 0; k < L; ++k) {
+        const bool prof = profiling_;
+        if (prof) {
+            if (ev_used_ == ev_pool_.size()) {
+                if (ev_pool_.size() >= 4096) flush_profile();
+                if (ev_used_ == ev_pool_.size()) {
+                    hipEvent_t a, b;
+                    if (int rc = check(hipEventCreate(&a), "event")) return rc;
+                    if (int rc = check(hipEventCreate(&b), "event")) return rc;
+                    ev_pool_.push_back({a, b});
+                }
+            }
+            (void)hipEventRecord(ev_pool_[ev_used_].first, stream_);
+        }
+        if (int rc = check(launch_step(jobs, st, tiles, nt, k, max_blocks_, stream_), "step")) return rc;
+        if (prof) {
+            (void)hipEventRecord(ev_pool_[ev_used_].second, stream_);
+            ev_used_++;
+        }
+    }
+    if (int rc = check(launch_job_advance(jobs, st, nj, L, stream_), "advance")) return rc;
+    return 0;
+}
+
+// Tile list of one job (job index 0 relative to its slot), cached.
+int Engine::tiles_for_slot(int slot, int* nt) {
+    const int n = jobs_[slot].n;
+    const int need = (n + 63) / 64;
+    *nt = need;
+    if (tiles1_n_ == n) return 0;
+    if (need > cap_tiles1_) {
+        (void)hipFree(d_tiles1_);
+        d_tiles1_ = nullptr;
+        cap_tiles1_ = 0;
+        if (int rc = check(hipMalloc(&d_tiles1_, need * sizeof(int2)), "tile table")) return rc;
+        cap_tiles1_ = need;
+    }
+    std::vector<int2> t1(need);
+    for (int t = 0; t < need; ++t) t1[t] = make_int2(0, 64 * t);
+    if (int rc = synchronize()) return rc;
+    if (int rc = check(hipMemcpy(d_tiles1_, t1.data(), need * sizeof(int2), hipMemcpyHostToDevice), "tile upload"))
+        return rc;
+    tiles1_n_ = n;
+    return 0;
+}
+
+int Engine::capture_graph() {
+    drop_graph();
+    hipGraph_t g = nullptr;
+    if (int rc = check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture")) return rc;
+    int rc = launch_block(-1);
+    hipError_t e = hipStreamEndCapture(stream_, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (int r2 = check(e, "end capture")) return r2;
+    e = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        graph_ = nullptr;
+        return check(e, "graph instantiate");
+    }
+    return 0;
+}
+
+int Engine::run_blocks(int nblocks) {
+    if (hipSetDevice(device_) != hipSuccess) return fail(SOP_E_HIP, "hipSetDevice failed");
+    if (int rc = sync_tables()) return rc;
+    if (jobs_.empty()) return 0;
+    const bool graph = use_graph_ && !profiling_;
+    if (graph && !graph_) {
+        if (int rc = capture_graph()) return rc;
+    }
+    for (int b = 0; b < nblocks; ++b) {
+        if (graph) {
+            if (int rc = check(hipGraphLaunch(graph_, stream_), "graph launch")) return rc;
+        } else {
+            if (int rc = launch_block(-1)) return rc;
+        }
+        for (auto& j : jobs_) j.step = std::min(j.step + L_, j.end_step);
+    }
+    return 0;
+}
+
+int Engine::run_job(int id) {
+    if (hipSetDevice(device_) != hipSuccess) return fail(SOP_E_HIP, "hipSetDevice failed");
+    if (int rc = sync_tables()) return rc;
+    int slot = slot_of(id);
+    if (slot < 0) return fail(SOP_E_ARG, "no such job");
+    HostJob& hj = jobs_[slot];
+    while (hj.step < hj.end_step) {
+        if (int rc = launch_block(slot)) return rc;
+        hj.step = std::min(hj.step + hj.dj.stride, hj.end_step);
+    }
+    return 0;
+}
+
+int Engine::synchronize() {
+    if (hipSetDevice(device_) != hipSuccess) return fail(SOP_E_HIP, "hipSetDevice failed");
+    return check(hipStreamSynchronize(stream_), "synchronize");
+}
+
+int Engine::read_state(int slot, DevJobState* out) {
+    if (int rc = sync_tables()) return rc;
+    if (int rc = synchronize()) return rc;
+    return check(hipMemcpy(out, d_state_ + slot, sizeof(DevJobState), hipMemcpyDeviceToHost), "state read");
+}
+
+int Engine::job_status(int id, long long* step, long long* end, int* overflow) {
+    int slot = slot_of(id);
+    if (slot < 0) return fail(SOP_E_ARG, "no such job");
+    DevJobState s;
+    if (int rc = read_state(slot, &s)) return rc;
+    if (step) *step = s.step;
+    if (end) *end = jobs_[slot].end_step;
+    if (overflow) *overflow = s.overflow;
+    return 0;
+}
+
+int Engine::get_indentation(int id, float* out) {
+    int slot = slot_of(id);
+    if (slot < 0) return fail(SOP_E_ARG, "no such job");
+    DevJobState s;
+    if (int rc = read_state(slot, &s)) return rc;
+    for (int d = 0; d < 3; ++d) {
+        out[d] = s.tip[d];
+        out[3 + d] = s.chip[d];
+        out[6 + d] = s.micaR[d];
+    }
+    for (int d = 0; d < 4; ++d) out[9 + d] = s.tipFav[d];
+    return 0;
+}
+
+int Engine::get_coords(int id, float* out4) {
+    int slot = slot_of(id);
+    if (slot < 0) return fail(SOP_E_ARG, "no such job");
+    DevJobState s;
+    if (int rc = read_state(slot, &s)) return rc;
+    const HostJob& hj = jobs_[slot];
+    return check(hipMemcpy(out4, hj.dj.coord[s.step & 1], (size_t)hj.n * 16, hipMemcpyDeviceToHost), "coords read");
+}
+
+int Engine::set_coords(int id, const float* in4) {
+    int slot = slot_of(id);
+    if (slot < 0) return fail(SOP_E_ARG, "no such job");
+    DevJobState s;
+    if (int rc = read_state(slot, &s)) return rc;
+    const HostJob& hj = jobs_[slot];
+    return check(hipMemcpy(hj.dj.coord[s.step & 1], in4, (size_t)hj.n * 16, hipMemcpyHostToDevice), "coords write");
+}
+
+int Engine::get_array(int id, int what, void* out) {
+    int slot = slot_of(id);
+    if (slot < 0) return fail(SOP_E_ARG, "no such job");
+    if (int rc = synchronize()) return rc;
+    const HostJob& hj = jobs_[slot];
+    const size_t n = hj.n;
+    switch (what) {
+    case 0: return check(hipMemcpy(out, hj.dj.T, n * 4, hipMemcpyDeviceToHost), "T read");
+    case 1: return check(hipMemcpy(out, hj.dj.rng, n * 16, hipMemcpyDeviceToHost), "rng read");
+    case 2:
+        if (!hj.dj.ext) {
+            std::memset(out, 0, n * 16);
+            return 0;
+        }
+        return check(hipMemcpy(out, hj.dj.ext, n * 16, hipMemcpyDeviceToHost), "ext read");
+    default: return fail(SOP_E_ARG, "unknown array");
+    }
+}
+
+int Engine::list_cap(int id, int which, int* cap) {
+    HostJob* hj = find(id);
+    if (!hj) return fail(SOP_E_ARG, "no such job");
+    switch (which) {
+    case 0: *cap = hj->dj.cov_cap; break;
+    case 1: *cap = hj->dj.nat_cap; break;
+    case 2: *cap = hj->dj.pair_cap; break;
+    case 3: *cap = hj->dj.pp_cap; break;
+    default: return fail(SOP_E_ARG, "unknown list");
+    }
+    return 0;
+}
+
+int Engine::get_list(int id, int which, int* counts, int* entries) {
+    HostJob* hj = find(id);
+    if (!hj) return fail(SOP_E_ARG, "no such job");
+    if (int rc = synchronize()) return rc;
+    const DevJob& J = hj->dj;
+    const size_t n = hj->n;
+    const int* c;
+    const int* e;
+    int cap;
+    switch (which) {
+    case 0: c = J.cov_cnt; e = J.cov_j; cap = J.cov_cap; break;
+    case 1: c = J.nat_cnt; e = J.nat_j; cap = J.nat_cap; break;
+    case 2: c = J.pair_cnt; e = J.pair_j; cap = J.pair_cap; break;
+    case 3: c = J.pp_cnt; e = J.pp_j; cap = J.pp_cap; break;
+    default: return fail(SOP_E_ARG, "unknown list");
+    }
+    if (int rc = check(hipMemcpy(counts, c, n * 4, hipMemcpyDeviceToHost), "list counts")) return rc;
+    return check(hipMemcpy(entries, e, n * cap * 4, hipMemcpyDeviceToHost), "list entries");
+}
+
+int Engine::compute_forces(int id, float* out4) {
+    int slot = slot_of(id);
+    if (slot < 0) return fail(SOP_E_ARG, "no such job");
+    if (int rc = sync_tables()) return rc;
+    const HostJob& hj = jobs_[slot];
+    if ((size_t)hj.n > scratch4_n_) {
+        (void)hipFree(d_scratch4_);
+        d_scratch4_ = nullptr;
+        if (int rc = check(hipMalloc(&d_scratch4_, (size_t)hj.n * 16), "scratch")) return rc;
+        scratch4_n_ = hj.n;
+    }
+    if (int rc = check(launch_forces(d_jobs_, d_state_, slot, hj.n, d_scratch4_, stream_), "forces")) return rc;
+    if (int rc = synchronize()) return rc;
+    return check(hipMemcpy(out4, d_scratch4_, (size_t)hj.n * 16, hipMemcpyDeviceToHost), "forces read");
+}
+
+// SOPPotential::computeEnergy + sumEnergies (gsop.cu:255-271): per-bead
+// energies, summed in double and halved because every pair is stored on
+// both beads.
+int Engine::compute_energies(int id, float* per_bead, float* sums3) {
+    int slot = slot_of(id);
+    if (slot < 0) return fail(SOP_E_ARG, "no such job");
+    if (int rc = sync_tables()) return rc;
+    const HostJob& hj = jobs_[slot];
+    const size_t n = hj.n;
+    if (int rc = check(launch_energies(d_jobs_, d_state_, slot, hj.n, stream_), "energies")) return rc;
+    if (int rc = synchronize()) return rc;
+    std::vector<float> e(3 * n);
+    if (int rc = check(hipMemcpy(e.data(), hj.dj.energies, 3 * n * 4, hipMemcpyDeviceToHost), "energies read"))
+        return rc;
+    if (per_bead) std::memcpy(per_bead, e.data(), 3 * n * 4);
+    if (sums3) {
+        for (int p = 0; p < 3; ++p) {
+            double acc = 0.0;
+            for (size_t i = 0; i < n; ++i) acc += e[p * n + i];
+            sums3[p] = (float)(acc * 0.5f);
+        }
+    }
+    return 0;
+}
+
+int Engine::rebuild_lists(int id, int mask) {
+    int slot = slot_of(id);
+    if (slot < 0) return fail(SOP_E_ARG, "no such job");
+    if (int rc = sync_tables()) return rc;
+    int nt = 0;
+    if (int rc = tiles_for_slot(slot, &nt)) return rc;
+    if (mask & 1)
+        if (int rc = check(launch_possible(d_jobs_ + slot, d_state_ + slot, d_tiles1_, nt, 1, stream_),
+                           "possible pairs"))
+            return rc;
+    if (mask & 2)
+        if (int rc = check(launch_verlet(d_jobs_ + slot, d_state_ + slot, d_tiles1_, nt, 1, stream_),
+                           "verlet"))
+            return rc;
+    return synchronize();
+}
+
+int Engine::list_stats(double* avg4, long long* beads) {
+    if (int rc = synchronize()) return rc;
+    double sum[4] = {0, 0, 0, 0};
+    long long nb = 0;
+    for (auto& hj : jobs_) {
+        const int* cnts[4] = {hj.dj.cov_cnt, hj.dj.nat_cnt, hj.dj.pair_cnt, hj.dj.pp_cnt};
+        std::vector<int> c(hj.n);
+        for (int w = 0; w < 4; ++w) {
+            if (int rc = check(hipMemcpy(c.data(), cnts[w], hj.n * 4, hipMemcpyDeviceToHost), "counts")) return rc;
+            for (int v : c) sum[w] += v;
+        }
+        nb += hj.n;
+    }
+    for (int w = 0; w < 4; ++w) avg4[w] = nb ? sum[w] / nb : 0.0;
+    *beads = nb;
+    return 0;
+}
+
+// Times the step kernel over real blocks (results do not depend on the
+// launch shape, so tuning runs are ordinary progress) for a ladder of
+// persistent grid sizes and keeps the fastest.
+int Engine::autotune(int* best_out) {
+    if (int rc = sync_tables()) return rc;
+    if (jobs_.empty()) {
+        *best_out = max_blocks_;
+        return 0;
+    }
+    const int full = (ntiles_ + 3) / 4;
+    std::vector<int> cand = {0};
+    for (int b : {256, 512, 1024, 2048, 4096, 8192})
+        if (b < full) cand.push_back(b);
+    const bool saved_prof = profiling_;
+    int best = 0;
+    double best_ms = 1e30;
+    for (int c : cand) {
+        max_blocks_ = c;
+        profiling_ = true;
+        long long nl;
+        double ms;
+        profile_read(&nl, &ms);
+        bool done = true;
+        for (auto& j : jobs_)
+            if (j.step < j.end_step) done = false;
+        if (done) break;
+        if (int rc = run_blocks(2)) return rc;
+        profile_read(&nl, &ms);
+        if (nl > 0 && ms / nl < best_ms) {
+            best_ms = ms / nl;
+            best = c;
+        }
+    }
+    profiling_ = saved_prof;
+    max_blocks_ = best;
+    drop_graph();
+    *best_out = best;
+    return 0;
+}
+
+}  // namespace sop

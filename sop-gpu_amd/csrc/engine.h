@@ -1,0 +1,145 @@
+// engine.h -- the device runtime: an HBM arena, a table of jobs, and the
+// block loop that advances them.
+//
+// One Engine serves both ways of running:
+//   * plain   -- run_job(): one job alone, kernels launched one by one in
+//                runGPU's order (src/gsop.cu:111-163), the baseline;
+//   * batched -- run_blocks(): every admitted job advances together, one
+//                step kernel per step for all of them, replayed from a
+//                hipGraph.
+// Both launch the same kernels on the same per-job arrays, so a job's
+// trajectory does not depend on which way it ran or what ran beside it.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/sopgpu.h"
+#include "jobdesc.h"
+
+namespace sop {
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+// First-fit sub-allocator over device chunks, bounded by a byte budget.
+class Arena {
+   public:
+    explicit Arena(size_t budget) : budget_(budget) {}
+    ~Arena();
+    void* alloc(size_t bytes);  // nullptr when the budget is exhausted
+    void release(void* p);
+    size_t reserved() const { return reserved_; }
+    size_t in_use() const { return in_use_; }
+
+   private:
+    struct Chunk {
+        char* base;
+        size_t size;
+        std::map<size_t, size_t> free;  // offset -> length
+    };
+    std::vector<Chunk> chunks_;
+    std::map<void*, std::pair<int, size_t>> live_;  // ptr -> (chunk, length)
+    size_t budget_, reserved_ = 0, in_use_ = 0;
+};
+
+struct HostJob {
+    int id = -1;
+    int n = 0;
+    DevJob dj{};
+    DevJobState st0{};
+    long long step = 0;      // host mirror of the device step counter
+    long long end_step = 0;
+    std::vector<void*> blocks;
+    unsigned long long bytes = 0;
+};
+
+class Engine {
+   public:
+    Engine(int device, size_t budget);
+    ~Engine();
+
+    int add_job(const sop_params& p, const sop_system& s, long long numsteps, int* id);
+    int remove_job(int id);
+    static int job_bytes(const sop_params& p, const sop_system& s, unsigned long long* bytes);
+
+    int run_blocks(int nblocks);
+    int run_job(int id);
+    int synchronize();
+    int block_steps() const { return L_; }
+
+    HostJob* find(int id);
+    int slot_of(int id) const;
+    const std::vector<HostJob>& jobs() const { return jobs_; }
+
+    // read-back / hooks
+    int get_coords(int id, float* out4);
+    int set_coords(int id, const float* in4);
+    int get_array(int id, int what, void* out);
+    int list_cap(int id, int which, int* cap);
+    int get_list(int id, int which, int* counts, int* entries);
+    int compute_forces(int id, float* out4);
+    int compute_energies(int id, float* per_bead, float* sums3);
+    int rebuild_lists(int id, int mask);
+    int job_status(int id, long long* step, long long* end, int* overflow);
+    int get_indentation(int id, float* out13);
+    int list_stats(double* avg4, long long* beads);
+
+    void set_tuning(int max_blocks, int use_graph);
+    int autotune(int* best);
+    void profile(bool on);
+    void profile_read(long long* launches, double* ms);
+
+    hipStream_t stream() const { return stream_; }
+    int device() const { return device_; }
+
+   private:
+    int sync_tables();
+    int launch_block(int slot_only);  // slot_only < 0: all jobs
+    int capture_graph();
+    void drop_graph();
+    int check(hipError_t e, const char* what);
+    int read_state(int slot, DevJobState* out);
+    int pull_states();
+    int synced_jobs_ = 0;
+
+    int device_;
+    hipStream_t stream_ = nullptr;
+    Arena arena_;
+    std::vector<HostJob> jobs_;
+    int next_id_ = 1;
+    // device tables
+    DevJob* d_jobs_ = nullptr;
+    DevJobState* d_state_ = nullptr;
+    int2* d_tiles_ = nullptr;
+    int cap_jobs_ = 0, cap_tiles_ = 0, ntiles_ = 0;
+    bool tables_dirty_ = true;
+    int L_ = 1;
+    // single-job tables for plain runs
+    int2* d_tiles1_ = nullptr;
+    int cap_tiles1_ = 0;
+    int tiles1_n_ = -1;  // bead count the single-job tile list was built for
+    int tiles_for_slot(int slot, int* nt);
+    // tuning
+    int max_blocks_ = 0;
+    bool use_graph_ = true;
+    hipGraphExec_t graph_ = nullptr;
+    // profiling
+    bool profiling_ = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool_;
+    size_t ev_used_ = 0;
+    long long prof_launches_ = 0;
+    double prof_ms_ = 0.0;
+    void flush_profile();
+    float4* d_scratch4_ = nullptr;
+    size_t scratch4_n_ = 0;
+};
+
+void set_error(int code, const std::string& msg);
+int fail(int code, const std::string& msg);
+
+}  // namespace sop

@@ -1,0 +1,87 @@
+// jobdesc.h -- device-side description of one simulated system ("job").
+//
+// Every job owns its arrays inside the engine's HBM arena.  Neighbour
+// lists are slot-major as in the reference (entry k of bead i at
+// [k*n + i], src/Potentials/covalent.cu:85-96) so that one lane per bead
+// reads consecutive words, and they hold job-local bead indices.
+// Coordinates are double-buffered: the coordinates at local step s live in
+// coord[s & 1]; step s reads that buffer and writes the other, so one
+// kernel per step needs no grid-wide barrier.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sop {
+
+enum JobFlags : int {
+    JF_MINIMIZATION = 1,
+    JF_PULLING = 2,      // extForces in use (fixed / pulled beads)
+    JF_PULL_UPDATER = 4, // force-ramp cantilever updated every stride
+    JF_INDENTATION = 8,
+    JF_HEATING = 16,
+    JF_EXTRA_FORCE = 32, // forces accumulated by non-built-in potentials
+};
+
+struct DevJob {
+    int n;
+    int flags;
+    // arrays (device pointers into the arena)
+    float4* coord[2];
+    uint4* rng;
+    float* T;
+    float4* ext;    // x,y,z force, w: 0 free, 1 fixed, 2 pulled
+    float4* tipF;   // per-bead reaction on the indentation tip
+    float4* fextra; // optional extra forces (custom potentials)
+    int* cov_cnt;
+    int* cov_j;
+    float* cov_r0;
+    int* nat_cnt;
+    int* nat_j;
+    float2* nat_par; // (r0^2, -12 eh / r0^2)
+    int* pair_cnt;
+    int* pair_j;
+    int* pp_cnt;
+    int* pp_j;
+    int* excl_cnt;  // sorted covalent+native partners (possible-pair exclusions)
+    int* excl_j;
+    int* pulled;    // bead indices of pulled beads
+    float* energies; // 3n per-bead energies (output cadence)
+    int cov_cap, nat_cap, pair_cap, pp_cap, excl_cap;
+    int npulled;
+    // force-field constants, formed as the reference's constructors do
+    float ks, R2;                     // FENE: kspring_cov, R_limit^2
+    float a2, m6, rc2, el;            // repulsive LJ: a^2, -6 el/a^2, cutoff^2
+    float pl_cut, pp_thr;             // Verlet cutoff, possible-pairs threshold
+    float hOverZeta, tempNorm;
+    float h, zeta;
+    // cadence
+    int stride, nav, pairs_freq, pp_freq;
+    long long end_step;               // the job stops after this many steps
+    // heating
+    int tempFreq;
+    float initialT, deltaT;
+    // pulling (force-ramp)
+    int pulledEnd, pullFreq;
+    float k_trans, deltax;
+    float pullVec[3], cant0[3];
+    // indentation (continuous surface)
+    float tipR, tipAp, tipBp, surfAp, surfBp;
+    float indKs, tipZeta;
+    float dir[3], chip0[3], micaR0[3], micaN[3];
+    int moveSurface, fixTrans;
+    long long retraction;
+};
+
+struct DevJobState {
+    long long step;   // local step at the start of the current block
+    float var;        // sqrt(2 T zeta / h)
+    int overflow;     // bit0 possible list, bit1 Verlet list
+    float cant[3];    // cantilever chip position (pulling)
+    float extF[3];
+    float tip[3], chip[3], micaR[3];
+    float V, dx;
+    float tipFav[4];  // last averaged tip force (x,y,z,|f|)
+    int pad;
+};
+
+}  // namespace sop

@@ -1,0 +1,27 @@
+// kernels.h -- host-side launchers for the kernels in kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "jobdesc.h"
+
+namespace sop {
+
+// One Langevin step (local step = state.step + kstep) of every job that owns
+// a tile.  max_blocks caps the persistent grid (0 = one block per 4 tiles).
+hipError_t launch_step(const DevJob* jobs, const DevJobState* st, const int2* tiles, int ntiles, int kstep,
+                       int max_blocks, hipStream_t stream);
+// List rebuilds for jobs whose cadence is due (force != 0: rebuild all).
+hipError_t launch_possible(const DevJob* jobs, DevJobState* st, const int2* tiles, int ntiles, int force,
+                           hipStream_t stream);
+hipError_t launch_verlet(const DevJob* jobs, DevJobState* st, const int2* tiles, int ntiles, int force,
+                         hipStream_t stream);
+// Per-job updaters at a block start, and the step counter advance.
+hipError_t launch_job_update(const DevJob* jobs, DevJobState* st, int njobs, hipStream_t stream);
+hipError_t launch_job_advance(const DevJob* jobs, DevJobState* st, int njobs, int nsteps, hipStream_t stream);
+// Output / parity hooks.
+hipError_t launch_energies(const DevJob* jobs, const DevJobState* st, int job, int n, hipStream_t stream);
+hipError_t launch_forces(const DevJob* jobs, const DevJobState* st, int job, int n, float4* out, hipStream_t stream);
+hipError_t launch_rng_probe(uint4* state, int n, int count, int gaussian, float* out, hipStream_t stream);
+const void* step_kernel_symbol();
+
+}  // namespace sop

@@ -1,0 +1,431 @@
+"""ctypes bindings to libsopgpu.so (include/sopgpu.h).
+
+This module is plumbing for tests, benchmarks and scripting: all compute
+happens in the HIP library.  Importing it loads the library and fails
+loudly when it is missing; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libsopgpu.so")
+
+
+class SopError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"libsopgpu.so not built at {LIB_PATH}; run __graft_entry__.build()")
+    return C.CDLL(LIB_PATH)
+
+
+lib = _load()
+
+F3 = C.c_float * 3
+
+PARAM_FIELDS = [
+    ("kspring_cov", C.c_float), ("R_limit", C.c_float),
+    ("R_limit_bond", C.c_float),
+    ("pairs_cutoff", C.c_float), ("el", C.c_float), ("a", C.c_float),
+    ("pairlist_cutoff", C.c_float), ("pairs_threshold", C.c_float),
+    ("pairs_freq", C.c_int), ("possiblepairs_freq", C.c_int),
+    ("max_covalent", C.c_int), ("max_native", C.c_int), ("max_pairs", C.c_int), ("max_possiblePairs", C.c_int),
+    ("timestep", C.c_float), ("zeta", C.c_float), ("temperature", C.c_float),
+    ("minimization", C.c_int),
+    ("seed", C.c_int),
+    ("heating", C.c_int), ("tempFreq", C.c_int),
+    ("initialT", C.c_float), ("deltaT", C.c_float),
+    ("nav", C.c_int), ("outputtiming", C.c_int), ("dcdfreq", C.c_int),
+    ("pulling", C.c_int),
+    ("deltax", C.c_float), ("k_trans", C.c_float), ("fconst", C.c_float),
+    ("pullFreq", C.c_int),
+    ("pullDirectionVector", C.c_int),
+    ("pullVector", F3),
+    ("fixedEnd", C.c_int), ("pulledEnd", C.c_int),
+    ("indentation", C.c_int),
+    ("indChip", F3), ("indTip", F3), ("indDirection", F3), ("indSurfR0", F3), ("indSurfN", F3),
+    ("indTipR", C.c_float), ("indKs", C.c_float), ("indDeltaX", C.c_float),
+    ("indMoveSurface", C.c_int), ("indFixTrans", C.c_int),
+    ("indTipSigma", C.c_float), ("indSurfSigma", C.c_float), ("indTipEl", C.c_float), ("indSurfEl", C.c_float),
+    ("indTipA", C.c_float), ("indTipB", C.c_float), ("indSurfA", C.c_float), ("indSurfB", C.c_float),
+    ("indTipZeta", C.c_float),
+    ("indOutputFreq", C.c_int),
+    ("indRetractionStep", C.c_longlong),
+]
+
+
+class Params(C.Structure):
+    _fields_ = PARAM_FIELDS
+
+
+# Defaults of the reference's PARAMETER() declarations (Appendix B of the
+# reference's headers); fields the reference marks mandatory get the values
+# of its example configs.
+DEFAULTS = dict(
+    kspring_cov=20.0, R_limit=2.0, R_limit_bond=8.0,
+    pairs_cutoff=20.0, el=1.0, a=3.8,
+    pairlist_cutoff=20.0, pairs_threshold=200.0,
+    pairs_freq=1000, possiblepairs_freq=100000,
+    max_covalent=8, max_native=128, max_pairs=512, max_possiblePairs=4096,
+    timestep=0.16, zeta=50.0, temperature=0.6, minimization=0, seed=1,
+    heating=0, tempFreq=1, initialT=0.0, deltaT=0.0,
+    nav=1000, outputtiming=10000, dcdfreq=10000,
+    pulling=0, deltax=0.0, k_trans=0.05, fconst=0.0, pullFreq=1000, pullDirectionVector=0,
+    pullVector=(1.0, 0.0, 0.0), fixedEnd=0, pulledEnd=0,
+    indentation=0, indChip=(0, 0, 0), indTip=(0, 0, 0), indDirection=(0, 0, 1), indSurfR0=(0, 0, 0),
+    indSurfN=(0, 0, 1), indTipR=50.0, indKs=0.07, indDeltaX=0.0, indMoveSurface=0, indFixTrans=0,
+    indTipSigma=1.0, indSurfSigma=1.0, indTipEl=1.0, indSurfEl=1.0,
+    indTipA=0.0, indTipB=1.0, indSurfA=0.0, indSurfB=1.0, indTipZeta=5000.0, indOutputFreq=1000,
+    indRetractionStep=-1,
+)
+
+
+def make_params(struct_cls=Params, **kw):
+    d = dict(DEFAULTS)
+    for k in kw:
+        if k not in d:
+            raise KeyError(f"unknown parameter {k}")
+    d.update(kw)
+    p = struct_cls()
+    for name, ty in struct_cls._fields_:
+        v = d[name]
+        if isinstance(v, (tuple, list, np.ndarray)):
+            getattr(p, name)[:] = [float(x) for x in v]
+        else:
+            setattr(p, name, v)
+    return p
+
+
+class SystemStruct(C.Structure):
+    _fields_ = [
+        ("n", C.c_int), ("coords4", C.c_void_p),
+        ("nbonds", C.c_int), ("bond_i", C.c_void_p), ("bond_j", C.c_void_p), ("bond_r0", C.c_void_p),
+        ("nnative", C.c_int), ("nat_i", C.c_void_p), ("nat_j", C.c_void_p), ("nat_r0", C.c_void_p),
+        ("nat_eh", C.c_void_p),
+        ("npairs", C.c_int), ("pair_i", C.c_void_p), ("pair_j", C.c_void_p),
+        ("nfixed", C.c_int), ("fixed", C.c_void_p), ("npulled", C.c_int), ("pulled", C.c_void_p),
+    ]
+
+
+class SynthStruct(C.Structure):
+    _fields_ = [
+        ("n", C.c_int), ("coords4", C.POINTER(C.c_float)),
+        ("nbonds", C.c_int), ("bond_i", C.POINTER(C.c_int)), ("bond_j", C.POINTER(C.c_int)),
+        ("bond_r0", C.POINTER(C.c_float)),
+        ("nnative", C.c_int), ("nat_i", C.POINTER(C.c_int)), ("nat_j", C.POINTER(C.c_int)),
+        ("nat_r0", C.POINTER(C.c_float)), ("nat_eh", C.POINTER(C.c_float)),
+    ]
+
+
+@dataclass
+class System:
+    """One SOP system in .top order (numpy arrays)."""
+
+    coords: np.ndarray                      # (n,4) float32
+    bonds: np.ndarray                       # (nb,2) int32
+    bond_r0: np.ndarray                     # (nb,) float32
+    natives: np.ndarray                     # (nn,2) int32
+    nat_r0: np.ndarray
+    nat_eh: np.ndarray
+    pairs: np.ndarray = field(default_factory=lambda: np.zeros((0, 2), np.int32))
+    fixed: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    pulled: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+
+    @property
+    def n(self) -> int:
+        return int(self.coords.shape[0])
+
+    def normalized(self) -> "System":
+        c = np.ascontiguousarray(self.coords, dtype=np.float32)
+        if c.shape[1] == 3:
+            c = np.ascontiguousarray(np.concatenate([c, np.zeros((c.shape[0], 1), np.float32)], axis=1))
+        return System(
+            c,
+            np.ascontiguousarray(self.bonds, dtype=np.int32).reshape(-1, 2),
+            np.ascontiguousarray(self.bond_r0, dtype=np.float32),
+            np.ascontiguousarray(self.natives, dtype=np.int32).reshape(-1, 2),
+            np.ascontiguousarray(self.nat_r0, dtype=np.float32),
+            np.ascontiguousarray(self.nat_eh, dtype=np.float32),
+            np.ascontiguousarray(self.pairs, dtype=np.int32).reshape(-1, 2),
+            np.ascontiguousarray(self.fixed, dtype=np.int32),
+            np.ascontiguousarray(self.pulled, dtype=np.int32),
+        )
+
+    def struct(self):
+        s = self.normalized()
+        keep = [s.coords, np.ascontiguousarray(s.bonds[:, 0]), np.ascontiguousarray(s.bonds[:, 1]), s.bond_r0,
+                np.ascontiguousarray(s.natives[:, 0]), np.ascontiguousarray(s.natives[:, 1]), s.nat_r0, s.nat_eh,
+                np.ascontiguousarray(s.pairs[:, 0]), np.ascontiguousarray(s.pairs[:, 1]), s.fixed, s.pulled]
+        p = lambda a: a.ctypes.data if a.size else None  # noqa: E731
+        st = SystemStruct(s.n, p(keep[0]), len(s.bond_r0), p(keep[1]), p(keep[2]), p(keep[3]),
+                          len(s.nat_r0), p(keep[4]), p(keep[5]), p(keep[6]), p(keep[7]),
+                          len(s.pairs), p(keep[8]), p(keep[9]), len(s.fixed), p(keep[10]), len(s.pulled), p(keep[11]))
+        return st, keep
+
+
+def _sig(name, res, *args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = list(args)
+    return f
+
+
+VP = C.c_void_p
+_sig("sop_last_error", C.c_char_p)
+_sig("sop_version", C.c_char_p)
+_sig("sop_engine_create", C.c_int, C.c_int, C.c_ulonglong, C.POINTER(VP))
+_sig("sop_engine_destroy", None, VP)
+_sig("sop_engine_add_job", C.c_int, VP, C.POINTER(Params), C.POINTER(SystemStruct), C.c_longlong, C.POINTER(C.c_int))
+_sig("sop_engine_remove_job", C.c_int, VP, C.c_int)
+_sig("sop_engine_job_bytes", C.c_int, C.POINTER(Params), C.POINTER(SystemStruct), C.POINTER(C.c_ulonglong))
+_sig("sop_engine_set_tuning", C.c_int, VP, C.c_int, C.c_int)
+_sig("sop_engine_autotune", C.c_int, VP, C.POINTER(C.c_int))
+_sig("sop_engine_run_blocks", C.c_int, VP, C.c_int)
+_sig("sop_engine_run_job", C.c_int, VP, C.c_int)
+_sig("sop_engine_block_steps", C.c_int, VP, C.POINTER(C.c_int))
+_sig("sop_engine_synchronize", C.c_int, VP)
+_sig("sop_engine_job_status", C.c_int, VP, C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+     C.POINTER(C.c_int))
+_sig("sop_engine_active_jobs", C.c_int, VP, C.POINTER(C.c_int), C.POINTER(C.c_int))
+_sig("sop_engine_get_coords", C.c_int, VP, C.c_int, VP)
+_sig("sop_engine_set_coords", C.c_int, VP, C.c_int, VP)
+_sig("sop_engine_get_temperature", C.c_int, VP, C.c_int, VP)
+_sig("sop_engine_get_rng", C.c_int, VP, C.c_int, VP)
+_sig("sop_engine_get_ext", C.c_int, VP, C.c_int, VP)
+_sig("sop_engine_list_cap", C.c_int, VP, C.c_int, C.c_int, C.POINTER(C.c_int))
+_sig("sop_engine_get_list", C.c_int, VP, C.c_int, C.c_int, VP, VP)
+_sig("sop_engine_get_indentation", C.c_int, VP, C.c_int, VP)
+_sig("sop_engine_compute_forces", C.c_int, VP, C.c_int, VP)
+_sig("sop_engine_compute_energies", C.c_int, VP, C.c_int, VP, VP)
+_sig("sop_engine_rebuild_lists", C.c_int, VP, C.c_int, C.c_int)
+_sig("sop_engine_profile", C.c_int, VP, C.c_int)
+_sig("sop_engine_profile_read", C.c_int, VP, C.POINTER(C.c_longlong), C.POINTER(C.c_double))
+_sig("sop_engine_list_stats", C.c_int, VP, VP, C.POINTER(C.c_longlong))
+_sig("sop_generate_seeds", C.c_int, C.c_int, C.c_int, VP)
+_sig("sop_rng_probe", C.c_int, C.c_int, VP, C.c_int, C.c_int, C.c_int, VP, VP)
+_sig("sop_synth_chain", C.c_int, C.c_int, C.c_ulonglong, C.c_float, C.c_float, C.c_float, C.POINTER(SynthStruct))
+_sig("sop_synth_shell", C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.c_float, C.c_float, C.c_float,
+     C.POINTER(SynthStruct))
+_sig("sop_free_system", None, C.POINTER(SynthStruct))
+
+EXPORTED = [
+    "sop_last_error", "sop_version", "sop_engine_create", "sop_engine_destroy", "sop_engine_add_job",
+    "sop_engine_remove_job", "sop_engine_job_bytes", "sop_engine_set_tuning", "sop_engine_autotune",
+    "sop_engine_run_blocks", "sop_engine_run_job", "sop_engine_block_steps", "sop_engine_synchronize",
+    "sop_engine_job_status", "sop_engine_active_jobs", "sop_engine_get_coords", "sop_engine_set_coords",
+    "sop_engine_get_temperature", "sop_engine_get_rng", "sop_engine_get_ext", "sop_engine_list_cap",
+    "sop_engine_get_list", "sop_engine_get_indentation", "sop_engine_compute_forces",
+    "sop_engine_compute_energies", "sop_engine_rebuild_lists", "sop_engine_profile", "sop_engine_profile_read",
+    "sop_engine_list_stats", "sop_generate_seeds", "sop_rng_probe", "sop_synth_chain", "sop_synth_shell",
+    "sop_free_system",
+]
+
+
+def check(rc: int):
+    if rc != 0:
+        raise SopError(rc, lib.sop_last_error().decode())
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data
+
+
+def _from_synth(s: SynthStruct) -> System:
+    n = s.n
+    coords = np.ctypeslib.as_array(s.coords4, shape=(n * 4,)).reshape(n, 4).copy()
+    nb, nn = s.nbonds, s.nnative
+    bonds = np.stack([np.ctypeslib.as_array(s.bond_i, shape=(max(nb, 1),))[:nb],
+                      np.ctypeslib.as_array(s.bond_j, shape=(max(nb, 1),))[:nb]], axis=1).astype(np.int32)
+    br0 = np.ctypeslib.as_array(s.bond_r0, shape=(max(nb, 1),))[:nb].copy()
+    nat = np.stack([np.ctypeslib.as_array(s.nat_i, shape=(max(nn, 1),))[:nn],
+                    np.ctypeslib.as_array(s.nat_j, shape=(max(nn, 1),))[:nn]], axis=1).astype(np.int32)
+    nr0 = np.ctypeslib.as_array(s.nat_r0, shape=(max(nn, 1),))[:nn].copy()
+    neh = np.ctypeslib.as_array(s.nat_eh, shape=(max(nn, 1),))[:nn].copy()
+    return System(coords, bonds, br0, nat, nr0, neh)
+
+
+def synth_chain(n: int, seed: int, density: float = 175.0, eh=(0.9, 1.5)) -> System:
+    s = SynthStruct()
+    check(lib.sop_synth_chain(n, seed, density, eh[0], eh[1], C.byref(s)))
+    try:
+        return _from_synth(s)
+    finally:
+        lib.sop_free_system(C.byref(s))
+
+
+def synth_shell(nchains: int, n: int, seed: int, radius: float = 140.0, eh=(0.9, 1.5)) -> System:
+    s = SynthStruct()
+    check(lib.sop_synth_shell(nchains, n, seed, radius, eh[0], eh[1], C.byref(s)))
+    try:
+        return _from_synth(s)
+    finally:
+        lib.sop_free_system(C.byref(s))
+
+
+def generate_seeds(seed: int, n: int) -> np.ndarray:
+    out = np.zeros((n, 4), np.uint32)
+    check(lib.sop_generate_seeds(seed, n, _ptr(out)))
+    return out
+
+
+def rng_probe(seeds: np.ndarray, count: int, gaussian: bool, device: int = 0):
+    seeds = np.ascontiguousarray(seeds, dtype=np.uint32)
+    n = seeds.shape[0]
+    out = np.zeros((n, count, 3) if gaussian else (n, count), np.float32 if gaussian else np.uint32)
+    st = np.zeros((n, 4), np.uint32)
+    check(lib.sop_rng_probe(device, _ptr(seeds), n, count, int(gaussian), _ptr(out), _ptr(st)))
+    return out, st
+
+
+class Engine:
+    """An engine on one GPU holding any number of jobs."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        h = VP()
+        check(lib.sop_engine_create(device, hbm_budget, C.byref(h)))
+        self.h = h
+        self._n = {}
+
+    def close(self):
+        if self.h:
+            lib.sop_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def add_job(self, params: Params, system: System, numsteps: int) -> int:
+        st, keep = system.struct()
+        jid = C.c_int()
+        check(lib.sop_engine_add_job(self.h, C.byref(params), C.byref(st), numsteps, C.byref(jid)))
+        self._n[jid.value] = system.n
+        del keep
+        return jid.value
+
+    def remove_job(self, jid: int):
+        check(lib.sop_engine_remove_job(self.h, jid))
+        self._n.pop(jid, None)
+
+    def set_tuning(self, max_blocks: int = 0, use_graph: bool = True):
+        check(lib.sop_engine_set_tuning(self.h, max_blocks, int(use_graph)))
+
+    def autotune(self) -> int:
+        b = C.c_int()
+        check(lib.sop_engine_autotune(self.h, C.byref(b)))
+        return b.value
+
+    def run_blocks(self, nblocks: int):
+        check(lib.sop_engine_run_blocks(self.h, nblocks))
+
+    def run_job(self, jid: int):
+        check(lib.sop_engine_run_job(self.h, jid))
+
+    def block_steps(self) -> int:
+        L = C.c_int()
+        check(lib.sop_engine_block_steps(self.h, C.byref(L)))
+        return L.value
+
+    def synchronize(self):
+        check(lib.sop_engine_synchronize(self.h))
+
+    def status(self, jid: int):
+        s, e, o = C.c_longlong(), C.c_longlong(), C.c_int()
+        check(lib.sop_engine_job_status(self.h, jid, C.byref(s), C.byref(e), C.byref(o)))
+        return s.value, e.value, o.value
+
+    def active(self):
+        c, m = C.c_int(), C.c_int()
+        check(lib.sop_engine_active_jobs(self.h, C.byref(c), C.byref(m)))
+        return c.value, m.value
+
+    def coords(self, jid: int) -> np.ndarray:
+        out = np.zeros((self._n[jid], 4), np.float32)
+        check(lib.sop_engine_get_coords(self.h, jid, _ptr(out)))
+        return out
+
+    def set_coords(self, jid: int, c: np.ndarray):
+        c = np.ascontiguousarray(c, dtype=np.float32)
+        check(lib.sop_engine_set_coords(self.h, jid, _ptr(c)))
+
+    def temperature(self, jid: int) -> np.ndarray:
+        out = np.zeros(self._n[jid], np.float32)
+        check(lib.sop_engine_get_temperature(self.h, jid, _ptr(out)))
+        return out
+
+    def rng(self, jid: int) -> np.ndarray:
+        out = np.zeros((self._n[jid], 4), np.uint32)
+        check(lib.sop_engine_get_rng(self.h, jid, _ptr(out)))
+        return out
+
+    def ext(self, jid: int) -> np.ndarray:
+        out = np.zeros((self._n[jid], 4), np.float32)
+        check(lib.sop_engine_get_ext(self.h, jid, _ptr(out)))
+        return out
+
+    def list(self, jid: int, which: int):
+        cap = C.c_int()
+        check(lib.sop_engine_list_cap(self.h, jid, which, C.byref(cap)))
+        n = self._n[jid]
+        cnt = np.zeros(n, np.int32)
+        ent = np.zeros(cap.value * n, np.int32)
+        check(lib.sop_engine_get_list(self.h, jid, which, _ptr(cnt), _ptr(ent)))
+        return cnt, ent.reshape(cap.value, n)
+
+    def indentation(self, jid: int) -> np.ndarray:
+        out = np.zeros(13, np.float32)
+        check(lib.sop_engine_get_indentation(self.h, jid, _ptr(out)))
+        return out
+
+    def forces(self, jid: int) -> np.ndarray:
+        out = np.zeros((self._n[jid], 4), np.float32)
+        check(lib.sop_engine_compute_forces(self.h, jid, _ptr(out)))
+        return out
+
+    def energies(self, jid: int):
+        n = self._n[jid]
+        per = np.zeros(3 * n, np.float32)
+        sums = np.zeros(3, np.float32)
+        check(lib.sop_engine_compute_energies(self.h, jid, _ptr(per), _ptr(sums)))
+        return per.reshape(3, n), sums
+
+    def rebuild(self, jid: int, mask: int = 3):
+        check(lib.sop_engine_rebuild_lists(self.h, jid, mask))
+
+    def profile(self, on: bool):
+        check(lib.sop_engine_profile(self.h, int(on)))
+
+    def profile_read(self):
+        n, ms = C.c_longlong(), C.c_double()
+        check(lib.sop_engine_profile_read(self.h, C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
+    def list_stats(self):
+        avg = np.zeros(4, np.float64)
+        nb = C.c_longlong()
+        check(lib.sop_engine_list_stats(self.h, _ptr(avg), C.byref(nb)))
+        return avg, nb.value
+
+
+def job_bytes(params: Params, system: System) -> int:
+    st, keep = system.struct()
+    b = C.c_ulonglong()
+    check(lib.sop_engine_job_bytes(C.byref(params), C.byref(st), C.byref(b)))
+    return b.value

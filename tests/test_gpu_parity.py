@@ -1,0 +1,199 @@
+"""GPU parity: the HIP path through the C ABI against the oracle.
+
+Bars: lists, RNG streams, coordinates of whole trajectories and per-bead
+energies bit-exact with the oracle's FP32 restatement; forces bit-exact and
+within 1e-5 (norm-wise relative) of the FP64 oracle."""
+import numpy as np
+import pytest
+
+import oracle as O
+import sopgpu as S
+from helpers import c0_params, numpy_chain
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def eng():
+    e = S.Engine(0, 8 << 30)
+    yield e
+    e.close()
+
+
+def lists_equal(a_cnt, a_ent, b_cnt, b_ent):
+    if not np.array_equal(a_cnt, b_cnt):
+        return False
+    for i in range(len(a_cnt)):
+        if not np.array_equal(a_ent[: a_cnt[i], i], b_ent[: b_cnt[i], i]):
+            return False
+    return True
+
+
+def test_rng_stream_and_gaussians_bit_exact():
+    seeds = O.generate_seeds(123, 512)
+    u, st = S.rng_probe(seeds, 40, gaussian=False)
+    ou, ost = O.ht_stream(seeds[:64], 40)
+    np.testing.assert_array_equal(u[:64], ou)
+    g, gst = S.rng_probe(seeds, 10, gaussian=True)
+    og, _ = O.gaussians(seeds[:64], 10)
+    np.testing.assert_array_equal(g[:64], og)
+
+
+@pytest.mark.parametrize("n,seed", [(64, 1), (257, 2), (1000, 3)])
+def test_forces_bit_exact_and_fp64(eng, n, seed):
+    sysm = S.synth_chain(n, seed=seed)
+    kw = c0_params(pairs_threshold=32.0)
+    jid = eng.add_job(S.make_params(**kw), sysm, 0)
+    try:
+        eng.rebuild(jid, 3)
+        ref = O.Sim(O.make_params(**kw), sysm)
+        ref.build_possible()
+        ref.build_verlet()
+        assert lists_equal(*eng.list(jid, 3), *ref.list(3))
+        assert lists_equal(*eng.list(jid, 2), *ref.list(2))
+        f = eng.forces(jid)
+        np.testing.assert_array_equal(f, ref.forces32())
+        f64 = ref.forces64()
+        err = np.linalg.norm(f[:, :3] - f64) / np.linalg.norm(f64)
+        assert err < 1e-5, err
+        per, sums = eng.energies(jid)
+        np.testing.assert_array_equal(per, ref.energies32())
+        np.testing.assert_array_equal(sums, ref.sum_energies()[:, 0])
+    finally:
+        eng.remove_job(jid)
+
+
+def test_possible_pairs_with_many_exclusions(eng):
+    sysm = numpy_chain(300, seed=9)
+    kw = c0_params(pairs_threshold=20.0, pairlist_cutoff=12.0)
+    jid = eng.add_job(S.make_params(**kw), sysm, 0)
+    try:
+        eng.rebuild(jid, 3)
+        ref = O.Sim(O.make_params(**kw), sysm)
+        ref.build_possible()
+        ref.build_verlet()
+        assert lists_equal(*eng.list(jid, 3), *ref.list(3))
+        assert lists_equal(*eng.list(jid, 2), *ref.list(2))
+    finally:
+        eng.remove_job(jid)
+
+
+def _pull_system(n, seed):
+    sysm = numpy_chain(n, seed=seed) if n <= 400 else S.synth_chain(n, seed=seed)
+    sysm.fixed = np.array([0], np.int32)
+    sysm.pulled = np.array([n - 1], np.int32)
+    return sysm
+
+
+@pytest.mark.parametrize("temperature", [0.0, 0.6])
+def test_trajectory_bit_exact_plain(eng, temperature):
+    n = 400
+    sysm = _pull_system(n, 12)
+    kw = c0_params(temperature=temperature, pulling=1, fixedEnd=0, pulledEnd=n - 1, deltax=0.02, pullFreq=100,
+                   nav=100, outputtiming=100, dcdfreq=100)
+    jid = eng.add_job(S.make_params(**kw), sysm, 990)
+    try:
+        eng.run_job(jid)
+        ref = O.Sim(O.make_params(**kw), sysm)
+        assert ref.run(990) == 1000
+        assert eng.status(jid)[0] == 1000
+        np.testing.assert_array_equal(eng.coords(jid), ref.coords())
+        np.testing.assert_array_equal(eng.rng(jid), ref.rng())
+        np.testing.assert_array_equal(eng.temperature(jid), ref.temperature())
+        np.testing.assert_array_equal(eng.ext(jid), ref.ext())
+        assert lists_equal(*eng.list(jid, 2), *ref.list(2))
+    finally:
+        eng.remove_job(jid)
+
+
+def test_zero_temperature_1000_steps_vs_fp64(eng):
+    n = 1000
+    sysm = _pull_system(n, 13)
+    kw = c0_params(temperature=0.0, pulling=1, fixedEnd=0, pulledEnd=n - 1, deltax=0.05, pullFreq=10)
+    jid = eng.add_job(S.make_params(**kw), sysm, 990)
+    try:
+        eng.run_job(jid)
+        x = eng.coords(jid)[:, :3].astype(np.float64)
+    finally:
+        eng.remove_job(jid)
+    x64 = O.Sim(O.make_params(**kw), sysm).run64(990)
+    err = np.linalg.norm(x - x64) / np.linalg.norm(x64)
+    assert err < 1e-5, err
+
+
+def test_batched_equals_plain_and_alone_equals_mixed():
+    """Jobs of uneven sizes and protocols stepped together in one kernel per
+    step give the same bits as each job run alone."""
+    rng = np.random.default_rng(5)
+    sizes = [200, 333, 1000, 64, 517]
+    jobs = []
+    for k, n in enumerate(sizes):
+        sysm = _pull_system(n, 30 + k)
+        kw = c0_params(temperature=[0.0, 0.6][k % 2], seed=100 + k, pulling=1, fixedEnd=0, pulledEnd=n - 1,
+                       deltax=float(rng.uniform(0.005, 0.05)), k_trans=float(rng.uniform(0.014, 0.1)), pullFreq=100,
+                       nav=100, outputtiming=100, dcdfreq=100)
+        jobs.append((sysm, kw))
+    steps = 490
+    alone = []
+    with S.Engine(0, 4 << 30) as e1:
+        for sysm, kw in jobs:
+            jid = e1.add_job(S.make_params(**kw), sysm, steps)
+            e1.run_job(jid)
+            alone.append(e1.coords(jid))
+            e1.remove_job(jid)
+    with S.Engine(0, 4 << 30) as e2:
+        ids = [e2.add_job(S.make_params(**kw), sysm, steps) for sysm, kw in jobs]
+        L = e2.block_steps()
+        e2.run_blocks((steps // 10 + 1) * 10 // L)
+        for jid, ref in zip(ids, alone):
+            assert e2.status(jid)[0] == steps // 10 * 10 + 10
+            np.testing.assert_array_equal(e2.coords(jid), ref)
+
+
+def test_job_admitted_mid_run_matches_alone():
+    n1, n2 = 300, 450
+    s1, s2 = _pull_system(n1, 40), _pull_system(n2, 41)
+    k1 = c0_params(pulling=1, fixedEnd=0, pulledEnd=n1 - 1, deltax=0.02, pullFreq=100, seed=7)
+    k2 = c0_params(pulling=1, fixedEnd=0, pulledEnd=n2 - 1, deltax=0.03, pullFreq=100, seed=8)
+    with S.Engine(0, 2 << 30) as e:
+        a = e.add_job(S.make_params(**k1), s1, 390)
+        e.run_blocks(15)
+        b = e.add_job(S.make_params(**k2), s2, 290)
+        e.run_blocks(40)
+        ca, cb = e.coords(a), e.coords(b)
+        assert e.status(a)[0] == 400 and e.status(b)[0] == 300
+    ra = O.Sim(O.make_params(**k1), s1); ra.run(390)
+    rb = O.Sim(O.make_params(**k2), s2); rb.run(290)
+    np.testing.assert_array_equal(ca, ra.coords())
+    np.testing.assert_array_equal(cb, rb.coords())
+
+
+def test_indentation_bit_exact(eng):
+    n = 300
+    sysm = numpy_chain(n, seed=50)
+    c = sysm.coords[:, :3]
+    top = float(c[:, 2].max())
+    kw = c0_params(temperature=0.6, indentation=1, indTip=(0, 0, top + 53.0), indChip=(0, 0, top + 53.0),
+                   indDirection=(0, 0, -1), indSurfR0=(0, 0, float(c[:, 2].min()) - 5.0), indSurfN=(0, 0, 1),
+                   indTipR=50.0, indKs=0.07, indDeltaX=0.5, indTipZeta=5000.0, indSurfA=1.0, indSurfB=-2.0,
+                   indSurfSigma=2.0, indSurfEl=0.2, nav=50, outputtiming=50, dcdfreq=50)
+    jid = eng.add_job(S.make_params(**kw), sysm, 490)
+    try:
+        eng.run_job(jid)
+        ref = O.Sim(O.make_params(**kw), sysm)
+        ref.run(490)
+        np.testing.assert_array_equal(eng.coords(jid), ref.coords())
+        np.testing.assert_array_equal(eng.indentation(jid)[:9], ref.indentation())
+    finally:
+        eng.remove_job(jid)
+
+
+def test_pairlist_overflow_is_reported(eng):
+    sysm = numpy_chain(200, seed=60)
+    kw = c0_params(pairs_threshold=30.0, max_possiblePairs=8)
+    jid = eng.add_job(S.make_params(**kw), sysm, 0)
+    try:
+        eng.rebuild(jid, 1)
+        assert eng.status(jid)[2] & 1
+    finally:
+        eng.remove_job(jid)
