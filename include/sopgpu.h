@@ -74,6 +74,13 @@ typedef struct sop_params {
     float indTipZeta;
     int indOutputFreq;
     long long indRetractionStep;
+    /* src/Integrators/bdhitea.h:15-21 */
+    int hi_on, hi_exact;
+    float hi_a;
+    int hi_epsilon_freq, hi_capricious, hi_unlisted;
+    float hi_epsmax;
+    int hi_chol_freq;       /* refactorise every k steps (reference: 1) */
+    int hi_faithful_blocks; /* apply L's 3x3 blocks as integrateCholesky_L reads them */
 } sop_params;
 
 /* Topology and coordinates of one system in .top order
@@ -166,6 +173,22 @@ int sop_engine_profile_read(sop_engine* e, long long* launches, double* total_ms
 /* Average list lengths over admitted jobs' beads (covalent, native,
  * Verlet, possible) and the bead count, for the roofline byte count. */
 int sop_engine_list_stats(sop_engine* e, double* avg4, long long* beads);
+
+/* Hydrodynamics (src/Integrators/bdhitea.cu).  The exact path's trailing
+ * update runs as split-BF16 three-product MFMA (mode 0, default) or FP32
+ * MFMA (mode 1); diagonal blocks are always factorised in FP64. */
+int sop_engine_set_chol_mode(sop_engine* e, int mode);
+/* epsilon and beta' of the job's last TEA update (TeaUpdater::update,
+ * bdhitea.cu:133-176) */
+int sop_engine_get_hi(sop_engine* e, int job_id, double* epsilon, float* beta);
+/* the job's current Cholesky factor, 3n x 3n row-major, upper part zero */
+int sop_engine_get_hi_factor(sop_engine* e, int job_id, float* L);
+/* Standalone factorisation of an n x n SPD matrix (row-major; the lower
+ * triangle is read) into L (n x n, upper zero); repeats timed runs with
+ * HIP events, average in *ms_per_factor.  Replaces the one-block-per-
+ * system integrateCholesky_decompose (bdhitea_kernel.cu:154-190), which is
+ * limited to 3N <= 384. */
+int sop_chol_factor(int device, int n, const float* A, int mode, float* L, int repeats, double* ms_per_factor);
 
 /* RNG (src/Integrators/ht.cu): seed table on the host with the reference's
  * ran2 stream, and a device probe that advances `count` draws per bead and

@@ -48,6 +48,9 @@ class OrParams(C.Structure):
         ("indTipZeta", C.c_float),
         ("indOutputFreq", C.c_int),
         ("indRetractionStep", C.c_longlong),
+        ("hi_on", C.c_int), ("hi_exact", C.c_int), ("hi_a", C.c_float),
+        ("hi_epsilon_freq", C.c_int), ("hi_capricious", C.c_int), ("hi_unlisted", C.c_int),
+        ("hi_epsmax", C.c_float), ("hi_chol_freq", C.c_int), ("hi_faithful_blocks", C.c_int),
     ]
 
 
@@ -106,6 +109,9 @@ def lib():
         sig("or_sincosf", None, C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float))
         sig("or_rforce", None, VP, VP)
         sig("or_bench_steps", C.c_double, VP, C.c_int, C.c_int)
+        sig("or_hi_tea", None, VP, VP, VP)
+        sig("or_hi_factor", C.c_int, VP, VP)
+        sig("or_rpy_matrix", None, C.c_int, VP, C.c_double, VP)
         _lib = L
     return _lib
 
@@ -143,6 +149,8 @@ _DEFAULTS = dict(
     indTipSigma=1.0, indSurfSigma=1.0, indTipEl=1.0, indSurfEl=1.0,
     indTipA=0.0, indTipB=1.0, indSurfA=0.0, indSurfB=1.0, indTipZeta=5000.0, indOutputFreq=1000,
     indRetractionStep=-1,
+    hi_on=0, hi_exact=0, hi_a=1.8, hi_epsilon_freq=10, hi_capricious=1, hi_unlisted=1, hi_epsmax=999.0,
+    hi_chol_freq=1, hi_faithful_blocks=1,
 )
 
 
@@ -268,8 +276,29 @@ class Sim:
         m = lib().or_get_dat(self.h, o.ctypes.data, cap)
         return o[:min(m, cap)]
 
+    def hi_tea(self, ntr=1):
+        e = np.zeros(ntr, np.float64)
+        b = np.zeros(ntr, np.float32)
+        lib().or_hi_tea(self.h, e.ctypes.data, b.ctypes.data)
+        return e, b
+
+    def hi_factor(self):
+        m = 3 * self.n
+        L = np.zeros((m, m), np.float32)
+        if lib().or_hi_factor(self.h, L.ctypes.data) != 0:
+            raise RuntimeError("no Cholesky factor (hi_exact off)")
+        return L
+
     def bench(self, steps, threads):
         return lib().or_bench_steps(self.h, steps, threads)
+
+
+def rpy_matrix(coords, a):
+    c = np.ascontiguousarray(coords, np.float32)
+    n = c.shape[0]
+    D = np.zeros((3 * n, 3 * n), np.float64)
+    lib().or_rpy_matrix(n, c.ctypes.data, float(a), D.ctypes.data)
+    return D
 
 
 def generate_seeds(seed, n):

@@ -70,6 +70,12 @@ struct or_sim {
     /* dat rows */
     double* dat; int ndat, capdat;
     int nthreads;
+    /* hydrodynamics (bdhitea.cu) */
+    f4 *rforce, *mforce, *mforce0, *hcoord, *ci;
+    float* eps;
+    float* beta;       /* per trajectory */
+    double* eps_traj;  /* per trajectory, last TEA update */
+    float* L;          /* per trajectory (3N)^2 factor, row-major lower */
 };
 
 /* ------------------------------------------------------------------ */
@@ -436,6 +442,23 @@ or_sim* or_create(const or_params* p, int N, int ntr, const float* coords4,
         norm = sqrtf(s->micaN[0] * s->micaN[0] + s->micaN[1] * s->micaN[1] + s->micaN[2] * s->micaN[2]);
         s->micaN[0] /= norm; s->micaN[1] /= norm; s->micaN[2] /= norm;
     }
+    /* hydrodynamics, bdhitea.cu:30-100 */
+    if (p->hi_on) {
+        if (p->minimization) {
+            snprintf(g_err, sizeof g_err, "Using minimization with BDHI integrator is not supported and will never be.");
+            or_destroy(s);
+            return NULL;
+        }
+        s->rforce = (f4*)xcalloc(n, sizeof(f4));
+        s->mforce = (f4*)xcalloc(n, sizeof(f4));
+        s->mforce0 = (f4*)xcalloc(n, sizeof(f4));
+        s->hcoord = (f4*)xcalloc(n, sizeof(f4));
+        s->ci = (f4*)xcalloc(n, sizeof(f4));
+        s->eps = (float*)xcalloc(n, sizeof(float));
+        s->beta = (float*)xcalloc(ntr, sizeof(float));
+        s->eps_traj = (double*)xcalloc(ntr, sizeof(double));
+        if (p->hi_exact) s->L = (float*)xcalloc((size_t)ntr * 9 * N * N, sizeof(float));
+    }
     /* stride: gsop.cu:116-122 over the updaters initFF creates */
     {
         int st = p->nav;
@@ -447,6 +470,7 @@ or_sim* or_create(const or_params* p, int N, int ntr, const float* coords4,
         MINF(p->pairs_freq);
         MINF(p->dcdfreq);
         if (p->heating) MINF(p->tempFreq);
+        if (p->hi_on && !p->hi_exact) MINF(p->hi_epsilon_freq);
 #undef MINF
         s->stride = st;
     }
@@ -461,6 +485,8 @@ void or_destroy(or_sim* s) {
     free(s->nat_cnt); free(s->nat_j); free(s->nat_r02); free(s->nat_m12);
     free(s->pair_cnt); free(s->pair_j); free(s->pp_cnt); free(s->pp_j);
     free(s->ni); free(s->nj); free(s->nr0); free(s->fixed); free(s->pulled); free(s->dat);
+    free(s->rforce); free(s->mforce); free(s->mforce0); free(s->hcoord); free(s->ci); free(s->eps); free(s->beta);
+    free(s->eps_traj); free(s->L);
     free(s);
 }
 
@@ -864,6 +890,353 @@ void or_build_verlet(or_sim* s) {
     s->overflow |= ovf;
 }
 
+
+/* ------------------------------------------------------------------ */
+/* hydrodynamic interactions, bdhitea.cu + bdhitea_kernel.cu            */
+/* ------------------------------------------------------------------ */
+typedef struct { float xx, xy, xz, yy, yz, zz; } f6;
+
+/* integrateTea_RPY, bdhitea_kernel.cu:67-87: D_ij / D_ii for a normalised
+ * separation dr with its length in w. */
+static inline f6 rpy(const f4 dr, float a) {
+    f6 d;
+    const float ra = dr.w / a;
+    float crr, cii;
+    if (ra > 2.f) {
+        crr = 0.75f / ra * (1.f - 2.f / ra / ra);
+        cii = 0.75f / ra * (1.f + 2.f / 3.f / ra / ra);
+    } else {
+        crr = 3.f * ra / 32.f;
+        cii = 1.f - 9.f * ra / 32.f;
+    }
+    d.xx = dr.x * dr.x * crr + cii;
+    d.xy = dr.x * dr.y * crr;
+    d.xz = dr.x * dr.z * crr;
+    d.yy = dr.y * dr.y * crr + cii;
+    d.yz = dr.y * dr.z * crr;
+    d.zz = dr.z * dr.z * crr + cii;
+    return d;
+}
+
+static inline f4 unit_sep(const f4 a, const f4 b) { /* b - a, normalised */
+    f4 dr = b;
+    dr.x -= a.x; dr.y -= a.y; dr.z -= a.z;
+    dr.w = sqrtf(dr.x * dr.x + dr.y * dr.y + dr.z * dr.z);
+    dr.x /= dr.w; dr.y /= dr.w; dr.z /= dr.w;
+    return dr;
+}
+
+/* integrateTea_prepare, bdhitea_kernel.cu:33-65 */
+static void hi_prepare(or_sim* s) {
+    for (int d = 0; d < s->n; d++) {
+        float g[4];
+        or_rforce(&s->rng[4 * d], g);
+        f4 df = {g[0] * s->var, g[1] * s->var, g[2] * s->var, g[3]};
+        s->rforce[d] = df;
+        f4 f = s->force[d];
+        if (s->p.pulling) {
+            f4 e = s->ext[d];
+            if (e.w == 1.0f) { f.x = 0.0f; f.y = 0.0f; f.z = 0.0f; }
+            if (e.w == 2.0f) { f.x += e.x; f.y += e.y; f.z += e.z; }
+        }
+        s->mforce[d] = f;
+        s->force[d].x = s->force[d].y = s->force[d].z = s->force[d].w = 0.0f;
+        s->hcoord[d] = s->coord[d];
+    }
+}
+
+/* integrateTea_epsilon_local, bdhitea_kernel.cu:192-210 */
+static inline void eps_local(const or_sim* s, const f4 c1, int j, f4* sum) {
+    f4 dr = unit_sep(c1, s->coord[j]);
+    f6 d = rpy(dr, s->p.hi_a);
+    sum->w += d.xx + 2 * d.xy + 2 * d.xz + d.yy + 2 * d.yz + d.zz;
+    sum->x += d.xx * d.xx + d.xy * d.xy + d.xz * d.xz;
+    sum->y += d.xy * d.xy + d.yy * d.yy + d.yz * d.yz;
+    sum->z += d.xz * d.xz + d.yz * d.yz + d.zz * d.zz;
+}
+
+/* integrateTea_epsilon[_unlisted] (bdhitea_kernel.cu:212-249) and
+ * TeaUpdater::update (bdhitea.cu:133-176, FP64 on the host). */
+static int tea_update(or_sim* s) {
+    const int n = s->n, N = s->N;
+    for (int d = 0; d < n; d++) {
+        const f4 c = s->coord[d];
+        f4 sum = {0.f, 0.f, 0.f, 0.f};
+        f4 part;
+        if (s->p.hi_unlisted) {
+            int i0 = (d / N) * N;
+            for (int i = i0; i < i0 + N; i++) {
+                if (i == d) continue;
+                part = (f4){0.f, 0.f, 0.f, 0.f};
+                eps_local(s, c, i, &part);
+                sum.x += part.x; sum.y += part.y; sum.z += part.z; sum.w += part.w;
+            }
+        } else {
+            for (int k = 0; k < s->cov_cnt[d]; k++) {
+                part = (f4){0.f, 0.f, 0.f, 0.f};
+                eps_local(s, c, s->cov_j[k * n + d], &part);
+                sum.x += part.x; sum.y += part.y; sum.z += part.z; sum.w += part.w;
+            }
+            for (int k = 0; k < s->nat_cnt[d]; k++) {
+                part = (f4){0.f, 0.f, 0.f, 0.f};
+                eps_local(s, c, s->nat_j[k * n + d], &part);
+                sum.x += part.x; sum.y += part.y; sum.z += part.z; sum.w += part.w;
+            }
+            for (int k = 0; k < s->pair_cnt[d]; k++) {
+                part = (f4){0.f, 0.f, 0.f, 0.f};
+                eps_local(s, c, s->pair_j[k * n + d], &part);
+                sum.x += part.x; sum.y += part.y; sum.z += part.z; sum.w += part.w;
+            }
+        }
+        s->ci[d] = sum;
+        s->eps[d] = sum.w;
+    }
+    for (int t = 0; t < s->ntr; t++) {
+        double epsilon = 0.0;
+        for (int i = 0; i < N; i++) epsilon += s->eps[t * N + i];
+        if (N > 1) epsilon /= 3. * N * (3. * N - 3.);
+        if (epsilon > 1.0) {
+            if (s->p.hi_capricious) {
+                snprintf(g_err, sizeof g_err, "HI tensor is not diagonal enough for trajectory %d: epsilon = %lf -> 1.0!", t, epsilon);
+                return -1;
+            }
+            epsilon = 1.0;
+        }
+        if (epsilon > s->p.hi_epsmax) {
+            snprintf(g_err, sizeof g_err, "HI tensor is not diagonal enough for trajectory %d: epsilon = %lf > %f = hi_epsmax!", t, epsilon, s->p.hi_epsmax);
+            return -1;
+        }
+        double a = (3. * N - 1.) * epsilon * epsilon - (3. * N - 2.) * epsilon;
+        if (fabs(a) < 1e-7) {
+            s->beta[t] = .5f;
+            if (s->p.hi_capricious && s->p.hi_a > 0.0f) {
+                snprintf(g_err, sizeof g_err, "HI tensor is too diagonal for trajectory %d", t);
+                return -1;
+            }
+        } else {
+            s->beta[t] = (float)((1. - sqrt(1. - a)) / a);
+        }
+        s->eps_traj[t] = epsilon;
+    }
+    return 0;
+}
+
+/* integrateTea_force, bdhitea_kernel.cu:252-300; neighbour coordinates
+ * from the step-start copy (the reference reads them through a texture of
+ * the array the same launch writes, bdhitea_kernel.cu:256 vs :450). */
+static inline void tea_force(const or_sim* s, const f4 c1, int j, const float* ci, float* f) {
+    f4 dr = unit_sep(c1, s->hcoord[j]);
+    f4 g = s->mforce[j];
+    const f4 r = s->rforce[j];
+    g.x += r.x * ci[0];
+    g.y += r.y * ci[1];
+    g.z += r.z * ci[2];
+    f6 D = rpy(dr, s->p.hi_a);
+    f[0] = D.xx * g.x + D.xy * g.y + D.xz * g.z;
+    f[1] = D.xy * g.x + D.yy * g.y + D.yz * g.z;
+    f[2] = D.xz * g.x + D.yz * g.y + D.zz * g.z;
+}
+
+static inline void hi_finish(or_sim* s, int d, f4 f, f4 coord) {
+    if (s->p.pulling) {
+        f4 e = s->ext[d];
+        if (e.w == 1.0f) { f.x = 0.0f; f.y = 0.0f; f.z = 0.0f; }
+    }
+    const float mult = s->hOverZeta;
+    float dx = mult * f.x, dy = mult * f.y, dz = mult * f.z;
+    coord.x += dx; coord.y += dy; coord.z += dz;
+    s->coord[d] = coord;
+    float T = s->T[d];
+    T += s->tempNorm * (dx * dx + dy * dy + dz * dz);
+    s->T[d] = T;
+}
+
+/* integrateTea_kernel[_unlisted], bdhitea_kernel.cu:364-515 */
+static void tea_integrate(or_sim* s) {
+    const int n = s->n, N = s->N;
+    f4* out = (f4*)xcalloc(n, sizeof(f4));
+    for (int d = 0; d < n; d++) {
+        f4 cs = s->ci[d];
+        f4 f = s->mforce[d];
+        f4 df = s->rforce[d];
+        const float beta = s->beta[d / N];
+        float ci[3];
+        cs.w = beta * beta;
+        ci[0] = 1.f / sqrtf(1.f + cs.w * cs.x);
+        ci[1] = 1.f / sqrtf(1.f + cs.w * cs.y);
+        ci[2] = 1.f / sqrtf(1.f + cs.w * cs.z);
+        const f4 coord = s->hcoord[d];
+        f.x += df.x * ci[0];
+        f.y += df.y * ci[1];
+        f.z += df.z * ci[2];
+        ci[0] *= beta; ci[1] *= beta; ci[2] *= beta;
+        float t[3];
+        if (s->p.hi_unlisted) {
+            int i0 = (d / N) * N;
+            for (int i = i0; i < i0 + N; i++) {
+                if (i == d) continue;
+                tea_force(s, coord, i, ci, t);
+                f.x += t[0]; f.y += t[1]; f.z += t[2];
+            }
+        } else {
+            for (int k = 0; k < s->cov_cnt[d]; k++) { tea_force(s, coord, s->cov_j[k * n + d], ci, t); f.x += t[0]; f.y += t[1]; f.z += t[2]; }
+            for (int k = 0; k < s->nat_cnt[d]; k++) { tea_force(s, coord, s->nat_j[k * n + d], ci, t); f.x += t[0]; f.y += t[1]; f.z += t[2]; }
+            for (int k = 0; k < s->pair_cnt[d]; k++) { tea_force(s, coord, s->pair_j[k * n + d], ci, t); f.x += t[0]; f.y += t[1]; f.z += t[2]; }
+        }
+        out[d] = f;
+    }
+    for (int d = 0; d < n; d++) hi_finish(s, d, out[d], s->hcoord[d]);
+    free(out);
+}
+
+/* integrateCholesky_D, bdhitea_kernel.cu:89-152: D/D_ii (identity diagonal
+ * blocks) into M when `build`, and F_i + sum_j D_ij F_j from the step-start
+ * forces (the reference reads them through a texture of the array it
+ * writes in the same launch). */
+static void chol_D(or_sim* s, int build) {
+    const int n = s->n, N = s->N;
+    const int m = 3 * N;
+    memcpy(s->mforce0, s->mforce, sizeof(f4) * n);
+    for (int d = 0; d < n; d++) {
+        const int t = d / N, i0 = t * N;
+        float* M = s->L + (size_t)t * m * m;
+        const f4 c = s->hcoord[d];
+        f4 f = s->mforce0[d];
+        for (int i = i0; i < i0 + N; i++) {
+            const int r = 3 * (i - i0), col = 3 * (d - i0);
+            if (i == d) {
+                if (build)
+                    for (int a = 0; a < 3; a++)
+                        for (int b = 0; b < 3; b++) M[(size_t)(r + a) * m + col + b] = a == b ? 1.f : 0.f;
+                continue;
+            }
+            f4 dr = unit_sep(c, s->hcoord[i]);
+            f6 D = rpy(dr, s->p.hi_a);
+            if (build) {
+                float blk[9] = {D.xx, D.xy, D.xz, D.xy, D.yy, D.yz, D.xz, D.yz, D.zz};
+                for (int a = 0; a < 3; a++)
+                    for (int b = 0; b < 3; b++) M[(size_t)(r + a) * m + col + b] = blk[3 * a + b];
+            }
+            const f4 g = s->mforce0[i];
+            f.x += D.xx * g.x + D.xy * g.y + D.xz * g.z;
+            f.y += D.xy * g.x + D.yy * g.y + D.yz * g.z;
+            f.z += D.xz * g.x + D.yz * g.y + D.zz * g.z;
+        }
+        s->mforce[d] = f;
+    }
+}
+
+/* integrateCholesky_decompose, bdhitea_kernel.cu:154-190: column-oriented
+ * Cholesky, L(t,j) = (A(j,t) - sum_k L(t,k) L(j,k)) / L(j,j), k ascending;
+ * the result is kept in the lower triangle (the reference mirrors it). */
+static void chol_decompose(or_sim* s) {
+    const int m = 3 * s->N;
+    float* row = (float*)xcalloc(m, sizeof(float));
+    for (int t = 0; t < s->ntr; t++) {
+        float* M = s->L + (size_t)t * m * m;
+        for (int j = 0; j < m; j++) {
+            for (int c = 0; c < m; c++) row[c] = c <= j ? M[(size_t)j * m + c] : M[(size_t)c * m + j];
+            float diag = 0.f;
+            for (int tid = j; tid < m; tid++) {
+                float val = tid == j ? row[j] : M[(size_t)tid * m + j];
+                for (int k = 0; k < j; k++) val -= M[(size_t)tid * m + k] * row[k];
+                if (tid == j) diag = sqrtf(val);
+                row[tid] = val; /* unscaled, consumed below */
+            }
+            for (int tid = j; tid < m; tid++) M[(size_t)tid * m + j] = row[tid] / diag;
+            for (int c = j + 1; c < m; c++) M[(size_t)j * m + c] = 0.f;
+        }
+    }
+    free(row);
+}
+
+/* integrateCholesky_L, bdhitea_kernel.cu:303-361 */
+static void chol_L(or_sim* s) {
+    const int n = s->n, N = s->N, m = 3 * N;
+    f4* out = (f4*)xcalloc(n, sizeof(f4));
+    for (int d = 0; d < n; d++) {
+        const int t = d / N, i0 = t * N, r = 3 * (d - i0);
+        const float* M = s->L + (size_t)t * m * m;
+        f4 f = s->mforce[d];
+        for (int i = i0; i <= d; i++) {
+            const f4 g = s->rforce[i];
+            const int c = 3 * (i - i0);
+            if (s->p.hi_faithful_blocks) {
+                /* the six entries the reference reads, applied as a
+                 * symmetric block (bdhitea_kernel.cu:322-337) */
+                float xx = M[(size_t)r * m + c], xy = M[(size_t)(r + 1) * m + c], xz = M[(size_t)(r + 2) * m + c];
+                float yy = M[(size_t)(r + 1) * m + c + 1], yz = M[(size_t)(r + 2) * m + c + 1];
+                float zz = M[(size_t)(r + 2) * m + c + 2];
+                f.x += xx * g.x + xy * g.y + xz * g.z;
+                f.y += xy * g.x + yy * g.y + yz * g.z;
+                f.z += xz * g.x + yz * g.y + zz * g.z;
+            } else {
+                const float* B = M + (size_t)r * m + c;
+                f.x += B[0] * g.x + B[1] * g.y + B[2] * g.z;
+                f.y += B[m] * g.x + B[m + 1] * g.y + B[m + 2] * g.z;
+                f.z += B[2 * m] * g.x + B[2 * m + 1] * g.y + B[2 * m + 2] * g.z;
+            }
+        }
+        out[d] = f;
+    }
+    for (int d = 0; d < n; d++) hi_finish(s, d, out[d], s->hcoord[d]);
+    free(out);
+}
+
+/* TeaIntegrator::integrate, bdhitea.cu:76-109 */
+static void hi_integrate(or_sim* s) {
+    hi_prepare(s);
+    if (!s->p.hi_exact) {
+        tea_integrate(s);
+    } else {
+        const int refactor = (s->step % (s->p.hi_chol_freq > 0 ? s->p.hi_chol_freq : 1)) == 0;
+        chol_D(s, refactor);
+        if (refactor) chol_decompose(s);
+        chol_L(s);
+    }
+}
+
+void or_hi_tea(const or_sim* s, double* eps, float* beta) {
+    for (int t = 0; t < s->ntr; t++) {
+        eps[t] = s->eps_traj ? s->eps_traj[t] : 0.0;
+        beta[t] = s->beta ? s->beta[t] : 0.0f;
+    }
+}
+
+int or_hi_factor(const or_sim* s, float* L) {
+    if (!s->L) return -1;
+    memcpy(L, s->L, sizeof(float) * 9 * (size_t)s->N * s->N);
+    return 0;
+}
+
+void or_rpy_matrix(int n, const float* c4, double a, double* D) {
+    const int m = 3 * n;
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) {
+            double blk[9];
+            if (i == j) {
+                for (int k = 0; k < 9; k++) blk[k] = (k % 4 == 0) ? 1.0 : 0.0;
+            } else {
+                double x = (double)c4[4 * j] - c4[4 * i], y = (double)c4[4 * j + 1] - c4[4 * i + 1],
+                       z = (double)c4[4 * j + 2] - c4[4 * i + 2];
+                double r = sqrt(x * x + y * y + z * z), ra = r / a, crr, cii;
+                x /= r; y /= r; z /= r;
+                if (ra > 2.0) {
+                    crr = 0.75 / ra * (1.0 - 2.0 / (ra * ra));
+                    cii = 0.75 / ra * (1.0 + 2.0 / (3.0 * ra * ra));
+                } else {
+                    crr = 3.0 * ra / 32.0;
+                    cii = 1.0 - 9.0 * ra / 32.0;
+                }
+                double u[3] = {x, y, z};
+                for (int p = 0; p < 3; p++)
+                    for (int q = 0; q < 3; q++) blk[3 * p + q] = u[p] * u[q] * crr + (p == q ? cii : 0.0);
+            }
+            for (int p = 0; p < 3; p++)
+                for (int q = 0; q < 3; q++) D[(size_t)(3 * i + p) * m + 3 * j + q] = blk[3 * p + q];
+        }
+}
+
 /* ------------------------------------------------------------------ */
 /* updaters                                                             */
 /* ------------------------------------------------------------------ */
@@ -974,10 +1347,18 @@ long long or_run(or_sim* s, long long numsteps) {
         output_update(s);
         if (s->step % p->pairs_freq == 0) or_build_verlet(s);
         if (p->heating) temperature_update(s);
+        if (p->hi_on && !p->hi_exact && s->step % p->hi_epsilon_freq == 0)
+            if (tea_update(s)) return -1;
         for (int j = 0; j < stride; j++) {
             forces_pass(s);
-            integrate_pass(s);
+            if (p->hi_on) {
+                hi_integrate(s);
+                s->step += 1;
+            } else {
+                integrate_pass(s);
+            }
         }
+        if (p->hi_on) s->step -= stride;
         s->step += stride;
         done += stride;
     }

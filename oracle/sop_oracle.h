@@ -74,6 +74,13 @@ typedef struct or_params {
     float indTipZeta;
     int indOutputFreq;
     long long indRetractionStep;
+    /* src/Integrators/bdhitea.h:15-21 */
+    int hi_on, hi_exact;
+    float hi_a;
+    int hi_epsilon_freq, hi_capricious, hi_unlisted;
+    float hi_epsmax;
+    int hi_chol_freq;       /* Cholesky refactorisation interval (reference: every step) */
+    int hi_faithful_blocks; /* apply L's 3x3 blocks as integrateCholesky_L reads them */
 } or_params;
 
 typedef struct or_sim or_sim;
@@ -121,6 +128,13 @@ long long or_run(or_sim* s, long long numsteps);
 /* FP64 twin of or_run (no noise: temperature must be 0 or minimization on). */
 long long or_run_f64(or_sim* s, long long numsteps, double* coords_out3);
 int or_stride(const or_sim* s);
+
+/* HI pieces: per-trajectory epsilon and beta' of the last TEA update, and
+ * the factor (3N x 3N row-major lower triangle, trajectory 0). */
+void or_hi_tea(const or_sim* s, double* eps, float* beta);
+int or_hi_factor(const or_sim* s, float* L);
+/* Standalone RPY (D/D_ii) matrix of coordinates, FP64, 3n x 3n row-major. */
+void or_rpy_matrix(int n, const float* coords4, double a, double* D);
 
 /* Indentation tip state (x,y,z of tip, chip, mica point) */
 void or_get_indentation(const or_sim* s, float* out9);

@@ -8,6 +8,7 @@
 #include "../../include/sopgpu.h"
 #include "engine.h"
 #include "hostutil.h"
+#include "hi.h"
 #include "kernels.h"
 
 namespace sop {
@@ -213,6 +214,78 @@ int sop_engine_list_stats(sop_engine* e, double* avg4, long long* beads) {
     ENG(e);
     if (!avg4 || !beads) return sop::fail(SOP_E_ARG, "null output");
     return E.list_stats(avg4, beads);
+}
+
+int sop_engine_set_chol_mode(sop_engine* e, int mode) {
+    ENG(e);
+    if (mode != sop::SOP_CHOL_BF16X3 && mode != sop::SOP_CHOL_F32) return sop::fail(SOP_E_ARG, "unknown mode");
+    E.set_exact_mode(mode);
+    return 0;
+}
+
+int sop_engine_get_hi(sop_engine* e, int job_id, double* epsilon, float* beta) {
+    ENG(e);
+    return E.get_hi(job_id, epsilon, beta);
+}
+
+int sop_engine_get_hi_factor(sop_engine* e, int job_id, float* L) {
+    ENG(e);
+    if (!L) return sop::fail(SOP_E_ARG, "null output");
+    return E.get_hi_factor(job_id, L);
+}
+
+int sop_chol_factor(int device, int n, const float* A, int mode, float* L, int repeats, double* ms_per_factor) {
+    if (n <= 0 || !A || repeats < 0) return sop::fail(SOP_E_ARG, "bad arguments");
+    if (mode != sop::SOP_CHOL_BF16X3 && mode != sop::SOP_CHOL_F32) return sop::fail(SOP_E_ARG, "unknown mode");
+    if (hipSetDevice(device) != hipSuccess) return sop::fail(SOP_E_HIP, "hipSetDevice failed");
+    const size_t mp = ((size_t)n + 63) / 64 * 64;
+    std::vector<float> h(mp * mp, 0.0f);
+    for (size_t r = 0; r < (size_t)n; ++r)
+        for (size_t c = 0; c <= r; ++c) h[r * mp + c] = A[r * n + c];
+    for (size_t r = n; r < mp; ++r) h[r * mp + r] = 1.0f;
+    float *dA = nullptr, *dM = nullptr;
+    double* linv = nullptr;
+    unsigned short *phi = nullptr, *plo = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 0;
+    hipError_t err = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (err == hipSuccess) err = hipMalloc(&dA, mp * mp * 4);
+    if (err == hipSuccess) err = hipMalloc(&dM, mp * mp * 4);
+    if (err == hipSuccess) err = hipMalloc(&linv, 64 * 64 * 8);
+    if (err == hipSuccess) err = hipMalloc(&phi, mp * 64 * 2);
+    if (err == hipSuccess) err = hipMalloc(&plo, mp * 64 * 2);
+    if (err == hipSuccess) err = hipEventCreate(&e0);
+    if (err == hipSuccess) err = hipEventCreate(&e1);
+    if (err == hipSuccess) err = hipMemcpy(dA, h.data(), mp * mp * 4, hipMemcpyHostToDevice);
+    double total = 0.0;
+    const int reps = repeats > 0 ? repeats : 1;
+    for (int r = 0; r < reps && err == hipSuccess; ++r) {
+        err = hipMemcpyAsync(dM, dA, mp * mp * 4, hipMemcpyDeviceToDevice, s);
+        if (err == hipSuccess) err = hipEventRecord(e0, s);
+        if (err == hipSuccess) err = sop::launch_chol_factor(dM, mp, (int)mp, mode, linv, phi, plo, nullptr, s);
+        if (err == hipSuccess) err = hipEventRecord(e1, s);
+        if (err == hipSuccess) err = hipEventSynchronize(e1);
+        float ms = 0.0f;
+        if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+        total += ms;
+    }
+    if (err == hipSuccess && L) {
+        err = hipMemcpy(h.data(), dM, mp * mp * 4, hipMemcpyDeviceToHost);
+        for (size_t r = 0; r < (size_t)n; ++r)
+            for (size_t c = 0; c < (size_t)n; ++c) L[r * n + c] = c <= r ? h[r * mp + c] : 0.0f;
+    }
+    if (err != hipSuccess) rc = sop::fail(SOP_E_HIP, std::string("cholesky: ") + hipGetErrorString(err));
+    if (ms_per_factor) *ms_per_factor = total / reps;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(dA);
+    (void)hipFree(dM);
+    (void)hipFree(linv);
+    (void)hipFree(phi);
+    (void)hipFree(plo);
+    if (s) (void)hipStreamDestroy(s);
+    return rc;
 }
 
 int sop_generate_seeds(int seed, int n, uint32_t* out4) {

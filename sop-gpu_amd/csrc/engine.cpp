@@ -246,7 +246,7 @@ static int build_job(const sop_params& p, const sop_system& s, long long numstep
     b.pp_j.assign((size_t)b.pp_cap * n, 0);
     for (int i = 0; i < n; ++i) {
         b.pp_cnt[i] = (int)pp[i].size();
-        for (size_t k = 0; k < pp[i].size(); ++k) b.pp_j[k * n + i] = pp[i][k];
+        for (size_t k = 0; k < pp[i].size(); ++k) b.pp_j[(size_t)i * b.pp_cap + k] = pp[i][k];
     }
     // coordinates, w = 0 (gsop.cu:196)
     b.coord.resize(n);
@@ -365,6 +365,24 @@ static int build_job(const sop_params& p, const sop_system& s, long long numstep
         norm = sqrtf(J.micaN[0] * J.micaN[0] + J.micaN[1] * J.micaN[1] + J.micaN[2] * J.micaN[2]);
         for (int d = 0; d < 3; ++d) J.micaN[d] /= norm;
     }
+    // hydrodynamics (bdhitea.cu:30-100)
+    if (p.hi_on) {
+        if (p.minimization)
+            return fail(SOP_E_CONFIG, "Using minimization with BDHI integrator is not supported and will never be.");
+        J.flags |= JF_HI;
+        J.hi_a = p.hi_a;
+        if (p.hi_exact) {
+            J.flags |= JF_HI_EXACT;
+            if (p.hi_faithful_blocks) J.flags |= JF_HI_FAITHFUL;
+            J.hi_chol_freq = p.hi_chol_freq > 0 ? p.hi_chol_freq : 1;
+        } else {
+            if (p.hi_epsilon_freq <= 0) return fail(SOP_E_CONFIG, "hi_epsilon_freq must be positive");
+            J.hi_eps_freq = p.hi_epsilon_freq;
+            J.hi_capricious = p.hi_capricious;
+            J.hi_epsmax = p.hi_epsmax;
+            if (!p.hi_unlisted) J.flags |= JF_HI_LISTED;
+        }
+    }
     // stride = min(nav, every created updater's frequency) (gsop.cu:116-122)
     int st = p.nav;
     auto mn = [&st](int f) { st = std::min(st, f); };
@@ -375,6 +393,7 @@ static int build_job(const sop_params& p, const sop_system& s, long long numstep
     mn(p.pairs_freq);
     mn(p.dcdfreq);
     if (p.heating) mn(p.tempFreq);
+    if (p.hi_on && !p.hi_exact) mn(p.hi_epsilon_freq);
     J.stride = st;
     if (numsteps < 0) return fail(SOP_E_ARG, "numsteps must be >= 0");
     // the outer loop runs i = 0 .. numsteps/stride inclusive (gsop.cu:129)
@@ -405,6 +424,14 @@ static unsigned long long built_bytes(const BuiltJob& b, bool pulling, bool inde
     t += round_up(n * b.pair_cap * 4) + round_up(n * b.pp_cap * 4) + round_up(n * b.excl_cap * 4);
     t += round_up(std::max<size_t>(1, b.pulled.size()) * 4);
     t += round_up(n * 12);                // energies
+    t += round_up(n * b.pair_cap * 4);    // Verlet scratch
+    if (b.dj.flags & JF_HI) {
+        t += 3 * round_up(n * 16) + round_up(n * 4);  // rforce, mforce, ci, eps
+        if (b.dj.flags & JF_HI_EXACT) {
+            const unsigned long long mp = (3 * n + 63) / 64 * 64;
+            t += round_up(n * 16) + round_up(mp * mp * 4) + round_up(64 * 64 * 8) + 2 * round_up(mp * 64 * 2);
+        }
+    }
     return t;
 }
 
@@ -455,6 +482,28 @@ int Engine::add_job(const sop_params& p, const sop_system& s, long long numsteps
     J.pulled = (int*)A(std::max<size_t>(1, b.pulled.size()) * 4);
     J.npulled = (int)b.pulled.size();
     J.energies = (float*)A(n * 12);
+    J.pair_tmp = (int*)A(n * b.pair_cap * 4);
+    HiExactWork hw{};
+    if (J.flags & JF_HI) {
+        J.rforce = (float4*)A(n * 16);
+        J.mforce = (float4*)A(n * 16);
+        J.ci = (float4*)A(n * 16);
+        J.eps = (float*)A(n * 4);
+        if (J.flags & JF_HI_EXACT) {
+            J.fdf = (float4*)A(n * 16);
+            const size_t mp = (3 * n + 63) / 64 * 64;
+            J.chol_ld = mp;
+            J.chol_M = (float*)A(mp * mp * 4);
+            hw.M = J.chol_M;
+            hw.ld = mp;
+            hw.mp = (int)mp;
+            hw.mode = exact_mode_;
+            hw.linv = (double*)A(64 * 64 * 8);
+            hw.phi = (unsigned short*)A(mp * 64 * 2);
+            hw.plo = (unsigned short*)A(mp * 64 * 2);
+            hw.chol_freq = J.hi_chol_freq;
+        }
+    }
     if (oom) {
         for (void* q : hj.blocks) arena_.release(q);
         return fail(SOP_E_MEM, "HBM budget exhausted admitting a job of " + std::to_string(b.n) + " beads");
@@ -483,12 +532,23 @@ int Engine::add_job(const sop_params& p, const sop_system& s, long long numsteps
     H2D(J.excl_cnt, b.excl_cnt.data(), nbytes(b.excl_cnt));
     H2D(J.excl_j, b.excl_j.data(), nbytes(b.excl_j));
     if (!b.pulled.empty()) H2D(J.pulled, b.pulled.data(), nbytes(b.pulled));
+    if (J.chol_M) {
+        // RPY matrix storage: the padding beyond 3n is an identity block the
+        // factorisation leaves in place
+        const size_t mp = J.chol_ld;
+        if (!err) err = check(hipMemsetAsync(J.chol_M, 0, mp * mp * 4, s0), "matrix clear");
+        std::vector<float> one(1, 1.0f);
+        for (size_t r = 3 * n; r < mp && !err; ++r)
+            err = check(hipMemcpyAsync(J.chol_M + r * mp + r, one.data(), 4, hipMemcpyHostToDevice, s0), "pad");
+        if (!err) err = check(hipStreamSynchronize(s0), "pad sync");
+    }
     if (!err) err = check(hipStreamSynchronize(s0), "upload sync");
     if (err) {
         for (void* q : hj.blocks) arena_.release(q);
         return err;
     }
     hj.id = next_id_++;
+    hj.hw = hw;
     hj.dj = J;
     hj.st0 = b.st;
     hj.step = 0;
@@ -558,10 +618,17 @@ int Engine::sync_tables() {
         dj[k] = jobs_[k].dj;
         ds[k] = jobs_[k].st0;
         ds[k].step = jobs_[k].step;
+        jobs_[k].tile0 = (int)tiles.size();
         for (int t = 0; t < jobs_[k].n; t += 64) tiles.push_back(make_int2(k, t));
+        jobs_[k].ntile = (int)tiles.size() - jobs_[k].tile0;
         if (k == 0 || jobs_[k].dj.stride) g = std::gcd(g, jobs_[k].dj.stride);
     }
     L_ = g > 0 ? g : 1;
+    any_tea_ = any_exact_ = false;
+    for (auto& j : jobs_) {
+        if ((j.dj.flags & JF_HI) && !(j.dj.flags & JF_HI_EXACT)) any_tea_ = true;
+        if (j.dj.flags & JF_HI_EXACT) any_exact_ = true;
+    }
     ntiles_ = (int)tiles.size();
     if (ntiles_ > cap_tiles_) {
         int cap = std::max(256, ntiles_ * 2);
@@ -647,6 +714,11 @@ int Engine::launch_block(int slot_only) {
     if (int rc = check(launch_job_update(jobs, st, nj, stream_), "job update")) return rc;
     if (int rc = check(launch_possible(jobs, st, tiles, nt, 0, stream_), "possible pairs")) return rc;
     if (int rc = check(launch_verlet(jobs, st, tiles, nt, 0, stream_), "verlet")) return rc;
+    const bool tea = slot_only >= 0 ? ((jobs_[slot_only].dj.flags & JF_HI) && !(jobs_[slot_only].dj.flags & JF_HI_EXACT))
+                                    : any_tea_;
+    const bool exact = slot_only >= 0 ? (jobs_[slot_only].dj.flags & JF_HI_EXACT) != 0 : any_exact_;
+    if (tea)
+        if (int rc = check(launch_hi_tea_update(jobs, st, tiles, nt, nj, stream_), "TEA update")) return rc;
     for (int k = 0; k < L; ++k) {
         const bool prof = profiling_;
         if (prof) {
@@ -665,6 +737,24 @@ int Engine::launch_block(int slot_only) {
         if (prof) {
             (void)hipEventRecord(ev_pool_[ev_used_].second, stream_);
             ev_used_++;
+        }
+        if (tea)
+            if (int rc = check(launch_hi_tea_step(jobs, st, tiles, nt, k, stream_), "TEA step")) return rc;
+        if (exact) {
+            for (int q = 0; q < (int)jobs_.size(); ++q) {
+                if (slot_only >= 0 && q != slot_only) continue;
+                const HostJob& hj = jobs_[q];
+                if (!(hj.dj.flags & JF_HI_EXACT)) continue;
+                const int2* jt = slot_only >= 0 ? tiles : d_tiles_ + hj.tile0;
+                const int jn = slot_only >= 0 ? nt : hj.ntile;
+                const DevJob* tj = slot_only >= 0 ? jobs : d_jobs_;
+                const DevJobState* ts = slot_only >= 0 ? st : d_state_;
+                const long long sk = hj.step + k;
+                const bool refactor = sk < hj.end_step && sk % hj.hw.chol_freq == 0;
+                if (int rc = check(launch_chol_step(tj, ts, jt, jn, k, refactor ? &hj.hw : nullptr, stream_),
+                                   "Cholesky step"))
+                    return rc;
+            }
         }
     }
     if (int rc = check(launch_job_advance(jobs, st, nj, L, stream_), "advance")) return rc;
@@ -717,7 +807,9 @@ int Engine::run_blocks(int nblocks) {
     if (hipSetDevice(device_) != hipSuccess) return fail(SOP_E_HIP, "hipSetDevice failed");
     if (int rc = sync_tables()) return rc;
     if (jobs_.empty()) return 0;
-    const bool graph = use_graph_ && !profiling_;
+    // exact-HI factorisations are launched on the steps the host knows are
+    // refactorisation steps, so those blocks are not replayed from a graph
+    const bool graph = use_graph_ && !profiling_ && !any_exact_;
     if (graph && !graph_) {
         if (int rc = capture_graph()) return rc;
     }
@@ -848,7 +940,38 @@ int Engine::get_list(int id, int which, int* counts, int* entries) {
     default: return fail(SOP_E_ARG, "unknown list");
     }
     if (int rc = check(hipMemcpy(counts, c, n * 4, hipMemcpyDeviceToHost), "list counts")) return rc;
-    return check(hipMemcpy(entries, e, n * cap * 4, hipMemcpyDeviceToHost), "list entries");
+    if (which != 3) return check(hipMemcpy(entries, e, n * cap * 4, hipMemcpyDeviceToHost), "list entries");
+    // the possible list is kept bead-major on the device
+    std::vector<int> rows(n * cap);
+    if (int rc = check(hipMemcpy(rows.data(), e, n * cap * 4, hipMemcpyDeviceToHost), "list entries")) return rc;
+    for (size_t i = 0; i < n; ++i)
+        for (int k = 0; k < cap; ++k) entries[(size_t)k * n + i] = rows[i * cap + k];
+    return 0;
+}
+
+int Engine::get_hi(int id, double* eps, float* beta) {
+    int slot = slot_of(id);
+    if (slot < 0) return fail(SOP_E_ARG, "no such job");
+    DevJobState s;
+    if (int rc = read_state(slot, &s)) return rc;
+    if (eps) *eps = s.eps_traj;
+    if (beta) *beta = s.beta;
+    return 0;
+}
+
+// The factor's leading 3n x 3n block (lower triangle; upper set to zero).
+int Engine::get_hi_factor(int id, float* L) {
+    HostJob* hj = find(id);
+    if (!hj) return fail(SOP_E_ARG, "no such job");
+    if (!hj->dj.chol_M) return fail(SOP_E_STATE, "job has no Cholesky factor (hi_exact off)");
+    if (int rc = synchronize()) return rc;
+    const size_t m = 3 * (size_t)hj->n, ld = hj->dj.chol_ld;
+    std::vector<float> M(ld * ld);
+    if (int rc = check(hipMemcpy(M.data(), hj->dj.chol_M, ld * ld * 4, hipMemcpyDeviceToHost), "factor read"))
+        return rc;
+    for (size_t r = 0; r < m; ++r)
+        for (size_t c = 0; c < m; ++c) L[r * m + c] = c <= r ? M[r * ld + c] : 0.0f;
+    return 0;
 }
 
 int Engine::compute_forces(int id, float* out4) {

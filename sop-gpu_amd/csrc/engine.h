@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../include/sopgpu.h"
+#include "hi.h"
 #include "jobdesc.h"
 
 namespace sop {
@@ -56,6 +57,8 @@ struct HostJob {
     long long end_step = 0;
     std::vector<void*> blocks;
     unsigned long long bytes = 0;
+    HiExactWork hw{};      // exact-HI factorisation workspace (hw.M null otherwise)
+    int tile0 = 0, ntile = 0;  // the job's range in the tile table
 };
 
 class Engine {
@@ -90,6 +93,9 @@ class Engine {
     int list_stats(double* avg4, long long* beads);
 
     void set_tuning(int max_blocks, int use_graph);
+    void set_exact_mode(int mode) { exact_mode_ = mode; }
+    int get_hi(int id, double* eps, float* beta);
+    int get_hi_factor(int id, float* L);
     int autotune(int* best);
     void profile(bool on);
     void profile_read(long long* launches, double* ms);
@@ -126,6 +132,8 @@ class Engine {
     int tiles_for_slot(int slot, int* nt);
     // tuning
     int max_blocks_ = 0;
+    int exact_mode_ = SOP_CHOL_BF16X3;
+    bool any_tea_ = false, any_exact_ = false;
     bool use_graph_ = true;
     hipGraphExec_t graph_ = nullptr;
     // profiling

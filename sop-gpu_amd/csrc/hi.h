@@ -1,0 +1,36 @@
+// hi.h -- launchers of the hydrodynamics kernels (hi.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "jobdesc.h"
+
+namespace sop {
+
+enum { SOP_CHOL_BF16X3 = 0, SOP_CHOL_F32 = 1 };
+
+// Device workspace of one exact-HI job's blocked factorisation.
+struct HiExactWork {
+    float* M;
+    size_t ld;
+    int mp;                // padded order (multiple of 64)
+    int mode;              // SOP_CHOL_*
+    double* linv;          // 64x64 inverse of the current diagonal block
+    unsigned short* phi;   // split-BF16 panel, mp x 64
+    unsigned short* plo;
+    int chol_freq;         // refactorisation interval in steps
+};
+
+hipError_t launch_hi_tea_update(const DevJob* jobs, DevJobState* st, const int2* tiles, int ntiles, int njobs,
+                                hipStream_t stream);
+hipError_t launch_hi_tea_step(const DevJob* jobs, const DevJobState* st, const int2* tiles, int ntiles, int kstep,
+                              hipStream_t stream);
+// D and D.F for the step, the factorisation (when w is given: the caller
+// knows the step is a refactorisation step), then the noise and the Euler
+// step, for the exact-HI job the tiles belong to.
+hipError_t launch_chol_step(const DevJob* jobs, const DevJobState* st, const int2* tiles, int ntiles, int kstep,
+                            const HiExactWork* w, hipStream_t stream);
+// Standalone factorisation of M (mp x mp, row-major, lower triangle used).
+hipError_t launch_chol_factor(float* M, size_t ld, int mp, int mode, double* linv, unsigned short* phi,
+                              unsigned short* plo, const int* flag, hipStream_t stream);
+
+}  // namespace sop

@@ -20,6 +20,10 @@ enum JobFlags : int {
     JF_INDENTATION = 8,
     JF_HEATING = 16,
     JF_EXTRA_FORCE = 32, // forces accumulated by non-built-in potentials
+    JF_HI = 64,          // hydrodynamic interactions (bdhitea)
+    JF_HI_EXACT = 128,   // Cholesky path instead of TEA
+    JF_HI_LISTED = 256,  // TEA over the covalent/native/Verlet lists only
+    JF_HI_FAITHFUL = 512 // Cholesky noise applied block-symmetrised, as the reference reads L
 };
 
 struct DevJob {
@@ -41,7 +45,8 @@ struct DevJob {
     int* pair_cnt;
     int* pair_j;
     int* pp_cnt;
-    int* pp_j;
+    int* pp_j;      // bead-major: entry k of bead i at [i*pp_cap + k]
+    int* pair_tmp;  // bead-major scratch of the Verlet build
     int* excl_cnt;  // sorted covalent+native partners (possible-pair exclusions)
     int* excl_j;
     int* pulled;    // bead indices of pulled beads
@@ -70,6 +75,16 @@ struct DevJob {
     float dir[3], chip0[3], micaR0[3], micaN[3];
     int moveSurface, fixTrans;
     long long retraction;
+    // hydrodynamics
+    float4* rforce;  // scaled noise of the step
+    float4* mforce;  // forces (+ pulling) of the step
+    float4* ci;      // TEA row sums of (D_ij/D_ii)^2, w: sum of D_ij/D_ii
+    float* eps;
+    float4* fdf;     // F + D F (exact path)
+    float* chol_M;   // RPY matrix / its factor, row-major, chol_ld x chol_ld
+    size_t chol_ld;
+    float hi_a, hi_epsmax;
+    int hi_eps_freq, hi_capricious, hi_chol_freq;
 };
 
 struct DevJobState {
@@ -81,7 +96,8 @@ struct DevJobState {
     float tip[3], chip[3], micaR[3];
     float V, dx;
     float tipFav[4];  // last averaged tip force (x,y,z,|f|)
-    int pad;
+    float beta;       // TEA beta' (bdhitea.cu:162-170)
+    double eps_traj;  // TEA epsilon of the last update
 };
 
 }  // namespace sop

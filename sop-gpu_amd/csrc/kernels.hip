@@ -98,43 +98,80 @@ __global__ __launch_bounds__(256) void k_step(const DevJob* __restrict__ jobs, c
                 fz += w * z;
             }
         }
-        // native LJ 12-6 (native_kernel.cu:36-49)
+        // native LJ 12-6 (native_kernel.cu:36-49), gathers four ahead
         {
             const int nn = J.nat_cnt[i];
             const int* __restrict__ nj = J.nat_j + i;
-            const float2* __restrict__ np = J.nat_par + i;
-            for (int k = 0; k < nn; ++k) {
-                const int j = nj[(size_t)k * n];
-                const float2 par = np[(size_t)k * n];
-                const float4 o = cin[j];
-                const float x = o.x - c.x, y = o.y - c.y, z = o.z - c.z;
-                float q = par.x / (x * x + y * y + z * z);
-                const float p6 = q * q * q;
-                q = q * (p6 * p6 - p6) * par.y;
-                fx += x * q;
-                fy += y * q;
-                fz += z * q;
+            const float2* __restrict__ npar = J.nat_par + i;
+#define SOP_NAT(o, par)                                                 \
+    {                                                                   \
+        const float x = (o).x - c.x, y = (o).y - c.y, z = (o).z - c.z;     \
+        float q = (par).x / (x * x + y * y + z * z);                    \
+        const float p6 = q * q * q;                                     \
+        q = q * (p6 * p6 - p6) * (par).y;                               \
+        fx += x * q;                                                    \
+        fy += y * q;                                                    \
+        fz += z * q;                                                    \
+    }
+            int k = 0;
+            for (; k + 4 <= nn; k += 4) {
+                int jj[4];
+                float2 pp[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    jj[u] = nj[(size_t)(k + u) * n];
+                    pp[u] = npar[(size_t)(k + u) * n];
+                }
+                float4 o[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) o[u] = cin[jj[u]];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) SOP_NAT(o[u], pp[u]);
             }
+            for (; k < nn; ++k) {
+                const float2 par = npar[(size_t)k * n];
+                const float4 o = cin[nj[(size_t)k * n]];
+                SOP_NAT(o, par);
+            }
+#undef SOP_NAT
         }
-        // repulsive LJ over the Verlet list (pairs_kernel.cu:41-56)
+        // repulsive LJ over the Verlet list (pairs_kernel.cu:41-56).  Indices
+        // and coordinates are fetched eight entries ahead of their use so
+        // each lane has eight independent gathers in flight; the sum is
+        // still formed in list order.
         {
             const int np = J.pair_cnt[i];
             const int* __restrict__ pj = J.pair_j + i;
             const float rc2 = J.rc2, a2 = J.a2, m6 = J.m6;
-            for (int k = 0; k < np; ++k) {
-                const int j = pj[(size_t)k * n];
-                const float4 o = cin[j];
-                const float x = o.x - c.x, y = o.y - c.y, z = o.z - c.z;
-                const float d2 = x * x + y * y + z * z;
-                if (d2 < rc2) {
-                    float q = a2 / d2;
-                    q = q * q;
-                    q = m6 * q * q;
-                    fx += x * q;
-                    fy += y * q;
-                    fz += z * q;
-                }
+#define SOP_PAIR(o)                                                 \
+    {                                                               \
+        const float x = (o).x - c.x, y = (o).y - c.y, z = (o).z - c.z; \
+        const float d2 = x * x + y * y + z * z;                     \
+        if (d2 < rc2) {                                             \
+            float q = a2 / d2;                                      \
+            q = q * q;                                              \
+            q = m6 * q * q;                                         \
+            fx += x * q;                                            \
+            fy += y * q;                                            \
+            fz += z * q;                                            \
+        }                                                           \
+    }
+            int k = 0;
+            for (; k + 8 <= np; k += 8) {
+                int jj[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) jj[u] = pj[(size_t)(k + u) * n];
+                float4 o[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) o[u] = cin[jj[u]];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) SOP_PAIR(o[u]);
             }
+            for (; k < np; ++k) {
+                const float4 o = cin[pj[(size_t)k * n]];
+                SOP_PAIR(o);
+            }
+#undef SOP_PAIR
         }
         const DevJobState& S = st[td.x];
         // indentation tip and substrate plane (indentation_kernel.cu:71-128)
@@ -165,6 +202,27 @@ __global__ __launch_bounds__(256) void k_step(const DevJob* __restrict__ jobs, c
             fx += J.micaN[0] * sw + dfx;
             fy += J.micaN[1] * sw + dfy;
             fz += J.micaN[2] * sw + dfz;
+        }
+        // hydrodynamic jobs: noise and forces for the HI integrator
+        // (integrateTea_prepare, bdhitea_kernel.cu:33-65); the update itself
+        // runs in hi.hip once every bead's forces are known
+        if (J.flags & JF_HI) {
+            uint4 rs = J.rng[i];
+            const float var = S.var;
+            const float3 g = gauss3(rs);
+            J.rng[i] = rs;
+            J.rforce[i] = make_float4(g.x * var, g.y * var, g.z * var, 0.0f);
+            if (J.flags & JF_PULLING) {
+                const float4 e = J.ext[i];
+                if (e.w == 1.0f) {
+                    fx = 0.0f; fy = 0.0f; fz = 0.0f;
+                }
+                if (e.w == 2.0f) {
+                    fx += e.x; fy += e.y; fz += e.z;
+                }
+            }
+            J.mforce[i] = make_float4(fx, fy, fz, 0.0f);
+            continue;
         }
         // Langevin step (langevin_kernel.cu:24-56)
         if (!(J.flags & JF_MINIMIZATION)) {
@@ -207,6 +265,15 @@ __device__ __forceinline__ bool due(const DevJob& J, long long s, int freq) {
     return s < J.end_step && (s % J.stride == 0) && (s % freq == 0);
 }
 
+__device__ __forceinline__ unsigned long long lanes_below(int lane) {
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// Possible pairs, one wavefront per bead: lanes test 64 consecutive j at a
+// time; an order-preserving ballot/popcount compaction appends the accepted
+// j to the bead's row (bead-major, pp_j[i*pp_cap + k]).  Exclusions are the
+// bead's sorted covalent/native partners, turned into a 64-bit mask per
+// chunk by a scalar walk.
 __global__ __launch_bounds__(256) void k_possible(const DevJob* __restrict__ jobs, DevJobState* __restrict__ st,
                                                   const int2* __restrict__ tiles, int ntiles, int force) {
     const int lane = threadIdx.x & 63;
@@ -217,40 +284,50 @@ __global__ __launch_bounds__(256) void k_possible(const DevJob* __restrict__ job
         const long long s = st[td.x].step;
         if (!force && !due(J, s, J.pp_freq)) continue;
         const int n = J.n;
-        const int i = td.y + lane;
-        const bool live = i < n;
-        const int ii = live ? i : n - 1;
         const float4* __restrict__ cur = J.coord[s & 1];
-        const float4 r1 = cur[ii];
         const float thr = J.pp_thr;
         const int cap = J.pp_cap;
-        const int ne = J.excl_cnt[ii];
-        int pe = 0;
-        int nextx = ne > 0 ? J.excl_j[ii] : INT_MAX;
-        int cnt = 0;
-        int* __restrict__ out = J.pp_j + ii;
-        for (int j = 0; j < n; ++j) {
-            const float4 r2 = cur[j];
-            const float x = r2.x - r1.x, y = r2.y - r1.y, z = r2.z - r1.z;
-            const float d = sqrtf(x * x + y * y + z * z);
-            bool ok = (d < thr) && (j != ii);
-            if (j == nextx) {
-                ok = false;
-                ++pe;
-                nextx = pe < ne ? J.excl_j[(size_t)pe * n + ii] : INT_MAX;
+        const int iend = min(td.y + 64, n);
+        for (int i = td.y; i < iend; ++i) {
+            const float4 r1 = cur[i];
+            const int ne = J.excl_cnt[i];
+            int pe = 0;
+            int cnt = 0;
+            int* __restrict__ out = J.pp_j + (size_t)i * cap;
+            for (int base = 0; base < n; base += 64) {
+                const int j = base + lane;
+                bool ok = j < n && j != i;
+                if (ok) {
+                    const float4 r2 = cur[j];
+                    const float x = r2.x - r1.x, y = r2.y - r1.y, z = r2.z - r1.z;
+                    ok = sqrtf(x * x + y * y + z * z) < thr;
+                }
+                unsigned long long ex = 0ull;
+                while (pe < ne) {
+                    const int e = J.excl_j[(size_t)pe * n + i];
+                    if (e >= base + 64) break;
+                    ex |= 1ull << (e - base);
+                    ++pe;
+                }
+                ok = ok && !((ex >> lane) & 1ull);
+                const unsigned long long m = __ballot(ok);
+                if (ok) {
+                    const int pos = cnt + __popcll(m & lanes_below(lane));
+                    if (pos < cap) out[pos] = j;
+                }
+                cnt += __popcll(m);
             }
-            if (ok && live) {
-                if (cnt < cap) out[(size_t)cnt * n] = j;
-                ++cnt;
+            if (lane == 0) {
+                J.pp_cnt[i] = cnt < cap ? cnt : cap;
+                if (cnt > cap) atomicOr(&st[td.x].overflow, 1);
             }
-        }
-        if (live) {
-            J.pp_cnt[i] = cnt < cap ? cnt : cap;
-            if (cnt > cap) atomicOr(&st[td.x].overflow, 1);
         }
     }
 }
 
+// Verlet list, one wavefront per bead: the bead's possible row filtered in
+// stored order by the same ballot compaction into a bead-major scratch row;
+// k_pairs_transpose then lays the rows out slot-major for k_step.
 __global__ __launch_bounds__(256) void k_verlet(const DevJob* __restrict__ jobs, DevJobState* __restrict__ st,
                                                 const int2* __restrict__ tiles, int ntiles, int force) {
     const int lane = threadIdx.x & 63;
@@ -261,28 +338,72 @@ __global__ __launch_bounds__(256) void k_verlet(const DevJob* __restrict__ jobs,
         const long long s = st[td.x].step;
         if (!force && !due(J, s, J.pairs_freq)) continue;
         const int n = J.n;
-        const int i = td.y + lane;
-        if (i >= n) continue;
         const float4* __restrict__ cur = J.coord[s & 1];
-        const float4 r1 = cur[i];
         const float cut = J.pl_cut;
         const int cap = J.pair_cap;
-        const int npp = J.pp_cnt[i];
-        const int* __restrict__ in = J.pp_j + i;
-        int* __restrict__ out = J.pair_j + i;
-        int cnt = 0;
-        for (int k = 0; k < npp; ++k) {
-            const int j = in[(size_t)k * n];
-            const float4 r2 = cur[j];
-            const float x = r2.x - r1.x, y = r2.y - r1.y, z = r2.z - r1.z;
-            const float d = sqrtf(x * x + y * y + z * z);
-            if (d < cut) {
-                if (cnt < cap) out[(size_t)cnt * n] = j;
-                ++cnt;
+        const int iend = min(td.y + 64, n);
+        for (int i = td.y; i < iend; ++i) {
+            const float4 r1 = cur[i];
+            const int npp = J.pp_cnt[i];
+            const int* __restrict__ in = J.pp_j + (size_t)i * J.pp_cap;
+            int* __restrict__ out = J.pair_tmp + (size_t)i * cap;
+            int cnt = 0;
+            for (int base = 0; base < npp; base += 64) {
+                const int k = base + lane;
+                bool ok = k < npp;
+                int j = 0;
+                if (ok) {
+                    j = in[k];
+                    const float4 r2 = cur[j];
+                    const float x = r2.x - r1.x, y = r2.y - r1.y, z = r2.z - r1.z;
+                    ok = sqrtf(x * x + y * y + z * z) < cut;
+                }
+                const unsigned long long m = __ballot(ok);
+                if (ok) {
+                    const int pos = cnt + __popcll(m & lanes_below(lane));
+                    if (pos < cap) out[pos] = j;
+                }
+                cnt += __popcll(m);
+            }
+            if (lane == 0) {
+                J.pair_cnt[i] = cnt < cap ? cnt : cap;
+                if (cnt > cap) atomicOr(&st[td.x].overflow, 2);
             }
         }
-        J.pair_cnt[i] = cnt < cap ? cnt : cap;
-        if (cnt > cap) atomicOr(&st[td.x].overflow, 2);
+    }
+}
+
+// Bead-major scratch rows -> slot-major Verlet list (pair_j[k*n + i]), one
+// 64-bead tile per 64-thread workgroup, 64x64 blocks through LDS so both the
+// reads and the writes are whole rows.
+__global__ __launch_bounds__(64) void k_pairs_transpose(const DevJob* __restrict__ jobs,
+                                                        const DevJobState* __restrict__ st,
+                                                        const int2* __restrict__ tiles, int ntiles, int force) {
+    __shared__ int buf[64][65];
+    const int lane = threadIdx.x;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int2 td = tiles[t];
+        const DevJob& J = jobs[td.x];
+        const long long s = st[td.x].step;
+        if (!force && !due(J, s, J.pairs_freq)) continue;
+        const int n = J.n;
+        const int cap = J.pair_cap;
+        const int rows = min(64, n - td.y);
+        int c = lane < rows ? J.pair_cnt[td.y + lane] : 0;
+        int maxc = c;
+        for (int off = 32; off > 0; off >>= 1) maxc = max(maxc, __shfl_xor(maxc, off));
+        for (int kc = 0; kc < maxc; kc += 64) {
+            __syncthreads();
+            for (int b = 0; b < rows; ++b) {
+                const int k = kc + lane;
+                const int cb = __shfl(c, b);
+                buf[b][lane] = k < cb ? J.pair_tmp[(size_t)(td.y + b) * cap + k] : 0;
+            }
+            __syncthreads();
+            const int kend = min(64, maxc - kc);
+            for (int k = 0; k < kend; ++k)
+                if (lane < rows && kc + k < c) J.pair_j[(size_t)(kc + k) * n + td.y + lane] = buf[lane][k];
+        }
     }
 }
 
@@ -538,6 +659,7 @@ hipError_t launch_verlet(const DevJob* jobs, DevJobState* st, const int2* tiles,
                          hipStream_t stream) {
     if (ntiles <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_verlet, dim3(grid_for(ntiles, 0)), dim3(256), 0, stream, jobs, st, tiles, ntiles, force);
+    hipLaunchKernelGGL(k_pairs_transpose, dim3(ntiles), dim3(64), 0, stream, jobs, st, tiles, ntiles, force);
     return hipGetLastError();
 }
 

@@ -60,6 +60,9 @@ PARAM_FIELDS = [
     ("indTipZeta", C.c_float),
     ("indOutputFreq", C.c_int),
     ("indRetractionStep", C.c_longlong),
+    ("hi_on", C.c_int), ("hi_exact", C.c_int), ("hi_a", C.c_float),
+    ("hi_epsilon_freq", C.c_int), ("hi_capricious", C.c_int), ("hi_unlisted", C.c_int),
+    ("hi_epsmax", C.c_float), ("hi_chol_freq", C.c_int), ("hi_faithful_blocks", C.c_int),
 ]
 
 
@@ -86,6 +89,8 @@ DEFAULTS = dict(
     indTipSigma=1.0, indSurfSigma=1.0, indTipEl=1.0, indSurfEl=1.0,
     indTipA=0.0, indTipB=1.0, indSurfA=0.0, indSurfB=1.0, indTipZeta=5000.0, indOutputFreq=1000,
     indRetractionStep=-1,
+    hi_on=0, hi_exact=0, hi_a=1.8, hi_epsilon_freq=10, hi_capricious=1, hi_unlisted=1, hi_epsmax=999.0,
+    hi_chol_freq=1, hi_faithful_blocks=1,
 )
 
 
@@ -210,6 +215,10 @@ _sig("sop_engine_rebuild_lists", C.c_int, VP, C.c_int, C.c_int)
 _sig("sop_engine_profile", C.c_int, VP, C.c_int)
 _sig("sop_engine_profile_read", C.c_int, VP, C.POINTER(C.c_longlong), C.POINTER(C.c_double))
 _sig("sop_engine_list_stats", C.c_int, VP, VP, C.POINTER(C.c_longlong))
+_sig("sop_engine_set_chol_mode", C.c_int, VP, C.c_int)
+_sig("sop_engine_get_hi", C.c_int, VP, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_float))
+_sig("sop_engine_get_hi_factor", C.c_int, VP, C.c_int, VP)
+_sig("sop_chol_factor", C.c_int, C.c_int, C.c_int, VP, C.c_int, VP, C.c_int, C.POINTER(C.c_double))
 _sig("sop_generate_seeds", C.c_int, C.c_int, C.c_int, VP)
 _sig("sop_rng_probe", C.c_int, C.c_int, VP, C.c_int, C.c_int, C.c_int, VP, VP)
 _sig("sop_synth_chain", C.c_int, C.c_int, C.c_ulonglong, C.c_float, C.c_float, C.c_float, C.POINTER(SynthStruct))
@@ -225,7 +234,8 @@ EXPORTED = [
     "sop_engine_get_temperature", "sop_engine_get_rng", "sop_engine_get_ext", "sop_engine_list_cap",
     "sop_engine_get_list", "sop_engine_get_indentation", "sop_engine_compute_forces",
     "sop_engine_compute_energies", "sop_engine_rebuild_lists", "sop_engine_profile", "sop_engine_profile_read",
-    "sop_engine_list_stats", "sop_generate_seeds", "sop_rng_probe", "sop_synth_chain", "sop_synth_shell",
+    "sop_engine_list_stats", "sop_engine_set_chol_mode", "sop_engine_get_hi", "sop_engine_get_hi_factor",
+    "sop_chol_factor", "sop_generate_seeds", "sop_rng_probe", "sop_synth_chain", "sop_synth_shell",
     "sop_free_system",
 ]
 
@@ -417,11 +427,39 @@ class Engine:
         check(lib.sop_engine_profile_read(self.h, C.byref(n), C.byref(ms)))
         return n.value, ms.value
 
+    def set_chol_mode(self, mode: int):
+        check(lib.sop_engine_set_chol_mode(self.h, mode))
+
+    def hi(self, jid: int):
+        e, b = C.c_double(), C.c_float()
+        check(lib.sop_engine_get_hi(self.h, jid, C.byref(e), C.byref(b)))
+        return e.value, b.value
+
+    def hi_factor(self, jid: int) -> np.ndarray:
+        m = 3 * self._n[jid]
+        L = np.zeros((m, m), np.float32)
+        check(lib.sop_engine_get_hi_factor(self.h, jid, _ptr(L)))
+        return L
+
     def list_stats(self):
         avg = np.zeros(4, np.float64)
         nb = C.c_longlong()
         check(lib.sop_engine_list_stats(self.h, _ptr(avg), C.byref(nb)))
         return avg, nb.value
+
+
+CHOL_BF16X3, CHOL_F32 = 0, 1
+
+
+def chol_factor(A: np.ndarray, mode: int = CHOL_BF16X3, repeats: int = 1, device: int = 0):
+    """Blocked Cholesky of a symmetric positive definite matrix on the GPU.
+    Returns (L, ms_per_factor)."""
+    A = np.ascontiguousarray(A, dtype=np.float32)
+    n = A.shape[0]
+    L = np.zeros_like(A)
+    ms = C.c_double()
+    check(lib.sop_chol_factor(device, n, _ptr(A), mode, _ptr(L), repeats, C.byref(ms)))
+    return L, ms.value
 
 
 def job_bytes(params: Params, system: System) -> int:
