@@ -64,6 +64,121 @@ def build_mix(S, njobs, seed):
     return jobs
 
 
+def serving_mix(S, njobs, seed, steps_lo, steps_hi):
+    """Request mix: 70% force-ramp chains (N log-uniform [100, 20000]), 20%
+    indentation assemblies (20-100 chains x 300 beads), 10% TEA-HI
+    force-clamp chains (N uniform [512, 4096], 100 pN)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(njobs):
+        kind = rng.choice(3, p=[0.7, 0.2, 0.1])
+        steps = int(log_uniform(rng, steps_lo, steps_hi, 1)[0])
+        if kind == 0:
+            n = int(log_uniform(rng, 100, 20000, 1)[0])
+            sysm = S.synth_chain(n, seed=seed * 7919 + k)
+            sysm.fixed = np.array([0], np.int32)
+            sysm.pulled = np.array([n - 1], np.int32)
+            p = job_params(S, n, 20000 + k, float(log_uniform(rng, 0.1, 10.0, 1)[0]), float(rng.uniform(10, 70)))
+        elif kind == 1:
+            nch = int(rng.integers(20, 101))
+            radius = math.sqrt(nch * 84.0)
+            sysm = S.synth_shell(nch, 300, seed=seed * 7919 + k, radius=radius)
+            c = sysm.coords[:, :3]
+            top, bot = float(c[:, 2].max()), float(c[:, 2].min())
+            tip_r = 50.0
+            # tip above the assembly moving down at 1 um/s, substrate plane below
+            dx = 1.0 * 1e-8 * 1000 * H_TAU * TAU_H_PS
+            p = S.make_params(kspring_cov=14.0, R_limit=2.0, pairs_cutoff=20.0, pairlist_cutoff=20.0,
+                              pairs_threshold=32.0, pairs_freq=10, possiblepairs_freq=100, timestep=H_TAU,
+                              temperature=0.6, seed=30000 + k, nav=1000, indentation=1,
+                              indTip=(0.0, 0.0, top + tip_r + 4.0), indChip=(0.0, 0.0, top + tip_r + 4.0),
+                              indDirection=(0.0, 0.0, -1.0), indSurfR0=(0.0, 0.0, bot - 4.0),
+                              indSurfN=(0.0, 0.0, 1.0), indTipR=tip_r, indKs=0.07, indDeltaX=dx,
+                              indTipZeta=5000.0)
+        else:
+            n = int(rng.integers(512, 4097))
+            sysm = S.synth_chain(n, seed=seed * 7919 + k)
+            sysm.fixed = np.array([0], np.int32)
+            sysm.pulled = np.array([n - 1], np.int32)
+            p = S.make_params(kspring_cov=14.0, R_limit=2.0, pairs_cutoff=20.0, pairlist_cutoff=20.0,
+                              pairs_threshold=32.0, pairs_freq=10, possiblepairs_freq=100, timestep=H_TAU,
+                              temperature=0.6, seed=40000 + k, nav=1000, pulling=1, fixedEnd=0, pulledEnd=n - 1,
+                              fconst=100.0 / PN_PER_KCAL_A, hi_on=1, hi_a=1.9, hi_epsilon_freq=10,
+                              hi_capricious=0)
+        out.append((("ramp", "indentation", "hi-tea")[kind], sysm, p, steps))
+    return out
+
+
+def serving_phase(S, args, capacity):
+    mix = serving_mix(S, args.serve_jobs, args.seed + 11, args.serve_steps_lo, args.serve_steps_hi)
+    work = np.array([m[1].n * (m[3] // 10 + 1) * 10 for m in mix], dtype=np.float64)
+    rate = args.serve_load * capacity / work.mean()  # jobs per second
+    rng = np.random.default_rng(args.seed + 12)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, len(mix)))
+    srv = S.Server(0, int(args.serve_budget_gb * (1 << 30)))
+    tickets = [None] * len(mix)
+    nxt = 0
+    t0 = time.perf_counter()
+    while True:
+        now = time.perf_counter() - t0
+        while nxt < len(mix) and arrivals[nxt] <= now:
+            kind, sysm, p, steps = mix[nxt]
+            tickets[nxt] = srv.submit(p, sysm, steps)
+            nxt += 1
+        running, queued = srv.pump(1)
+        if nxt == len(mix) and running == 0 and queued == 0:
+            break
+        if running == 0 and queued == 0 and nxt < len(mix):
+            time.sleep(max(0.0, min(0.01, arrivals[nxt] - (time.perf_counter() - t0))))
+    makespan = time.perf_counter() - t0
+    lat, failed = [], 0
+    for t in tickets:
+        st, ts, tb, td = srv.poll(t)
+        if st != 2:
+            failed += 1
+            continue
+        lat.append(td - ts)
+    stats = srv.stats()
+    srv.close()
+    lat = np.array(lat)
+    kinds = {k: sum(1 for m in mix if m[0] == k) for k in ("ramp", "indentation", "hi-tea")}
+    return {
+        "jobs": len(mix), "kinds": kinds, "failed": failed,
+        "steps_per_job": [args.serve_steps_lo, args.serve_steps_hi],
+        "arrival_rate_per_s": rate, "offered_load": args.serve_load,
+        "p50_latency_s": float(np.percentile(lat, 50)) if len(lat) else None,
+        "p99_latency_s": float(np.percentile(lat, 99)) if len(lat) else None,
+        "makespan_s": makespan, "bead_steps": stats["bead_steps"],
+        "bead_steps_per_s": stats["bead_steps"] / makespan,
+        "hbm_budget_gb": args.serve_budget_gb, "peak_admitted_gb": stats["peak_bytes"] / 2 ** 30,
+    }, mix
+
+
+def plain_serial_estimate(S, mix, jobs_c2):
+    """Seconds the same requests take one at a time on the plain engine:
+    per-step time a + b N fitted on three plain runs of chains from the mix."""
+    pe = S.Engine(0, 8 << 30)
+    pts = []
+    for n in (500, 5000, 20000):
+        sysm = S.synth_chain(n, seed=n)
+        sysm.fixed = np.array([0], np.int32)
+        sysm.pulled = np.array([n - 1], np.int32)
+        p = job_params(S, n, 7, 1.0, 35.0)
+        jid = pe.add_job(p, sysm, 190)
+        pe.synchronize()
+        t0 = time.perf_counter()
+        pe.run_job(jid)
+        pe.synchronize()
+        pts.append((n, (time.perf_counter() - t0) / 200))
+        pe.remove_job(jid)
+    pe.close()
+    ns = np.array([q[0] for q in pts], dtype=np.float64)
+    ts = np.array([q[1] for q in pts])
+    b, a = np.polyfit(ns, ts, 1)
+    total = sum(((m[3] // 10 + 1) * 10) * (a + b * m[1].n) for m in mix)
+    return total, {"a_ms": a * 1e3, "b_us_per_bead": b * 1e6}
+
+
 def bytes_per_bead_step(avg):
     """Algorithmic HBM bytes per bead-step of the fused step kernel:
     coordinates read+write 32, RNG state read+write 32, three list counts 12,
@@ -109,6 +224,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=60)
+    ap.add_argument("--serve-steps-lo", type=int, default=2000)
+    ap.add_argument("--serve-steps-hi", type=int, default=20000)
+    ap.add_argument("--serve-load", type=float, default=0.6)
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build"}))
@@ -122,6 +243,8 @@ def main():
     beads = sum(j[0].n for j in jobs)
     print(f"# mix: {len(jobs)} jobs, {beads} beads, built in {t_build:.1f} s", file=sys.stderr, flush=True)
 
+    copy_gbps = S.copy_bandwidth(1 << 30, 20)
+    print(f"# streaming copy: {copy_gbps:.0f} GB/s", file=sys.stderr, flush=True)
     eng = S.Engine(0, 96 << 30)
     ids = [eng.add_job(p, sysm, 100000) for sysm, p in jobs]
     L = eng.block_steps()
@@ -182,6 +305,17 @@ def main():
     pe.close()
     plain_value = plain_bs / plain_t
 
+    serving = None
+    if not args.no_serve:
+        print("# serving phase", file=sys.stderr, flush=True)
+        serving, smix = serving_phase(S, args, value)
+        serial_s, model = plain_serial_estimate(S, smix, jobs)
+        serving["plain_serial_estimate_s"] = serial_s
+        serving["plain_model"] = model
+        serving["speedup_vs_plain_serial"] = serial_s / serving["makespan_s"]
+        print(f"# serving: p50 {serving['p50_latency_s']:.3f} s p99 {serving['p99_latency_s']:.3f} s",
+              file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -212,8 +346,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic, "kernel": "sop::k_step",
                      "avg_launch_ms": avg_ms, "launches": launches, "bytes_per_bead_step": bpb,
+                     "copy_kernel_gbps": copy_gbps, "frac_of_copy": achieved / (copy_gbps * 1e9),
                      "avg_lists": {"covalent": avg_lists[0], "native": avg_lists[1], "verlet": avg_lists[2],
                                    "possible": avg_lists[3]}},
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -105,6 +105,11 @@ typedef struct sop_system {
     const int* fixed;
     int npulled;
     const int* pulled;
+    /* optional initial Hybrid Taus state, n x 4 words (NULL: the seed table
+     * of `seed`); lets replica k of a many-runs set take beads
+     * [k*n, (k+1)*n) of one seed table as the reference does
+     * (src/Integrators/langevin.cu:23-24) */
+    const uint32_t* rng_state4;
 } sop_system;
 
 typedef struct sop_engine sop_engine;
@@ -190,11 +195,44 @@ int sop_engine_get_hi_factor(sop_engine* e, int job_id, float* L);
  * limited to 3N <= 384. */
 int sop_chol_factor(int device, int n, const float* A, int mode, float* L, int repeats, double* ms_per_factor);
 
+/* Streaming copy of `bytes` (read once, written once) repeated `repeats`
+ * times: achieved HBM GB/s (read + write), the reference point for the
+ * roofline and for scaling memory counters. */
+int sop_copy_bandwidth(int device, unsigned long long bytes, int repeats, double* gbps);
+
 /* RNG (src/Integrators/ht.cu): seed table on the host with the reference's
  * ran2 stream, and a device probe that advances `count` draws per bead and
  * returns the uint32 outputs (gaussian=0) or Gaussian triples (gaussian=1). */
 int sop_generate_seeds(int seed, int n, uint32_t* out4);
 int sop_rng_probe(int device, const uint32_t* seeds4, int n, int count, int gaussian, float* out, uint32_t* state_out4);
+
+/* The sop-gpu program (src/sop-gpu.cpp:18-88, runGPU in src/gsop.cu:111-163):
+ * reads a parameter file in the reference's format, its .top topology and
+ * PDB/XYZ coordinates, runs the configured protocol (Langevin or BDHI,
+ * pulling, indentation, heating, many runs per GPU) and writes the
+ * reference's output files (energy .dat, pull/indentation .dat, DCD,
+ * restart / reference / final PDBs) and screen log.  names/values override
+ * parameters of the file.  Returns SOP_E_CONFIG with a message on a bad
+ * parameter, topology or coordinate file. */
+int sop_run_config(const char* path, int noverride, const char* const* names, const char* const* values);
+
+/* Continuous-batching request loop (no counterpart in the reference, which
+ * runs one system or identical replicas per process, src/sop-gpu.cpp:18-65).
+ * submit copies the request and queues it; pump admits queued requests in
+ * arrival order while the HBM budget has room, advances every admitted job
+ * by nblocks blocks in one step kernel per step, and retires finished jobs
+ * (coordinates copied out, memory released).  Times are seconds since the
+ * server was created; state: 0 queued, 1 running, 2 done, -1 failed. */
+typedef struct sop_server sop_server;
+int sop_server_create(int device, unsigned long long hbm_budget, sop_server** out);
+void sop_server_destroy(sop_server* s);
+int sop_server_set_tuning(sop_server* s, int max_blocks, int use_graph);
+int sop_server_submit(sop_server* s, const sop_params* p, const sop_system* sys, long long numsteps, int* ticket);
+int sop_server_pump(sop_server* s, int nblocks, int* running, int* queued);
+int sop_server_poll(sop_server* s, int ticket, int* state, double* t_submit, double* t_start, double* t_done);
+int sop_server_result(sop_server* s, int ticket, float* coords4);
+int sop_server_stats(sop_server* s, double* bead_steps, double* busy_seconds, int* admitted, int* finished,
+                     unsigned long long* peak_bytes);
 
 /* Synthetic systems: a seeded chain of n beads (3.8 A bonds, compact
  * globule) with native contacts |i-j|>2 and r<8 A, eh in [eh_lo, eh_hi].
@@ -211,12 +249,30 @@ typedef struct sop_synth {
     int* nat_j;
     float* nat_r0;
     float* nat_eh;
+    int npairs; /* [ pairs ] of a .top file (initial possible list); 0 for synthetic systems */
+    int* pair_i;
+    int* pair_j;
 } sop_synth;
 int sop_synth_chain(int n, unsigned long long seed, float density, float eh_lo, float eh_hi, sop_synth* out);
 /* nchains chains of n beads each placed on a sphere of radius R,
  * inter-chain contacts added for r < 8 A. */
 int sop_synth_shell(int nchains, int n, unsigned long long seed, float radius, float eh_lo, float eh_hi, sop_synth* out);
 void sop_free_system(sop_synth* s);
+
+/* File formats of the reference.
+ * sop_read_topology: SOP::load (src/IO/topio.cpp:36-80) of a .top file and,
+ *   when coords_path is not NULL, readCoord (src/IO/pdbio.cpp:292-337) of a
+ *   PDB (ATOM records) or .xyz file into coords4 (w = 0); free with
+ *   sop_free_system.
+ * sop_write_pdb: savePDB (src/IO/pdbio.cpp:194-219) of the .top's atoms at
+ *   coords4 (hexadecimal serials from 100000 atoms on, CONECT records of the
+ *   bonds).
+ * sop_write_dcd: a DCD file (src/IO/dcdio.cpp:100-194) with the header of
+ *   DcdOutputManager (NFILE = NPRIV = NSAVC = freq, src/Updaters/dcd_manager.cpp:48-52)
+ *   and nframes frames; frames holds, per frame, X[natoms], Y[natoms], Z[natoms]. */
+int sop_read_topology(const char* top_path, const char* coords_path, sop_synth* out);
+int sop_write_pdb(const char* top_path, const float* coords4, int n, const char* out_path);
+int sop_write_dcd(const char* path, int natoms, int nframes, const float* frames, int freq, float delta);
 
 #ifdef __cplusplus
 }

@@ -288,6 +288,34 @@ int sop_chol_factor(int device, int n, const float* A, int mode, float* L, int r
     return rc;
 }
 
+int sop_copy_bandwidth(int device, unsigned long long bytes, int repeats, double* gbps) {
+    if (bytes < 16 || repeats <= 0 || !gbps) return sop::fail(SOP_E_ARG, "bad arguments");
+    if (hipSetDevice(device) != hipSuccess) return sop::fail(SOP_E_HIP, "hipSetDevice failed");
+    const size_t n = (size_t)(bytes / 16);
+    float4 *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t err = hipMalloc(&a, n * 16);
+    if (err == hipSuccess) err = hipMalloc(&b, n * 16);
+    if (err == hipSuccess) err = hipMemset(a, 0, n * 16);
+    if (err == hipSuccess) err = hipEventCreate(&e0);
+    if (err == hipSuccess) err = hipEventCreate(&e1);
+    if (err == hipSuccess) err = sop::launch_copy(a, b, n, nullptr);  // warm-up
+    if (err == hipSuccess) err = hipEventRecord(e0, nullptr);
+    for (int r = 0; r < repeats && err == hipSuccess; ++r) err = sop::launch_copy(a, b, n, nullptr);
+    if (err == hipSuccess) err = hipEventRecord(e1, nullptr);
+    if (err == hipSuccess) err = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+    int rc = 0;
+    if (err != hipSuccess) rc = sop::fail(SOP_E_HIP, std::string("copy: ") + hipGetErrorString(err));
+    else *gbps = 2.0 * (double)n * 16 * repeats / (ms * 1e-3) / 1e9;  // bytes read + written
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return rc;
+}
+
 int sop_generate_seeds(int seed, int n, uint32_t* out4) {
     if (n < 0 || (n > 0 && !out4)) return sop::fail(SOP_E_ARG, "bad arguments");
     sop::generate_seeds(seed, n, out4);

@@ -254,11 +254,15 @@ static int build_job(const sop_params& p, const sop_system& s, long long numstep
         b.coord[i] = make_float4(s.coords4[4 * i], s.coords4[4 * i + 1], s.coords4[4 * i + 2], 0.0f);
     // RNG seeds (langevin.cu:23-24)
     b.rng.resize(n);
-    generate_seeds(p.seed, n, reinterpret_cast<uint32_t*>(b.rng.data()));
+    if (s.rng_state4)
+        std::memcpy(b.rng.data(), s.rng_state4, (size_t)n * 16);
+    else
+        generate_seeds(p.seed, n, reinterpret_cast<uint32_t*>(b.rng.data()));
 
     DevJob& J = b.dj;
     DevJobState& S = b.st;
     J.n = n;
+    J.idx16 = n <= 65535 ? 1 : 0;
     J.cov_cap = b.cov_cap;
     J.nat_cap = b.nat_cap;
     J.pair_cap = b.pair_cap;
@@ -421,10 +425,11 @@ static unsigned long long built_bytes(const BuiltJob& b, bool pulling, bool inde
     t += round_up(n * 4) * 5;             // five count arrays
     t += round_up(n * b.cov_cap * 4) * 2; // cov j, r0
     t += round_up(n * b.nat_cap * 4) + round_up(n * b.nat_cap * 8);
-    t += round_up(n * b.pair_cap * 4) + round_up(n * b.pp_cap * 4) + round_up(n * b.excl_cap * 4);
+    const unsigned long long es = b.dj.idx16 ? 2 : 4;
+    t += round_up(n * b.pair_cap * es) + round_up(n * b.pp_cap * es) + round_up(n * b.excl_cap * 4);
     t += round_up(std::max<size_t>(1, b.pulled.size()) * 4);
     t += round_up(n * 12);                // energies
-    t += round_up(n * b.pair_cap * 4);    // Verlet scratch
+    t += round_up(n * b.pair_cap * es);   // Verlet scratch
     if (b.dj.flags & JF_HI) {
         t += 3 * round_up(n * 16) + round_up(n * 4);  // rforce, mforce, ci, eps
         if (b.dj.flags & JF_HI_EXACT) {
@@ -474,15 +479,16 @@ int Engine::add_job(const sop_params& p, const sop_system& s, long long numsteps
     J.nat_j = (int*)A(n * b.nat_cap * 4);
     J.nat_par = (float2*)A(n * b.nat_cap * 8);
     J.pair_cnt = (int*)A(n * 4);
-    J.pair_j = (int*)A(n * b.pair_cap * 4);
+    const size_t es = J.idx16 ? 2 : 4;
+    J.pair_j = A(n * b.pair_cap * es);
     J.pp_cnt = (int*)A(n * 4);
-    J.pp_j = (int*)A(n * b.pp_cap * 4);
+    J.pp_j = A(n * b.pp_cap * es);
     J.excl_cnt = (int*)A(n * 4);
     J.excl_j = (int*)A(n * b.excl_cap * 4);
     J.pulled = (int*)A(std::max<size_t>(1, b.pulled.size()) * 4);
     J.npulled = (int)b.pulled.size();
     J.energies = (float*)A(n * 12);
-    J.pair_tmp = (int*)A(n * b.pair_cap * 4);
+    J.pair_tmp = A(n * b.pair_cap * es);
     HiExactWork hw{};
     if (J.flags & JF_HI) {
         J.rforce = (float4*)A(n * 16);
@@ -528,7 +534,13 @@ int Engine::add_job(const sop_params& p, const sop_system& s, long long numsteps
     H2D(J.nat_par, b.nat_par.data(), nbytes(b.nat_par));
     H2D(J.pair_cnt, zeros.data(), n * 4);
     H2D(J.pp_cnt, b.pp_cnt.data(), nbytes(b.pp_cnt));
-    H2D(J.pp_j, b.pp_j.data(), nbytes(b.pp_j));
+    std::vector<unsigned short> pp16;
+    if (J.idx16) {
+        pp16.assign(b.pp_j.begin(), b.pp_j.end());
+        H2D(J.pp_j, pp16.data(), nbytes(pp16));
+    } else {
+        H2D(J.pp_j, b.pp_j.data(), nbytes(b.pp_j));
+    }
     H2D(J.excl_cnt, b.excl_cnt.data(), nbytes(b.excl_cnt));
     H2D(J.excl_j, b.excl_j.data(), nbytes(b.excl_j));
     if (!b.pulled.empty()) H2D(J.pulled, b.pulled.data(), nbytes(b.pulled));
@@ -783,6 +795,52 @@ int Engine::tiles_for_slot(int slot, int* nt) {
     return 0;
 }
 
+int Engine::stage_job_update(int mask) {
+    if (int rc = sync_tables()) return rc;
+    return check(launch_job_update(d_jobs_, d_state_, (int)jobs_.size(), stream_, mask), "job update");
+}
+int Engine::stage_possible() {
+    if (int rc = sync_tables()) return rc;
+    return check(launch_possible(d_jobs_, d_state_, d_tiles_, ntiles_, 0, stream_), "possible pairs");
+}
+int Engine::stage_verlet() {
+    if (int rc = sync_tables()) return rc;
+    return check(launch_verlet(d_jobs_, d_state_, d_tiles_, ntiles_, 0, stream_), "verlet");
+}
+int Engine::stage_tea_update() {
+    if (int rc = sync_tables()) return rc;
+    if (!any_tea_) return 0;
+    return check(launch_hi_tea_update(d_jobs_, d_state_, d_tiles_, ntiles_, (int)jobs_.size(), stream_), "TEA update");
+}
+int Engine::stage_step(int k) {
+    if (int rc = sync_tables()) return rc;
+    if (int rc = check(launch_step(d_jobs_, d_state_, d_tiles_, ntiles_, k, max_blocks_, stream_), "step")) return rc;
+    if (any_tea_)
+        if (int rc = check(launch_hi_tea_step(d_jobs_, d_state_, d_tiles_, ntiles_, k, stream_), "TEA step")) return rc;
+    if (any_exact_)
+        for (auto& hj : jobs_) {
+            if (!(hj.dj.flags & JF_HI_EXACT)) continue;
+            const long long sk = hj.step + k;
+            const bool refactor = sk < hj.end_step && sk % hj.hw.chol_freq == 0;
+            if (int rc = check(launch_chol_step(d_jobs_, d_state_, d_tiles_ + hj.tile0, hj.ntile, k,
+                                                refactor ? &hj.hw : nullptr, stream_),
+                               "Cholesky step"))
+                return rc;
+        }
+    return 0;
+}
+int Engine::stage_advance(int nsteps) {
+    if (int rc = sync_tables()) return rc;
+    if (int rc = check(launch_job_advance(d_jobs_, d_state_, (int)jobs_.size(), nsteps, stream_), "advance")) return rc;
+    for (auto& j : jobs_) j.step = std::min(j.step + nsteps, j.end_step);
+    return 0;
+}
+int Engine::read_state_of(int id, DevJobState* out) {
+    int slot = slot_of(id);
+    if (slot < 0) return fail(SOP_E_ARG, "no such job");
+    return read_state(slot, out);
+}
+
 int Engine::capture_graph() {
     drop_graph();
     hipGraph_t g = nullptr;
@@ -930,22 +988,32 @@ int Engine::get_list(int id, int which, int* counts, int* entries) {
     const DevJob& J = hj->dj;
     const size_t n = hj->n;
     const int* c;
-    const int* e;
+    const void* e;
     int cap;
+    bool typed = false;  // list words uint16/int32 per J.idx16
     switch (which) {
     case 0: c = J.cov_cnt; e = J.cov_j; cap = J.cov_cap; break;
     case 1: c = J.nat_cnt; e = J.nat_j; cap = J.nat_cap; break;
-    case 2: c = J.pair_cnt; e = J.pair_j; cap = J.pair_cap; break;
-    case 3: c = J.pp_cnt; e = J.pp_j; cap = J.pp_cap; break;
+    case 2: c = J.pair_cnt; e = J.pair_j; cap = J.pair_cap; typed = true; break;
+    case 3: c = J.pp_cnt; e = J.pp_j; cap = J.pp_cap; typed = true; break;
     default: return fail(SOP_E_ARG, "unknown list");
     }
     if (int rc = check(hipMemcpy(counts, c, n * 4, hipMemcpyDeviceToHost), "list counts")) return rc;
-    if (which != 3) return check(hipMemcpy(entries, e, n * cap * 4, hipMemcpyDeviceToHost), "list entries");
+    std::vector<int> words(n * cap);
+    if (typed && J.idx16) {
+        std::vector<unsigned short> w16(n * cap);
+        if (int rc = check(hipMemcpy(w16.data(), e, n * cap * 2, hipMemcpyDeviceToHost), "list entries")) return rc;
+        words.assign(w16.begin(), w16.end());
+    } else {
+        if (int rc = check(hipMemcpy(words.data(), e, n * cap * 4, hipMemcpyDeviceToHost), "list entries")) return rc;
+    }
+    if (which != 3) {
+        std::memcpy(entries, words.data(), n * cap * 4);
+        return 0;
+    }
     // the possible list is kept bead-major on the device
-    std::vector<int> rows(n * cap);
-    if (int rc = check(hipMemcpy(rows.data(), e, n * cap * 4, hipMemcpyDeviceToHost), "list entries")) return rc;
     for (size_t i = 0; i < n; ++i)
-        for (int k = 0; k < cap; ++k) entries[(size_t)k * n + i] = rows[i * cap + k];
+        for (int k = 0; k < cap; ++k) entries[(size_t)k * n + i] = words[i * cap + k];
     return 0;
 }
 

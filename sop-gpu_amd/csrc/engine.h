@@ -18,6 +18,7 @@
 
 #include "../../include/sopgpu.h"
 #include "hi.h"
+#include "hostutil.h"
 #include "jobdesc.h"
 
 namespace sop {
@@ -92,6 +93,16 @@ class Engine {
     int get_indentation(int id, float* out13);
     int list_stats(double* avg4, long long* beads);
 
+    // Staged block (the reference's updater passes one at a time, over all
+    // jobs, with the same device-side cadence checks as launch_block):
+    int stage_job_update(int mask);
+    int stage_possible();
+    int stage_verlet();
+    int stage_tea_update();
+    int stage_step(int kstep);
+    int stage_advance(int nsteps);
+    int read_state_of(int id, DevJobState* out);
+
     void set_tuning(int max_blocks, int use_graph);
     void set_exact_mode(int mode) { exact_mode_ = mode; }
     int get_hi(int id, double* eps, float* beta);
@@ -146,8 +157,5 @@ class Engine {
     float4* d_scratch4_ = nullptr;
     size_t scratch4_n_ = 0;
 };
-
-void set_error(int code, const std::string& msg);
-int fail(int code, const std::string& msg);
 
 }  // namespace sop

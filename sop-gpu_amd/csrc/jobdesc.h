@@ -42,16 +42,19 @@ struct DevJob {
     int* nat_cnt;
     int* nat_j;
     float2* nat_par; // (r0^2, -12 eh / r0^2)
+    // Verlet and possible lists hold job-local bead indices as uint16 when the
+    // job has fewer than 65536 beads (idx16), as int32 otherwise
     int* pair_cnt;
-    int* pair_j;
+    void* pair_j;   // slot-major: entry k of bead i at [k*n + i]
     int* pp_cnt;
-    int* pp_j;      // bead-major: entry k of bead i at [i*pp_cap + k]
-    int* pair_tmp;  // bead-major scratch of the Verlet build
+    void* pp_j;     // bead-major: entry k of bead i at [i*pp_cap + k]
+    void* pair_tmp; // bead-major scratch of the Verlet build
     int* excl_cnt;  // sorted covalent+native partners (possible-pair exclusions)
     int* excl_j;
     int* pulled;    // bead indices of pulled beads
     float* energies; // 3n per-bead energies (output cadence)
     int cov_cap, nat_cap, pair_cap, pp_cap, excl_cap;
+    int idx16;
     int npulled;
     // force-field constants, formed as the reference's constructors do
     float ks, R2;                     // FENE: kspring_cov, R_limit^2
@@ -96,8 +99,18 @@ struct DevJobState {
     float tip[3], chip[3], micaR[3];
     float V, dx;
     float tipFav[4];  // last averaged tip force (x,y,z,|f|)
+    float tipDr;      // |chip - tip| of the last tip update, before the tip moved
     float beta;       // TEA beta' (bdhitea.cu:162-170)
     double eps_traj;  // TEA epsilon of the last update
 };
+
+// List entry k of the Verlet list (slot-major) / possible list (bead-major).
+__device__ __forceinline__ int list_at(const void* p, int idx16, size_t at) {
+    return idx16 ? (int)static_cast<const unsigned short*>(p)[at] : static_cast<const int*>(p)[at];
+}
+__device__ __forceinline__ void list_put(void* p, int idx16, size_t at, int v) {
+    if (idx16) static_cast<unsigned short*>(p)[at] = (unsigned short)v;
+    else static_cast<int*>(p)[at] = v;
+}
 
 }  // namespace sop

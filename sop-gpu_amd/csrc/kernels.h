@@ -16,12 +16,13 @@ hipError_t launch_possible(const DevJob* jobs, DevJobState* st, const int2* tile
 hipError_t launch_verlet(const DevJob* jobs, DevJobState* st, const int2* tiles, int ntiles, int force,
                          hipStream_t stream);
 // Per-job updaters at a block start, and the step counter advance.
-hipError_t launch_job_update(const DevJob* jobs, DevJobState* st, int njobs, hipStream_t stream);
+hipError_t launch_job_update(const DevJob* jobs, DevJobState* st, int njobs, hipStream_t stream, int mask = 7);
 hipError_t launch_job_advance(const DevJob* jobs, DevJobState* st, int njobs, int nsteps, hipStream_t stream);
 // Output / parity hooks.
 hipError_t launch_energies(const DevJob* jobs, const DevJobState* st, int job, int n, hipStream_t stream);
 hipError_t launch_forces(const DevJob* jobs, const DevJobState* st, int job, int n, float4* out, hipStream_t stream);
 hipError_t launch_rng_probe(uint4* state, int n, int count, int gaussian, float* out, hipStream_t stream);
 const void* step_kernel_symbol();
+hipError_t launch_copy(const float4* src, float4* dst, size_t n, hipStream_t stream);
 
 }  // namespace sop

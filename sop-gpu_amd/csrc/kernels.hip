@@ -59,6 +59,51 @@ __device__ __forceinline__ TileRange xcd_tiles(int ntiles) {
 // ---------------------------------------------------------------------------
 // fused step
 // ---------------------------------------------------------------------------
+
+// Repulsive LJ over a bead's slot-major Verlet entries, in list order.  The
+// list words are streamed with non-temporal loads (each is used once per
+// step) and fetched one batch of eight ahead: while batch b is computed the
+// indices of batch b+1 and the coordinates of batch b are in flight.
+template <typename IdxT>
+__device__ __forceinline__ void pair_loop(const IdxT* __restrict__ pj, int np, int n, const float4* __restrict__ cin,
+                                          const float4 c, float rc2, float a2, float m6, float& fx, float& fy,
+                                          float& fz) {
+#define SOP_PAIR(o)                                                  \
+    {                                                                \
+        const float x = (o).x - c.x, y = (o).y - c.y, z = (o).z - c.z; \
+        const float d2 = x * x + y * y + z * z;                      \
+        if (d2 < rc2) {                                              \
+            float q = a2 / d2;                                       \
+            q = q * q;                                               \
+            q = m6 * q * q;                                          \
+            fx += x * q;                                             \
+            fy += y * q;                                             \
+            fz += z * q;                                             \
+        }                                                            \
+    }
+    int k = 0;
+    if (np >= 8) {
+        int jn[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) jn[u] = (int)__builtin_nontemporal_load(pj + (size_t)u * n);
+        for (; k + 8 <= np; k += 8) {
+            float4 o[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) o[u] = cin[jn[u]];
+            if (k + 16 <= np) {
+#pragma unroll
+                for (int u = 0; u < 8; ++u) jn[u] = (int)__builtin_nontemporal_load(pj + (size_t)(k + 8 + u) * n);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) SOP_PAIR(o[u]);
+        }
+    }
+    for (; k < np; ++k) {
+        const float4 o = cin[(int)__builtin_nontemporal_load(pj + (size_t)k * n)];
+        SOP_PAIR(o);
+    }
+#undef SOP_PAIR
+}
 __global__ __launch_bounds__(256) void k_step(const DevJob* __restrict__ jobs, const DevJobState* __restrict__ st,
                                               const int2* __restrict__ tiles, int ntiles, int kstep) {
     const int lane = threadIdx.x & 63;
@@ -135,44 +180,13 @@ __global__ __launch_bounds__(256) void k_step(const DevJob* __restrict__ jobs, c
             }
 #undef SOP_NAT
         }
-        // repulsive LJ over the Verlet list (pairs_kernel.cu:41-56).  Indices
-        // and coordinates are fetched eight entries ahead of their use so
-        // each lane has eight independent gathers in flight; the sum is
-        // still formed in list order.
-        {
-            const int np = J.pair_cnt[i];
-            const int* __restrict__ pj = J.pair_j + i;
-            const float rc2 = J.rc2, a2 = J.a2, m6 = J.m6;
-#define SOP_PAIR(o)                                                 \
-    {                                                               \
-        const float x = (o).x - c.x, y = (o).y - c.y, z = (o).z - c.z; \
-        const float d2 = x * x + y * y + z * z;                     \
-        if (d2 < rc2) {                                             \
-            float q = a2 / d2;                                      \
-            q = q * q;                                              \
-            q = m6 * q * q;                                         \
-            fx += x * q;                                            \
-            fy += y * q;                                            \
-            fz += z * q;                                            \
-        }                                                           \
-    }
-            int k = 0;
-            for (; k + 8 <= np; k += 8) {
-                int jj[8];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) jj[u] = pj[(size_t)(k + u) * n];
-                float4 o[8];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) o[u] = cin[jj[u]];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) SOP_PAIR(o[u]);
-            }
-            for (; k < np; ++k) {
-                const float4 o = cin[pj[(size_t)k * n]];
-                SOP_PAIR(o);
-            }
-#undef SOP_PAIR
-        }
+        // repulsive LJ over the Verlet list (pairs_kernel.cu:41-56)
+        if (J.idx16)
+            pair_loop(static_cast<const unsigned short*>(J.pair_j) + i, J.pair_cnt[i], n, cin, c, J.rc2, J.a2, J.m6,
+                      fx, fy, fz);
+        else
+            pair_loop(static_cast<const int*>(J.pair_j) + i, J.pair_cnt[i], n, cin, c, J.rc2, J.a2, J.m6, fx, fy,
+                      fz);
         const DevJobState& S = st[td.x];
         // indentation tip and substrate plane (indentation_kernel.cu:71-128)
         if (J.flags & JF_INDENTATION) {
@@ -293,7 +307,7 @@ __global__ __launch_bounds__(256) void k_possible(const DevJob* __restrict__ job
             const int ne = J.excl_cnt[i];
             int pe = 0;
             int cnt = 0;
-            int* __restrict__ out = J.pp_j + (size_t)i * cap;
+            const size_t row = (size_t)i * cap;
             for (int base = 0; base < n; base += 64) {
                 const int j = base + lane;
                 bool ok = j < n && j != i;
@@ -313,7 +327,7 @@ __global__ __launch_bounds__(256) void k_possible(const DevJob* __restrict__ job
                 const unsigned long long m = __ballot(ok);
                 if (ok) {
                     const int pos = cnt + __popcll(m & lanes_below(lane));
-                    if (pos < cap) out[pos] = j;
+                    if (pos < cap) list_put(J.pp_j, J.idx16, row + pos, j);
                 }
                 cnt += __popcll(m);
             }
@@ -345,15 +359,14 @@ __global__ __launch_bounds__(256) void k_verlet(const DevJob* __restrict__ jobs,
         for (int i = td.y; i < iend; ++i) {
             const float4 r1 = cur[i];
             const int npp = J.pp_cnt[i];
-            const int* __restrict__ in = J.pp_j + (size_t)i * J.pp_cap;
-            int* __restrict__ out = J.pair_tmp + (size_t)i * cap;
+            const size_t inrow = (size_t)i * J.pp_cap, outrow = (size_t)i * cap;
             int cnt = 0;
             for (int base = 0; base < npp; base += 64) {
                 const int k = base + lane;
                 bool ok = k < npp;
                 int j = 0;
                 if (ok) {
-                    j = in[k];
+                    j = list_at(J.pp_j, J.idx16, inrow + k);
                     const float4 r2 = cur[j];
                     const float x = r2.x - r1.x, y = r2.y - r1.y, z = r2.z - r1.z;
                     ok = sqrtf(x * x + y * y + z * z) < cut;
@@ -361,7 +374,7 @@ __global__ __launch_bounds__(256) void k_verlet(const DevJob* __restrict__ jobs,
                 const unsigned long long m = __ballot(ok);
                 if (ok) {
                     const int pos = cnt + __popcll(m & lanes_below(lane));
-                    if (pos < cap) out[pos] = j;
+                    if (pos < cap) list_put(J.pair_tmp, J.idx16, outrow + pos, j);
                 }
                 cnt += __popcll(m);
             }
@@ -397,12 +410,12 @@ __global__ __launch_bounds__(64) void k_pairs_transpose(const DevJob* __restrict
             for (int b = 0; b < rows; ++b) {
                 const int k = kc + lane;
                 const int cb = __shfl(c, b);
-                buf[b][lane] = k < cb ? J.pair_tmp[(size_t)(td.y + b) * cap + k] : 0;
+                buf[b][lane] = k < cb ? list_at(J.pair_tmp, J.idx16, (size_t)(td.y + b) * cap + k) : 0;
             }
             __syncthreads();
             const int kend = min(64, maxc - kc);
             for (int k = 0; k < kend; ++k)
-                if (lane < rows && kc + k < c) J.pair_j[(size_t)(kc + k) * n + td.y + lane] = buf[lane][k];
+                if (lane < rows && kc + k < c) list_put(J.pair_j, J.idx16, (size_t)(kc + k) * n + td.y + lane, buf[lane][k]);
         }
     }
 }
@@ -439,6 +452,7 @@ __device__ void tip_update(const DevJob& J, DevJobState& S, long long s) {
         for (int d = 0; d < 3; ++d) S.chip[d] = J.chip0[d] + J.dir[d] * S.dx;
     }
     const float dr0 = S.chip[0] - S.tip[0], dr1 = S.chip[1] - S.tip[1], dr2 = S.chip[2] - S.tip[2];
+    S.tipDr = sqrtf(dr0 * dr0 + dr1 * dr1 + dr2 * dr2);
     const float df0 = -f0 + dr0 * J.indKs, df1 = -f1 + dr1 * J.indKs, df2 = -f2 + dr2 * J.indKs;
     const float mult = ((J.h * ((float)J.nav)) / (J.tipZeta));
     if (J.fixTrans) {
@@ -461,16 +475,17 @@ __device__ void pull_update(const DevJob& J, DevJobState& S, long long s) {
     for (int k = 0; k < J.npulled; ++k) J.ext[J.pulled[k]] = make_float4(S.extF[0], S.extF[1], S.extF[2], 2.0f);
 }
 
-__global__ void k_job_update(const DevJob* __restrict__ jobs, DevJobState* __restrict__ st, int njobs) {
+// mask: 1 indentation tip, 2 pulling cantilever, 4 heating schedule
+__global__ void k_job_update(const DevJob* __restrict__ jobs, DevJobState* __restrict__ st, int njobs, int mask) {
     const int jb = blockIdx.x * blockDim.x + threadIdx.x;
     if (jb >= njobs) return;
     const DevJob& J = jobs[jb];
     DevJobState& S = st[jb];
     const long long s = S.step;
     if (s >= J.end_step || s % J.stride != 0) return;
-    if (J.flags & JF_INDENTATION) tip_update(J, S, s);
-    if (J.flags & JF_PULL_UPDATER) pull_update(J, S, s);
-    if ((J.flags & JF_HEATING) && s % J.tempFreq == 0) {
+    if ((mask & 1) && (J.flags & JF_INDENTATION)) tip_update(J, S, s);
+    if ((mask & 2) && (J.flags & JF_PULL_UPDATER)) pull_update(J, S, s);
+    if ((mask & 4) && (J.flags & JF_HEATING) && s % J.tempFreq == 0) {
         const float temp = J.initialT + (float)(s / J.tempFreq) * J.deltaT;
         S.var = sqrtf(2.0f * temp * J.zeta / J.h);
     }
@@ -522,7 +537,7 @@ __global__ __launch_bounds__(256) void k_energies(const DevJob* __restrict__ job
     J.energies[n + i] = pot;
     pot = 0.0f;
     for (int k = 0; k < J.pair_cnt[i]; ++k) {
-        const int j = J.pair_j[(size_t)k * n + i];
+        const int j = list_at(J.pair_j, J.idx16, (size_t)k * n + i);
         const float4 o = cur[j];
         const float x = o.x - c.x, y = o.y - c.y, z = o.z - c.z;
         const float d2 = x * x + y * y + z * z;
@@ -571,7 +586,7 @@ __global__ __launch_bounds__(256) void k_forces(const DevJob* __restrict__ jobs,
         fz += z * q;
     }
     for (int k = 0; k < J.pair_cnt[i]; ++k) {
-        const int j = J.pair_j[(size_t)k * n + i];
+        const int j = list_at(J.pair_j, J.idx16, (size_t)k * n + i);
         const float4 o = cin[j];
         const float x = o.x - c.x, y = o.y - c.y, z = o.z - c.z;
         const float d2 = x * x + y * y + z * z;
@@ -631,9 +646,20 @@ __global__ void k_rng_probe(uint4* __restrict__ state, int n, int count, int gau
     state[i] = s;
 }
 
+// Plain streaming copy (16 B per lane per iteration): the reference point
+// for HBM bandwidth and for scaling memory counters.
+__global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) dst[i] = src[i];
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+hipError_t launch_copy(const float4* src, float4* dst, size_t n, hipStream_t stream) {
+    hipLaunchKernelGGL(k_copy, dim3(8192), dim3(256), 0, stream, src, dst, n);
+    return hipGetLastError();
+}
+
 static int grid_for(int ntiles, int max_blocks) {
     int b = (ntiles + kTilesPerBlock - 1) / kTilesPerBlock;
     if (max_blocks > 0 && b > max_blocks) b = max_blocks;
@@ -663,9 +689,9 @@ hipError_t launch_verlet(const DevJob* jobs, DevJobState* st, const int2* tiles,
     return hipGetLastError();
 }
 
-hipError_t launch_job_update(const DevJob* jobs, DevJobState* st, int njobs, hipStream_t stream) {
+hipError_t launch_job_update(const DevJob* jobs, DevJobState* st, int njobs, hipStream_t stream, int mask) {
     if (njobs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_job_update, dim3((njobs + 63) / 64), dim3(64), 0, stream, jobs, st, njobs);
+    hipLaunchKernelGGL(k_job_update, dim3((njobs + 63) / 64), dim3(64), 0, stream, jobs, st, njobs, mask);
     return hipGetLastError();
 }
 

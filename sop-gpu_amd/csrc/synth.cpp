@@ -66,6 +66,17 @@ struct Grid {
         cellof(p, a, b, c);
         m[key(a, b, c)].push_back(id);
     }
+    void remove(const V3& p, int id) {
+        int a, b, c;
+        cellof(p, a, b, c);
+        auto& v = m[key(a, b, c)];
+        for (size_t k = 0; k < v.size(); ++k)
+            if (v[k] == id) {
+                v[k] = v.back();
+                v.pop_back();
+                return;
+            }
+    }
     template <typename F>
     void near(const V3& p, F f) const {
         int a, b, c;
@@ -88,19 +99,20 @@ static double dist(const V3& a, const V3& b) {
 }
 
 // Grows one confined self-avoiding walk of n beads starting at `start`,
-// appending to pos; `inside` tells whether a point is allowed.
+// appending to pos; `inside` tells whether a point is allowed.  A bead that
+// cannot be placed makes the walk back up a few beads and try again.
 template <typename Inside>
 static bool grow(int n, const V3& start, Rng& rng, Inside inside, std::vector<V3>& pos, Grid& g) {
     const int base = (int)pos.size();
     pos.push_back(start);
     g.add(start, base);
     double slack = 1.0;
-    for (int i = 1; i < n; ++i) {
+    int backtracks = 0;
+    while ((int)pos.size() - base < n) {
         const V3 prev = pos.back();
         bool ok = false;
         V3 q{};
-        for (int attempt = 0; attempt < 4000 && !ok; ++attempt) {
-            if (attempt && attempt % 1000 == 0) slack *= 1.05;
+        for (int attempt = 0; attempt < 600 && !ok; ++attempt) {
             double u = 2.0 * rng.uni() - 1.0, ph = 2.0 * M_PI * rng.uni(), s = std::sqrt(1.0 - u * u);
             q = V3{prev.x + kBond * s * std::cos(ph), prev.y + kBond * s * std::sin(ph), prev.z + kBond * u};
             if (!inside(q, slack)) continue;
@@ -111,9 +123,18 @@ static bool grow(int n, const V3& start, Rng& rng, Inside inside, std::vector<V3
             });
             ok = !clash;
         }
-        if (!ok) return false;
-        pos.push_back(q);
-        g.add(q, (int)pos.size() - 1);
+        if (ok) {
+            pos.push_back(q);
+            g.add(q, (int)pos.size() - 1);
+            continue;
+        }
+        if (++backtracks > 20000) return false;
+        if (backtracks % 200 == 0) slack *= 1.02;
+        const int drop = std::min<int>(1 + backtracks % 8, (int)pos.size() - base - 1);
+        for (int d = 0; d < drop; ++d) {
+            g.remove(pos.back(), (int)pos.size() - 1);
+            pos.pop_back();
+        }
     }
     return true;
 }
@@ -258,5 +279,7 @@ extern "C" void sop_free_system(sop_synth* s) {
     std::free(s->nat_j);
     std::free(s->nat_r0);
     std::free(s->nat_eh);
+    std::free(s->pair_i);
+    std::free(s->pair_j);
     std::memset(s, 0, sizeof(*s));
 }

@@ -9,6 +9,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 from dataclasses import dataclass, field
+from typing import Optional
 
 import numpy as np
 
@@ -118,6 +119,7 @@ class SystemStruct(C.Structure):
         ("nat_eh", C.c_void_p),
         ("npairs", C.c_int), ("pair_i", C.c_void_p), ("pair_j", C.c_void_p),
         ("nfixed", C.c_int), ("fixed", C.c_void_p), ("npulled", C.c_int), ("pulled", C.c_void_p),
+        ("rng_state4", C.c_void_p),
     ]
 
 
@@ -128,6 +130,7 @@ class SynthStruct(C.Structure):
         ("bond_r0", C.POINTER(C.c_float)),
         ("nnative", C.c_int), ("nat_i", C.POINTER(C.c_int)), ("nat_j", C.POINTER(C.c_int)),
         ("nat_r0", C.POINTER(C.c_float)), ("nat_eh", C.POINTER(C.c_float)),
+        ("npairs", C.c_int), ("pair_i", C.POINTER(C.c_int)), ("pair_j", C.POINTER(C.c_int)),
     ]
 
 
@@ -173,7 +176,8 @@ class System:
         p = lambda a: a.ctypes.data if a.size else None  # noqa: E731
         st = SystemStruct(s.n, p(keep[0]), len(s.bond_r0), p(keep[1]), p(keep[2]), p(keep[3]),
                           len(s.nat_r0), p(keep[4]), p(keep[5]), p(keep[6]), p(keep[7]),
-                          len(s.pairs), p(keep[8]), p(keep[9]), len(s.fixed), p(keep[10]), len(s.pulled), p(keep[11]))
+                          len(s.pairs), p(keep[8]), p(keep[9]), len(s.fixed), p(keep[10]), len(s.pulled), p(keep[11]),
+                          None)
         return st, keep
 
 
@@ -219,12 +223,27 @@ _sig("sop_engine_set_chol_mode", C.c_int, VP, C.c_int)
 _sig("sop_engine_get_hi", C.c_int, VP, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_float))
 _sig("sop_engine_get_hi_factor", C.c_int, VP, C.c_int, VP)
 _sig("sop_chol_factor", C.c_int, C.c_int, C.c_int, VP, C.c_int, VP, C.c_int, C.POINTER(C.c_double))
+_sig("sop_server_create", C.c_int, C.c_int, C.c_ulonglong, C.POINTER(VP))
+_sig("sop_server_destroy", None, VP)
+_sig("sop_server_set_tuning", C.c_int, VP, C.c_int, C.c_int)
+_sig("sop_server_submit", C.c_int, VP, C.POINTER(Params), C.POINTER(SystemStruct), C.c_longlong, C.POINTER(C.c_int))
+_sig("sop_server_pump", C.c_int, VP, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int))
+_sig("sop_server_poll", C.c_int, VP, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double),
+     C.POINTER(C.c_double))
+_sig("sop_server_result", C.c_int, VP, C.c_int, VP)
+_sig("sop_server_stats", C.c_int, VP, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int),
+     C.POINTER(C.c_int), C.POINTER(C.c_ulonglong))
+_sig("sop_copy_bandwidth", C.c_int, C.c_int, C.c_ulonglong, C.c_int, C.POINTER(C.c_double))
 _sig("sop_generate_seeds", C.c_int, C.c_int, C.c_int, VP)
 _sig("sop_rng_probe", C.c_int, C.c_int, VP, C.c_int, C.c_int, C.c_int, VP, VP)
 _sig("sop_synth_chain", C.c_int, C.c_int, C.c_ulonglong, C.c_float, C.c_float, C.c_float, C.POINTER(SynthStruct))
 _sig("sop_synth_shell", C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.c_float, C.c_float, C.c_float,
      C.POINTER(SynthStruct))
 _sig("sop_free_system", None, C.POINTER(SynthStruct))
+_sig("sop_read_topology", C.c_int, C.c_char_p, C.c_char_p, C.POINTER(SynthStruct))
+_sig("sop_write_pdb", C.c_int, C.c_char_p, VP, C.c_int, C.c_char_p)
+_sig("sop_write_dcd", C.c_int, C.c_char_p, C.c_int, C.c_int, VP, C.c_int, C.c_float)
+_sig("sop_run_config", C.c_int, C.c_char_p, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_char_p))
 
 EXPORTED = [
     "sop_last_error", "sop_version", "sop_engine_create", "sop_engine_destroy", "sop_engine_add_job",
@@ -235,8 +254,10 @@ EXPORTED = [
     "sop_engine_get_list", "sop_engine_get_indentation", "sop_engine_compute_forces",
     "sop_engine_compute_energies", "sop_engine_rebuild_lists", "sop_engine_profile", "sop_engine_profile_read",
     "sop_engine_list_stats", "sop_engine_set_chol_mode", "sop_engine_get_hi", "sop_engine_get_hi_factor",
-    "sop_chol_factor", "sop_generate_seeds", "sop_rng_probe", "sop_synth_chain", "sop_synth_shell",
-    "sop_free_system",
+    "sop_chol_factor", "sop_server_create", "sop_server_destroy", "sop_server_set_tuning", "sop_server_submit",
+    "sop_server_pump", "sop_server_poll", "sop_server_result", "sop_server_stats", "sop_copy_bandwidth",
+    "sop_generate_seeds", "sop_rng_probe", "sop_synth_chain", "sop_synth_shell",
+    "sop_free_system", "sop_run_config", "sop_read_topology", "sop_write_pdb", "sop_write_dcd",
 ]
 
 
@@ -260,7 +281,13 @@ def _from_synth(s: SynthStruct) -> System:
                     np.ctypeslib.as_array(s.nat_j, shape=(max(nn, 1),))[:nn]], axis=1).astype(np.int32)
     nr0 = np.ctypeslib.as_array(s.nat_r0, shape=(max(nn, 1),))[:nn].copy()
     neh = np.ctypeslib.as_array(s.nat_eh, shape=(max(nn, 1),))[:nn].copy()
-    return System(coords, bonds, br0, nat, nr0, neh)
+    npr = s.npairs
+    if npr:
+        pairs = np.stack([np.ctypeslib.as_array(s.pair_i, shape=(npr,)),
+                          np.ctypeslib.as_array(s.pair_j, shape=(npr,))], axis=1).astype(np.int32)
+    else:
+        pairs = np.zeros((0, 2), np.int32)
+    return System(coords, bonds, br0, nat, nr0, neh, pairs)
 
 
 def synth_chain(n: int, seed: int, density: float = 175.0, eh=(0.9, 1.5)) -> System:
@@ -279,6 +306,30 @@ def synth_shell(nchains: int, n: int, seed: int, radius: float = 140.0, eh=(0.9,
         return _from_synth(s)
     finally:
         lib.sop_free_system(C.byref(s))
+
+
+def read_topology(top_path: str, coords_path: Optional[str] = None) -> System:
+    """SOP::load of a .top file (src/IO/topio.cpp:36-80), plus readCoord of a
+    PDB/XYZ file (src/IO/pdbio.cpp:292-337) when coords_path is given."""
+    s = SynthStruct()
+    check(lib.sop_read_topology(top_path.encode(), coords_path.encode() if coords_path else None, C.byref(s)))
+    try:
+        return _from_synth(s)
+    finally:
+        lib.sop_free_system(C.byref(s))
+
+
+def write_pdb(top_path: str, coords: np.ndarray, out_path: str) -> None:
+    """savePDB (src/IO/pdbio.cpp:194-219) of the .top's atoms at coords."""
+    c = np.ascontiguousarray(np.asarray(coords, np.float32)[:, :3])
+    c4 = np.ascontiguousarray(np.concatenate([c, np.zeros((c.shape[0], 1), np.float32)], axis=1))
+    check(lib.sop_write_pdb(top_path.encode(), _ptr(c4), c4.shape[0], out_path.encode()))
+
+
+def write_dcd(path: str, frames: np.ndarray, freq: int = 1000, delta: float = 0.0) -> None:
+    """DCD file (src/IO/dcdio.cpp:100-194); frames is [nframes, 3, natoms]."""
+    f = np.ascontiguousarray(frames, dtype=np.float32)
+    check(lib.sop_write_dcd(path.encode(), f.shape[2], f.shape[0], _ptr(f), freq, delta))
 
 
 def generate_seeds(seed: int, n: int) -> np.ndarray:
@@ -462,8 +513,82 @@ def chol_factor(A: np.ndarray, mode: int = CHOL_BF16X3, repeats: int = 1, device
     return L, ms.value
 
 
+def copy_bandwidth(nbytes: int = 1 << 30, repeats: int = 20, device: int = 0) -> float:
+    """GB/s (read + write) of a plain streaming copy kernel."""
+    g = C.c_double()
+    check(lib.sop_copy_bandwidth(device, nbytes, repeats, C.byref(g)))
+    return g.value
+
+
+def run_config(path: str, **overrides) -> None:
+    """Run the sop-gpu program on a parameter file (src/sop-gpu.cpp:18-88);
+    keyword arguments override parameters of the file."""
+    names = [k.encode() for k in overrides]
+    values = [str(v).encode() for v in overrides.values()]
+    n = len(names)
+    check(lib.sop_run_config(path.encode(), n, (C.c_char_p * max(n, 1))(*names), (C.c_char_p * max(n, 1))(*values)))
+
+
 def job_bytes(params: Params, system: System) -> int:
     st, keep = system.struct()
     b = C.c_ulonglong()
     check(lib.sop_engine_job_bytes(C.byref(params), C.byref(st), C.byref(b)))
     return b.value
+
+
+class Server:
+    """Continuous-batching request loop on one GPU (sop_server_*)."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        h = VP()
+        check(lib.sop_server_create(device, hbm_budget, C.byref(h)))
+        self.h = h
+        self._n = {}
+
+    def close(self):
+        if self.h:
+            lib.sop_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_tuning(self, max_blocks: int = 0, use_graph: bool = True):
+        check(lib.sop_server_set_tuning(self.h, max_blocks, int(use_graph)))
+
+    def submit(self, params: Params, system: System, numsteps: int) -> int:
+        st, keep = system.struct()
+        t = C.c_int()
+        check(lib.sop_server_submit(self.h, C.byref(params), C.byref(st), numsteps, C.byref(t)))
+        self._n[t.value] = system.n
+        return t.value
+
+    def pump(self, nblocks: int = 1):
+        r, q = C.c_int(), C.c_int()
+        check(lib.sop_server_pump(self.h, nblocks, C.byref(r), C.byref(q)))
+        return r.value, q.value
+
+    def poll(self, ticket: int):
+        st, a, b, c = C.c_int(), C.c_double(), C.c_double(), C.c_double()
+        check(lib.sop_server_poll(self.h, ticket, C.byref(st), C.byref(a), C.byref(b), C.byref(c)))
+        return st.value, a.value, b.value, c.value
+
+    def result(self, ticket: int) -> np.ndarray:
+        out = np.zeros((self._n[ticket], 4), np.float32)
+        check(lib.sop_server_result(self.h, ticket, _ptr(out)))
+        return out
+
+    def stats(self):
+        bs, busy, adm, fin, peak = C.c_double(), C.c_double(), C.c_int(), C.c_int(), C.c_ulonglong()
+        check(lib.sop_server_stats(self.h, C.byref(bs), C.byref(busy), C.byref(adm), C.byref(fin), C.byref(peak)))
+        return dict(bead_steps=bs.value, busy_s=busy.value, admitted=adm.value, finished=fin.value,
+                    peak_bytes=peak.value)
