@@ -179,6 +179,85 @@ def plain_serial_estimate(S, mix, jobs_c2):
     return total, {"a_ms": a * 1e3, "b_us_per_bead": b * 1e6}
 
 
+def rpy_matrix(coords, a):
+    """D_ij / D_ii of the Rotne-Prager-Yamakawa tensor (3n x 3n, FP64):
+    r > 2a: (3a/4r)[(1 + 2a^2/3r^2) I + (1 - 2a^2/r^2) rr^T/r^2];
+    r <= 2a: (1 - 9r/32a) I + (3r/32a) rr^T/r^2; identity diagonal blocks."""
+    x = np.asarray(coords, np.float64)[:, :3]
+    n = x.shape[0]
+    D = np.zeros((3 * n, 3 * n))
+    for b0 in range(0, n, 512):
+        b1 = min(n, b0 + 512)
+        d = x[None, :, :] - x[b0:b1, None, :]          # (rows, n, 3)
+        r = np.sqrt((d * d).sum(-1))
+        rs = np.where(r > 0, r, 1.0)
+        far = r > 2 * a
+        cii = np.where(far, 3 * a / (4 * rs) * (1 + 2 * a * a / (3 * rs * rs)), 1 - 9 * rs / (32 * a))
+        crr = np.where(far, 3 * a / (4 * rs) * (1 - 2 * a * a / (rs * rs)), 3 * rs / (32 * a))
+        u = d / rs[..., None]
+        blk = crr[..., None, None] * u[..., :, None] * u[..., None, :] + cii[..., None, None] * np.eye(3)
+        idx = np.arange(b0, b1)
+        blk[idx - b0, idx] = np.eye(3)
+        D[3 * b0:3 * b1, :] = blk.transpose(0, 2, 1, 3).reshape(3 * (b1 - b0), 3 * n)
+    return D
+
+
+def covariance_error(L, D):
+    """||L L^T - D||_F / ||D||_F in FP64 (on the GPU through torch when it
+    is there, else numpy)."""
+    try:
+        import torch
+        if torch.cuda.is_available():
+            Lt = torch.from_numpy(np.ascontiguousarray(L)).cuda().double()
+            Dt = torch.from_numpy(D).cuda()
+            e = (torch.linalg.norm(Lt @ Lt.T - Dt) / torch.linalg.norm(Dt)).item()
+            del Lt, Dt
+            torch.cuda.empty_cache()
+            return e
+    except ImportError:
+        pass
+    Ld = L.astype(np.float64)
+    return float(np.linalg.norm(Ld @ Ld.T - D) / np.linalg.norm(D))
+
+
+def hi_phase(S, n=4096, reps=5, steps=50):
+    """Exact hydrodynamics at N = 4096 (12,288 degrees of freedom): the
+    blocked Cholesky of the RPY matrix, split-BF16 vs FP32 MFMA trailing
+    updates, timed and checked against the FP64 matrix; then the exact-HI
+    integrator (factor every 10 steps, 100 pN force clamp) in both modes."""
+    sysm = S.synth_chain(n, seed=4096)
+    sysm.fixed = np.array([0], np.int32)
+    sysm.pulled = np.array([n - 1], np.int32)
+    D = rpy_matrix(sysm.coords, 3.8)
+    D32 = D.astype(np.float32)
+    out = {"n_beads": n, "order": 3 * n, "hi_a": 3.8}
+    flops = (3 * n) ** 3 / 3.0
+    for mode, name in ((S.CHOL_F32, "f32"), (S.CHOL_BF16X3, "bf16x3")):
+        L, ms = S.chol_factor(D32, mode, repeats=reps)
+        out[name] = {"ms_per_factor": ms, "tflops": flops / (ms * 1e-3) / 1e12,
+                     "covariance_error": covariance_error(L, D)}
+        print(f"# cholesky {name}: {ms:.3f} ms, cov err {out[name]['covariance_error']:.2e}",
+              file=sys.stderr, flush=True)
+        p = S.make_params(kspring_cov=14.0, R_limit=2.0, pairs_cutoff=20.0, pairlist_cutoff=20.0,
+                          pairs_threshold=32.0, pairs_freq=10, possiblepairs_freq=100, timestep=H_TAU,
+                          temperature=0.6, seed=4096, nav=1000, pulling=1, fixedEnd=0, pulledEnd=n - 1,
+                          fconst=100.0 / PN_PER_KCAL_A, hi_on=1, hi_exact=1, hi_a=3.8, hi_chol_freq=10)
+        eng = S.Engine(0, 16 << 30)
+        eng.set_chol_mode(mode)
+        jid = eng.add_job(p, sysm, steps + 10)
+        eng.run_blocks(1)  # warm-up block
+        eng.synchronize()
+        t0 = time.perf_counter()
+        eng.run_blocks(steps // eng.block_steps())
+        eng.synchronize()
+        dt = time.perf_counter() - t0
+        done = (steps // eng.block_steps()) * eng.block_steps()
+        out[name]["integrator_steps_per_s"] = done / dt
+        eng.close()
+    out["speedup_bf16x3_over_f32"] = out["f32"]["ms_per_factor"] / out["bf16x3"]["ms_per_factor"]
+    return out
+
+
 def bytes_per_bead_step(avg):
     """Algorithmic HBM bytes per bead-step of the fused step kernel:
     coordinates read+write 32, RNG state read+write 32, three list counts 12,
@@ -225,6 +304,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-hi", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=60)
     ap.add_argument("--serve-steps-lo", type=int, default=2000)
     ap.add_argument("--serve-steps-hi", type=int, default=20000)
@@ -316,6 +396,11 @@ def main():
         print(f"# serving: p50 {serving['p50_latency_s']:.3f} s p99 {serving['p99_latency_s']:.3f} s",
               file=sys.stderr, flush=True)
 
+    hi = None
+    if not args.no_hi:
+        print("# exact-HI phase", file=sys.stderr, flush=True)
+        hi = hi_phase(S)
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -350,6 +435,7 @@ def main():
                      "avg_lists": {"covalent": avg_lists[0], "native": avg_lists[1], "verlet": avg_lists[2],
                                    "possible": avg_lists[3]}},
         "serving": serving,
+        "hi_cholesky": hi,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -134,6 +134,13 @@ int sop_engine_job_bytes(const sop_params* p, const sop_system* s, unsigned long
  * workgroup per 4 tiles); use_graph != 0 replays each block from a
  * captured hipGraph. */
 int sop_engine_set_tuning(sop_engine* e, int max_blocks, int use_graph);
+/* Possible-pairs builder (src/Updaters/possiblepairlist_kernel.cu:18-71
+ * scans all same-trajectory pairs): jobs of min_beads beads and more admitted
+ * after this call (default 8192; 0 keeps the threshold, < 0 none) get a cell
+ * grid and build the identical list, in the identical order, from the 27
+ * cells around each bead.  enabled = 0 makes every job use the all-pairs
+ * scan again (the cell grids stay allocated). */
+int sop_engine_set_cells(sop_engine* e, int min_beads, int enabled);
 /* Pick max_blocks by timing candidate grids on the admitted jobs. */
 int sop_engine_autotune(sop_engine* e, int* max_blocks_out);
 

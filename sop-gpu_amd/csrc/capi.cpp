@@ -84,6 +84,12 @@ int sop_engine_set_tuning(sop_engine* e, int max_blocks, int use_graph) {
     return 0;
 }
 
+int sop_engine_set_cells(sop_engine* e, int min_beads, int enabled) {
+    ENG(e);
+    E.set_cells(min_beads, enabled);
+    return 0;
+}
+
 int sop_engine_autotune(sop_engine* e, int* max_blocks_out) {
     ENG(e);
     int b = 0;
@@ -246,15 +252,18 @@ int sop_chol_factor(int device, int n, const float* A, int mode, float* L, int r
     float *dA = nullptr, *dM = nullptr;
     double* linv = nullptr;
     unsigned short *phi = nullptr, *plo = nullptr;
-    hipStream_t s = nullptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipStream_t s = nullptr, aux = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr, ef = nullptr, et = nullptr;
     int rc = 0;
     hipError_t err = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (err == hipSuccess) err = hipStreamCreateWithFlags(&aux, hipStreamNonBlocking);
+    if (err == hipSuccess) err = hipEventCreateWithFlags(&ef, hipEventDisableTiming);
+    if (err == hipSuccess) err = hipEventCreateWithFlags(&et, hipEventDisableTiming);
     if (err == hipSuccess) err = hipMalloc(&dA, mp * mp * 4);
     if (err == hipSuccess) err = hipMalloc(&dM, mp * mp * 4);
     if (err == hipSuccess) err = hipMalloc(&linv, 64 * 64 * 8);
-    if (err == hipSuccess) err = hipMalloc(&phi, mp * 64 * 2);
-    if (err == hipSuccess) err = hipMalloc(&plo, mp * 64 * 2);
+    if (err == hipSuccess) err = hipMalloc(&phi, 2 * mp * sop::kCholPanel * 2);
+    if (err == hipSuccess) err = hipMalloc(&plo, 2 * mp * sop::kCholPanel * 2);
     if (err == hipSuccess) err = hipEventCreate(&e0);
     if (err == hipSuccess) err = hipEventCreate(&e1);
     if (err == hipSuccess) err = hipMemcpy(dA, h.data(), mp * mp * 4, hipMemcpyHostToDevice);
@@ -263,7 +272,8 @@ int sop_chol_factor(int device, int n, const float* A, int mode, float* L, int r
     for (int r = 0; r < reps && err == hipSuccess; ++r) {
         err = hipMemcpyAsync(dM, dA, mp * mp * 4, hipMemcpyDeviceToDevice, s);
         if (err == hipSuccess) err = hipEventRecord(e0, s);
-        if (err == hipSuccess) err = sop::launch_chol_factor(dM, mp, (int)mp, mode, linv, phi, plo, nullptr, s);
+        if (err == hipSuccess)
+            err = sop::launch_chol_factor(dM, mp, (int)mp, mode, linv, phi, plo, nullptr, s, aux, ef, et);
         if (err == hipSuccess) err = hipEventRecord(e1, s);
         if (err == hipSuccess) err = hipEventSynchronize(e1);
         float ms = 0.0f;
@@ -279,6 +289,9 @@ int sop_chol_factor(int device, int n, const float* A, int mode, float* L, int r
     if (ms_per_factor) *ms_per_factor = total / reps;
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
+    if (ef) (void)hipEventDestroy(ef);
+    if (et) (void)hipEventDestroy(et);
+    if (aux) (void)hipStreamDestroy(aux);
     (void)hipFree(dA);
     (void)hipFree(dM);
     (void)hipFree(linv);

@@ -118,6 +118,12 @@ Engine::~Engine() {
     (void)hipFree(d_tiles_);
     (void)hipFree(d_tiles1_);
     (void)hipFree(d_scratch4_);
+    if (aux_stream_) {
+        (void)hipStreamSynchronize(aux_stream_);
+        (void)hipStreamDestroy(aux_stream_);
+    }
+    if (ev_f_) (void)hipEventDestroy(ev_f_);
+    if (ev_t2_) (void)hipEventDestroy(ev_t2_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -429,12 +435,13 @@ static unsigned long long built_bytes(const BuiltJob& b, bool pulling, bool inde
     t += round_up(n * b.pair_cap * es) + round_up(n * b.pp_cap * es) + round_up(n * b.excl_cap * 4);
     t += round_up(std::max<size_t>(1, b.pulled.size()) * 4);
     t += round_up(n * 12);                // energies
-    t += round_up(n * b.pair_cap * es);   // Verlet scratch
+    if (n >= (unsigned long long)kCellMinBeads && n <= 64ull * kMaxCellWords)
+        t += round_up((n + 64) * 4) + round_up((n + 65) * 4) + 3 * round_up(n * 4);  // cell grid
     if (b.dj.flags & JF_HI) {
         t += 3 * round_up(n * 16) + round_up(n * 4);  // rforce, mforce, ci, eps
         if (b.dj.flags & JF_HI_EXACT) {
             const unsigned long long mp = (3 * n + 63) / 64 * 64;
-            t += round_up(n * 16) + round_up(mp * mp * 4) + round_up(64 * 64 * 8) + 2 * round_up(mp * 64 * 2);
+            t += round_up(n * 16) + round_up(mp * mp * 4) + round_up(64 * 64 * 8) + 2 * round_up(2 * mp * kCholPanel * 2);
         }
     }
     return t;
@@ -488,7 +495,14 @@ int Engine::add_job(const sop_params& p, const sop_system& s, long long numsteps
     J.pulled = (int*)A(std::max<size_t>(1, b.pulled.size()) * 4);
     J.npulled = (int)b.pulled.size();
     J.energies = (float*)A(n * 12);
-    J.pair_tmp = A(n * b.pair_cap * es);
+    if (uses_cells(n)) {
+        J.cell_cap = (int)n + 64;
+        J.cell_cnt = (int*)A((size_t)J.cell_cap * 4);
+        J.cell_start = (int*)A((size_t)(J.cell_cap + 1) * 4);
+        J.cell_beads = (int*)A(n * 4);
+        J.bead_cell = (int*)A(n * 4);
+        J.bead_slot = (int*)A(n * 4);
+    }
     HiExactWork hw{};
     if (J.flags & JF_HI) {
         J.rforce = (float4*)A(n * 16);
@@ -505,9 +519,18 @@ int Engine::add_job(const sop_params& p, const sop_system& s, long long numsteps
             hw.mp = (int)mp;
             hw.mode = exact_mode_;
             hw.linv = (double*)A(64 * 64 * 8);
-            hw.phi = (unsigned short*)A(mp * 64 * 2);
-            hw.plo = (unsigned short*)A(mp * 64 * 2);
+            hw.phi = (unsigned short*)A(2 * mp * kCholPanel * 2);
+            hw.plo = (unsigned short*)A(2 * mp * kCholPanel * 2);
             hw.chol_freq = J.hi_chol_freq;
+            if (!aux_stream_) {  // the factorisation's lookahead stream, shared by the exact-HI jobs
+                if (hipStreamCreateWithFlags(&aux_stream_, hipStreamNonBlocking) != hipSuccess ||
+                    hipEventCreateWithFlags(&ev_f_, hipEventDisableTiming) != hipSuccess ||
+                    hipEventCreateWithFlags(&ev_t2_, hipEventDisableTiming) != hipSuccess)
+                    oom = true;
+            }
+            hw.aux = aux_stream_;
+            hw.ev_f = ev_f_;
+            hw.ev_t2 = ev_t2_;
         }
     }
     if (oom) {
@@ -637,7 +660,11 @@ int Engine::sync_tables() {
     }
     L_ = g > 0 ? g : 1;
     any_tea_ = any_exact_ = false;
+    cell_words_ = 0;
+    any_idx32_ = 0;
     for (auto& j : jobs_) {
+        if (!j.dj.idx16) any_idx32_ = 1;
+        cell_words_ = std::max(cell_words_, cell_words_of(j.dj));
         if ((j.dj.flags & JF_HI) && !(j.dj.flags & JF_HI_EXACT)) any_tea_ = true;
         if (j.dj.flags & JF_HI_EXACT) any_exact_ = true;
     }
@@ -673,6 +700,19 @@ void Engine::drop_graph() {
         (void)hipGraphExecDestroy(graph_);
         graph_ = nullptr;
     }
+}
+
+bool Engine::uses_cells(size_t n) const {
+    return cell_min_ > 0 && n >= (size_t)cell_min_ && n <= (size_t)64 * kMaxCellWords;
+}
+
+int Engine::cell_words_of(const DevJob& J) const { return (use_cells_ && J.cell_cnt) ? (J.n + 63) / 64 : 0; }
+
+void Engine::set_cells(int min_beads, int enabled) {
+    if (min_beads != 0) cell_min_ = min_beads;
+    use_cells_ = enabled != 0;
+    tables_dirty_ = true;
+    drop_graph();
 }
 
 void Engine::set_tuning(int max_blocks, int use_graph) {
@@ -724,8 +764,10 @@ int Engine::launch_block(int slot_only) {
         L = jobs_[slot_only].dj.stride;
     }
     if (int rc = check(launch_job_update(jobs, st, nj, stream_), "job update")) return rc;
-    if (int rc = check(launch_possible(jobs, st, tiles, nt, 0, stream_), "possible pairs")) return rc;
-    if (int rc = check(launch_verlet(jobs, st, tiles, nt, 0, stream_), "verlet")) return rc;
+    const int cw = slot_only >= 0 ? cell_words_of(jobs_[slot_only].dj) : cell_words_;
+    if (int rc = check(launch_possible(jobs, st, tiles, nt, nj, cw, 0, stream_), "possible pairs")) return rc;
+    const int i32 = slot_only >= 0 ? !jobs_[slot_only].dj.idx16 : any_idx32_;
+    if (int rc = check(launch_verlet(jobs, st, tiles, nt, i32, 0, stream_), "verlet")) return rc;
     const bool tea = slot_only >= 0 ? ((jobs_[slot_only].dj.flags & JF_HI) && !(jobs_[slot_only].dj.flags & JF_HI_EXACT))
                                     : any_tea_;
     const bool exact = slot_only >= 0 ? (jobs_[slot_only].dj.flags & JF_HI_EXACT) != 0 : any_exact_;
@@ -801,11 +843,12 @@ int Engine::stage_job_update(int mask) {
 }
 int Engine::stage_possible() {
     if (int rc = sync_tables()) return rc;
-    return check(launch_possible(d_jobs_, d_state_, d_tiles_, ntiles_, 0, stream_), "possible pairs");
+    return check(launch_possible(d_jobs_, d_state_, d_tiles_, ntiles_, (int)jobs_.size(), cell_words_, 0, stream_),
+                 "possible pairs");
 }
 int Engine::stage_verlet() {
     if (int rc = sync_tables()) return rc;
-    return check(launch_verlet(d_jobs_, d_state_, d_tiles_, ntiles_, 0, stream_), "verlet");
+    return check(launch_verlet(d_jobs_, d_state_, d_tiles_, ntiles_, any_idx32_, 0, stream_), "verlet");
 }
 int Engine::stage_tea_update() {
     if (int rc = sync_tables()) return rc;
@@ -1090,11 +1133,13 @@ int Engine::rebuild_lists(int id, int mask) {
     int nt = 0;
     if (int rc = tiles_for_slot(slot, &nt)) return rc;
     if (mask & 1)
-        if (int rc = check(launch_possible(d_jobs_ + slot, d_state_ + slot, d_tiles1_, nt, 1, stream_),
+        if (int rc = check(launch_possible(d_jobs_ + slot, d_state_ + slot, d_tiles1_, nt, 1,
+                                           cell_words_of(jobs_[slot].dj), 1, stream_),
                            "possible pairs"))
             return rc;
     if (mask & 2)
-        if (int rc = check(launch_verlet(d_jobs_ + slot, d_state_ + slot, d_tiles1_, nt, 1, stream_),
+        if (int rc = check(launch_verlet(d_jobs_ + slot, d_state_ + slot, d_tiles1_, nt, !jobs_[slot].dj.idx16, 1,
+                                         stream_),
                            "verlet"))
             return rc;
     return synchronize();

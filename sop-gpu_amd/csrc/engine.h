@@ -104,6 +104,10 @@ class Engine {
     int read_state_of(int id, DevJobState* out);
 
     void set_tuning(int max_blocks, int use_graph);
+    // Possible-pairs cell grid: jobs of min_beads beads and more admitted
+    // from now on get one (0 keeps the threshold, < 0 gives none); enabled = 0
+    // builds every list by the all-pairs scan.
+    void set_cells(int min_beads, int enabled);
     void set_exact_mode(int mode) { exact_mode_ = mode; }
     int get_hi(int id, double* eps, float* beta);
     int get_hi_factor(int id, float* L);
@@ -126,6 +130,8 @@ class Engine {
 
     int device_;
     hipStream_t stream_ = nullptr;
+    hipStream_t aux_stream_ = nullptr;  // lookahead stream of the exact-HI factorisation
+    hipEvent_t ev_f_ = nullptr, ev_t2_ = nullptr;
     Arena arena_;
     std::vector<HostJob> jobs_;
     int next_id_ = 1;
@@ -143,6 +149,12 @@ class Engine {
     int tiles_for_slot(int slot, int* nt);
     // tuning
     int max_blocks_ = 0;
+    int cell_min_ = kCellMinBeads;
+    bool use_cells_ = true;
+    int cell_words_ = 0;
+    int any_idx32_ = 0;  // some job stores int32 list entries  // bitmap words of the largest job with a cell grid (0: none or disabled)
+    bool uses_cells(size_t n) const;
+    int cell_words_of(const DevJob& J) const;
     int exact_mode_ = SOP_CHOL_BF16X3;
     bool any_tea_ = false, any_exact_ = false;
     bool use_graph_ = true;

@@ -360,7 +360,8 @@ __global__ __launch_bounds__(256) void k_chol_L(const DevJob* __restrict__ jobs,
 }
 
 // ---- blocked Cholesky -------------------------------------------------------
-static constexpr int NB = 64;
+static constexpr int NB = 64;    // inner block: diagonal factor, panel solve
+static constexpr int NB2 = kCholPanel;  // outer panel: depth of the trailing update
 
 __device__ __forceinline__ unsigned short f2bf(float x) {  // round to nearest even
     unsigned int u = __float_as_uint(x);
@@ -369,13 +370,17 @@ __device__ __forceinline__ unsigned short f2bf(float x) {  // round to nearest e
 }
 __device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float(((unsigned int)h) << 16); }
 
-// Factor the k-th diagonal block in FP64 and store its inverse.
-__global__ __launch_bounds__(256) void k_potrf_diag(float* __restrict__ M, size_t ld, int k, double* __restrict__ linv,
+// Factor the k-th diagonal block in FP64 in LDS, 256 threads.  Step j
+// subtracts (a_rj/sqrt(p_j)) (a_cj/sqrt(p_j)) from the entries j < c <= r,
+// reading the unscaled column j, then column j is scaled; the scaling
+// touches no element the next step reads, so each step needs one barrier.
+// Writes L to M (FP32) and to ldiag (FP64, zero above the diagonal) for the
+// panel solve.
+__global__ __launch_bounds__(256) void k_potrf_diag(float* __restrict__ M, size_t ld, int k, double* __restrict__ ldiag,
                                                     const int* __restrict__ flag) {
     if (flag && !*flag) return;
     __shared__ double s[NB][NB + 1];
-    __shared__ double v[NB][NB + 1];
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
     const size_t o = (size_t)k * NB;
     for (int e = tid; e < NB * NB; e += 256) {
         const int r = e / NB, c = e % NB;
@@ -383,63 +388,102 @@ __global__ __launch_bounds__(256) void k_potrf_diag(float* __restrict__ M, size_
     }
     __syncthreads();
     for (int j = 0; j < NB; ++j) {
-        if (tid == 0) s[j][j] = sqrt(s[j][j]);
-        __syncthreads();
-        const double djj = s[j][j];
-        for (int r = j + 1 + tid; r < NB; r += 256) s[r][j] /= djj;
-        __syncthreads();
-        const int rem = NB - 1 - j;
-        for (int e = tid; e < rem * rem; e += 256) {
-            const int r = j + 1 + e / rem, c = j + 1 + e % rem;
-            if (c <= r) s[r][c] -= s[r][j] * s[c][j];
+        const double d = sqrt(s[j][j]);
+        const double rd = 1.0 / d;
+        // every operand of the thread's (up to) 4x4 entries is read before any
+        // entry is written, so the reads are issued back to back
+        double lr[4], lc[4], t[4][4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const int r = j + 1 + ty + 16 * a, c = j + 1 + tx + 16 * a;
+            lr[a] = r < NB ? s[r][j] : 0.0;
+            lc[a] = c < NB ? s[c][j] : 0.0;
         }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int r = j + 1 + ty + 16 * a, c = j + 1 + tx + 16 * b;
+                t[a][b] = (r < NB && c <= r) ? s[r][c] : 0.0;
+            }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int r = j + 1 + ty + 16 * a, c = j + 1 + tx + 16 * b;
+                if (r < NB && c <= r) s[r][c] = fma(-(lr[a] * rd), lc[b] * rd, t[a][b]);
+            }
         __syncthreads();
+        if (tid > j && tid < NB) s[tid][j] *= rd;
+        if (tid == j) s[j][j] = d;
     }
+    __syncthreads();
     for (int e = tid; e < NB * NB; e += 256) {
         const int r = e / NB, c = e % NB;
         if (c <= r) M[(o + r) * ld + o + c] = (float)s[r][c];
+        ldiag[e] = c <= r ? s[r][c] : 0.0;
     }
-    // inverse of the lower-triangular factor, one column per thread
-    if (tid < NB) {
-        const int c = tid;
-        for (int r = 0; r < NB; ++r) v[r][c] = 0.0;
-        for (int r = c; r < NB; ++r) {
-            double x = r == c ? 1.0 : 0.0;
-            for (int q = c; q < r; ++q) x -= s[r][q] * v[q][c];
-            v[r][c] = x / s[r][r];
-        }
-    }
-    __syncthreads();
-    for (int e = tid; e < NB * NB; e += 256) linv[e] = v[e / NB][e % NB];
 }
 
-// Panel below block k: L_ik = A_ik Linv^T, FP64 accumulation, one row per
-// thread; also writes the split-BF16 copy of the panel row.
+// Steps T of the panel solve (a template parameter, so every register index
+// below is a constant): the owner of column T solves x_T, the four threads
+// of the row exchange it, and each updates its columns c = 4u + p > T.
+template <int T>
+__device__ __forceinline__ void trsm_step(double (&a)[16], const double (*lp)[4][16], const double* inv, int p,
+                                          int g0) {
+    constexpr int owner = T & 3, ut = T >> 2;
+    const double xo = p == owner ? a[ut] * inv[T] : 0.0;
+    const double xt = __shfl(xo, g0 | owner);
+    if (p == owner) a[ut] = xt;
+    const double2* __restrict__ cf = reinterpret_cast<const double2*>(&lp[T][p][0]);
+#pragma unroll
+    for (int h = ut / 2; h < 8; ++h) {
+        const double2 q = cf[h];
+        const int u0 = 2 * h, u1 = 2 * h + 1;
+        if (4 * u0 + p > T) a[u0] = fma(-xt, q.x, a[u0]);
+        if (4 * u1 + p > T) a[u1] = fma(-xt, q.y, a[u1]);
+    }
+    if constexpr (T + 1 < NB) trsm_step<T + 1>(a, lp, inv, p, g0);
+}
+
+// Panel below block k: X L^T = A for the rows below the diagonal block, in
+// FP64 by forward substitution.  Four threads share a row, thread p owning
+// the columns c = 4u + p; at step t the owner of column t solves x_t and
+// hands it to the other three, which update their columns c > t.  The
+// factor sits in LDS permuted so that a thread's sixteen coefficients of a
+// step are contiguous.  Writes X to M (FP32) and its split-BF16 copy into
+// the outer panel buffers (row stride NB2, block k at column poff).
 __global__ __launch_bounds__(256) void k_trsm(float* __restrict__ M, size_t ld, int k, int rows_end,
-                                              const double* __restrict__ linv, unsigned short* __restrict__ phi,
-                                              unsigned short* __restrict__ plo, const int* __restrict__ flag) {
+                                              const double* __restrict__ ldiag, unsigned short* __restrict__ phi,
+                                              unsigned short* __restrict__ plo, int poff,
+                                              const int* __restrict__ flag) {
     if (flag && !*flag) return;
-    __shared__ double li[NB][NB + 1];
-    for (int e = threadIdx.x; e < NB * NB; e += 256) li[e / NB][e % NB] = linv[e];
+    __shared__ __attribute__((aligned(16))) double lp[NB][4][16];  // lp[t][p][u] = L[4u+p][t]
+    __shared__ double inv[NB];
+    for (int e = threadIdx.x; e < NB * NB; e += 256) {
+        const int c = e / NB, t = e % NB;  // L[c][t]
+        lp[t][c & 3][c >> 2] = ldiag[e];
+    }
+    if (threadIdx.x < NB) inv[threadIdx.x] = 1.0 / ldiag[threadIdx.x * NB + threadIdx.x];
     __syncthreads();
+    const int lane = threadIdx.x & 63, p = threadIdx.x & 3;
+    const int rl = threadIdx.x >> 2;
     const size_t o = (size_t)k * NB;
-    const size_t row = o + NB + (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (row >= (size_t)rows_end) return;
-    float a[NB];
+    const size_t row = o + NB + (size_t)blockIdx.x * 64 + rl;
+    if (row >= (size_t)rows_end) return;  // the four threads of a row leave together
     float* __restrict__ Ar = M + row * ld + o;
+    double a[16];
 #pragma unroll
-    for (int t = 0; t < NB; ++t) a[t] = Ar[t];
-#pragma unroll 4
-    for (int c = 0; c < NB; ++c) {
-        double acc = 0.0;
+    for (int u = 0; u < 16; ++u) a[u] = (double)Ar[4 * u + p];
+    trsm_step<0>(a, lp, inv, p, lane & ~3);
 #pragma unroll
-        for (int t = 0; t < NB; ++t)
-            if (t <= c) acc += (double)a[t] * li[c][t];
-        const float x = (float)acc;
-        Ar[c] = x;
-        const unsigned short h = f2bf(x);
-        phi[row * NB + c] = h;
-        plo[row * NB + c] = f2bf(x - bf2f(h));
+    for (int u = 0; u < 16; ++u) {
+        const int c = 4 * u + p;
+        const float v = (float)a[u];
+        Ar[c] = v;
+        const unsigned short h = f2bf(v);
+        phi[row * NB2 + poff + c] = h;
+        plo[row * NB2 + poff + c] = f2bf(v - bf2f(h));
     }
 }
 
@@ -451,20 +495,21 @@ __device__ __forceinline__ void tri_index(int idx, int& I, int& J) {
     J = idx - i * (i + 1) / 2;
 }
 
-// Trailing update with split-BF16 MFMA: C_IJ -= P_I P_J^T over the 64-wide
-// panel, three products per K-chunk, FP32 accumulation.  One 64x64 tile
-// per workgroup; wavefront w owns rows 16w..16w+15.
+// Update of the outer panel's remaining columns by inner block k: tile
+// (I, J) of 64x64 with J = k+1+blockIdx.y (a column block of the panel)
+// and I = J + blockIdx.x (rows down to the end), C_IJ -= L_Ik L_Jk^T.
+// Split-BF16 MFMA, three products per K-chunk, FP32 accumulation; wave w
+// owns rows 16w..16w+15 of the tile.
 // v_mfma_f32_16x16x32_bf16 lane layout: A[m=l%16][k=8(l/16)+0..7],
 // B[k=8(l/16)+0..7][n=l%16], C[4(l/16)+0..3][l%16].
-__global__ __launch_bounds__(256) void k_syrk_bf16x3(float* __restrict__ M, size_t ld, int k,
+__global__ __launch_bounds__(256) void k_syrk_bf16x3(float* __restrict__ M, size_t ld, int k, int ntiles,
                                                      const unsigned short* __restrict__ phi,
-                                                     const unsigned short* __restrict__ plo,
+                                                     const unsigned short* __restrict__ plo, int poff,
                                                      const int* __restrict__ flag) {
     if (flag && !*flag) return;
-    int I, Jt;
-    tri_index(blockIdx.x, I, Jt);
-    const size_t base = (size_t)(k + 1) * NB;
-    const size_t r0 = base + (size_t)I * NB, c0 = base + (size_t)Jt * NB;
+    const int Jt = k + 1 + (int)blockIdx.y, I = Jt + (int)blockIdx.x;
+    if (I >= ntiles) return;
+    const size_t r0 = (size_t)I * NB, c0 = (size_t)Jt * NB;
     const int l = threadIdx.x & 63, w = wave_of();
     const int lr = l & 15, lk = (l >> 4) * 8;
     f32x4 acc[4];
@@ -473,13 +518,13 @@ __global__ __launch_bounds__(256) void k_syrk_bf16x3(float* __restrict__ M, size
     const size_t arow = r0 + 16 * w + lr;
 #pragma unroll
     for (int kc = 0; kc < NB; kc += 32) {
-        const bf16x8 ah = *reinterpret_cast<const bf16x8*>(phi + arow * NB + kc + lk);
-        const bf16x8 al = *reinterpret_cast<const bf16x8*>(plo + arow * NB + kc + lk);
+        const bf16x8 ah = *reinterpret_cast<const bf16x8*>(phi + arow * NB2 + poff + kc + lk);
+        const bf16x8 al = *reinterpret_cast<const bf16x8*>(plo + arow * NB2 + poff + kc + lk);
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const size_t brow = c0 + 16 * t + lr;
-            const bf16x8 bh = *reinterpret_cast<const bf16x8*>(phi + brow * NB + kc + lk);
-            const bf16x8 bl = *reinterpret_cast<const bf16x8*>(plo + brow * NB + kc + lk);
+            const bf16x8 bh = *reinterpret_cast<const bf16x8*>(phi + brow * NB2 + poff + kc + lk);
+            const bf16x8 bl = *reinterpret_cast<const bf16x8*>(plo + brow * NB2 + poff + kc + lk);
             acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[t], 0, 0, 0);
             acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[t], 0, 0, 0);
             acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[t], 0, 0, 0);
@@ -500,13 +545,13 @@ __global__ __launch_bounds__(256) void k_syrk_bf16x3(float* __restrict__ M, size
 // FP32 path of the same update: v_mfma_f32_16x16x4_f32, lane layout
 // A[m=l%16][k=l/16], B[k=l/16][n=l%16], C as above; operands read from the
 // panel in M itself.
-__global__ __launch_bounds__(256) void k_syrk_f32(float* __restrict__ M, size_t ld, int k,
+__global__ __launch_bounds__(256) void k_syrk_f32(float* __restrict__ M, size_t ld, int k, int ntiles,
                                                   const int* __restrict__ flag) {
     if (flag && !*flag) return;
-    int I, Jt;
-    tri_index(blockIdx.x, I, Jt);
-    const size_t base = (size_t)(k + 1) * NB, pc = (size_t)k * NB;
-    const size_t r0 = base + (size_t)I * NB, c0 = base + (size_t)Jt * NB;
+    const int Jt = k + 1 + (int)blockIdx.y, I = Jt + (int)blockIdx.x;
+    if (I >= ntiles) return;
+    const size_t pc = (size_t)k * NB;
+    const size_t r0 = (size_t)I * NB, c0 = (size_t)Jt * NB;
     const int l = threadIdx.x & 63, w = wave_of();
     const int lr = l & 15, lk = l >> 4;
     f32x4 acc[4];
@@ -534,6 +579,214 @@ __global__ __launch_bounds__(256) void k_syrk_f32(float* __restrict__ M, size_t 
     }
 }
 
+// Trailing update after an outer panel: C_IJ -= P_I P_J^T over the panel's
+// K = kw columns, on 128x128 tiles of the lower triangle from row/column
+// s0 on.  Wave w computes the 64x64 quarter (w>>1, w&1) as 4x4 MFMA blocks;
+// operands are staged through LDS in K-chunks.  Rows at or beyond mp are
+// read as zero and not written.
+static constexpr int GT = 128;   // trailing tile edge
+static constexpr int KCB = 32;   // K-chunk of the split-BF16 update (bf16 elements)
+static constexpr int KCF = 16;   // K-chunk of the FP32 update (floats)
+static constexpr int LPB = KCB + 8;  // padded LDS row (bf16 elements)
+static constexpr int LPF = KCF + 4;  // padded LDS row (floats)
+
+// Tile (blockIdx.x, blockIdx.y) of the update region: rows [rs, mp),
+// columns [cs, ce).  Tiles wholly above the diagonal have nothing to do.
+__device__ __forceinline__ bool big_tile(int rs, int cs, size_t& r0, size_t& c0) {
+    r0 = (size_t)rs + (size_t)blockIdx.x * GT;
+    c0 = (size_t)cs + (size_t)blockIdx.y * GT;
+    return c0 < r0 + GT;
+}
+
+__device__ __forceinline__ void big_store(float* __restrict__ M, size_t ld, int mp, int ce, size_t r0, size_t c0,
+                                          const f32x4 (&acc)[4][4]) {
+    const int l = threadIdx.x & 63, w = wave_of();
+    const size_t rb = r0 + 64 * (w >> 1), cb = c0 + 64 * (w & 1);
+#pragma unroll
+    for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const size_t row = rb + 16 * bi + 4 * (l >> 4) + q;
+                const size_t col = cb + 16 * bj + (l & 15);
+                if (row >= (size_t)mp || col >= (size_t)ce || col > row) continue;
+                M[row * ld + col] -= acc[bi][bj][q];
+            }
+}
+
+// Operands of the next K-chunk are loaded into registers while the MFMAs of
+// the current chunk run from LDS; they are stored to LDS after the chunk.
+__global__ __launch_bounds__(256) void k_gemm_bf16x3(float* __restrict__ M, size_t ld, int rs, int cs, int ce, int mp,
+                                                     int kw, const unsigned short* __restrict__ phi,
+                                                     const unsigned short* __restrict__ plo) {
+    __shared__ __attribute__((aligned(16))) unsigned short sa[2][GT][LPB];  // hi, lo of the row tile
+    __shared__ __attribute__((aligned(16))) unsigned short sb[2][GT][LPB];  // hi, lo of the column tile
+    size_t r0, c0;
+    if (!big_tile(rs, cs, r0, c0)) return;
+    const int tid = threadIdx.x, l = tid & 63, w = wave_of();
+    const int lr = l & 15, lk = (l >> 4) * 8;
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // loader: thread t moves 16 bf16 (two 16-byte words) of row t/2 per array
+    const int ldr = tid >> 1, ldk = (tid & 1) * 16;
+    const size_t ga = r0 + ldr, gb = c0 + ldr;
+    const bool va = ga < (size_t)mp, vb = gb < (size_t)ce;
+    uint4 v[8];
+    auto fetch = [&](int kc) {
+        const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+        if (va) {
+            const uint4* ph = reinterpret_cast<const uint4*>(phi + ga * NB2 + kc + ldk);
+            const uint4* pl = reinterpret_cast<const uint4*>(plo + ga * NB2 + kc + ldk);
+            v[0] = ph[0], v[1] = ph[1], v[2] = pl[0], v[3] = pl[1];
+        } else {
+            v[0] = v[1] = v[2] = v[3] = z;
+        }
+        if (vb) {
+            const uint4* ph = reinterpret_cast<const uint4*>(phi + gb * NB2 + kc + ldk);
+            const uint4* pl = reinterpret_cast<const uint4*>(plo + gb * NB2 + kc + ldk);
+            v[4] = ph[0], v[5] = ph[1], v[6] = pl[0], v[7] = pl[1];
+        } else {
+            v[4] = v[5] = v[6] = v[7] = z;
+        }
+    };
+    auto stash = [&]() {
+        *reinterpret_cast<uint4*>(&sa[0][ldr][ldk]) = v[0];
+        *reinterpret_cast<uint4*>(&sa[0][ldr][ldk + 8]) = v[1];
+        *reinterpret_cast<uint4*>(&sa[1][ldr][ldk]) = v[2];
+        *reinterpret_cast<uint4*>(&sa[1][ldr][ldk + 8]) = v[3];
+        *reinterpret_cast<uint4*>(&sb[0][ldr][ldk]) = v[4];
+        *reinterpret_cast<uint4*>(&sb[0][ldr][ldk + 8]) = v[5];
+        *reinterpret_cast<uint4*>(&sb[1][ldr][ldk]) = v[6];
+        *reinterpret_cast<uint4*>(&sb[1][ldr][ldk + 8]) = v[7];
+    };
+    fetch(0);
+    stash();
+    __syncthreads();
+    const int ra = 64 * (w >> 1) + lr, rbb = 64 * (w & 1) + lr;
+    for (int kc = 0; kc < kw; kc += KCB) {
+        const bool more = kc + KCB < kw;
+        if (more) fetch(kc + KCB);
+        bf16x8 ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            ah[b] = *reinterpret_cast<const bf16x8*>(&sa[0][ra + 16 * b][lk]);
+            al[b] = *reinterpret_cast<const bf16x8*>(&sa[1][ra + 16 * b][lk]);
+            bh[b] = *reinterpret_cast<const bf16x8*>(&sb[0][rbb + 16 * b][lk]);
+            bl[b] = *reinterpret_cast<const bf16x8*>(&sb[1][rbb + 16 * b][lk]);
+        }
+        // the three products in separate sweeps over the sixteen accumulators,
+        // so consecutive MFMAs never wait on each other's results
+#pragma unroll
+        for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 4; ++bj)
+                acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[bi], bh[bj], acc[bi][bj], 0, 0, 0);
+#pragma unroll
+        for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 4; ++bj)
+                acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[bi], bl[bj], acc[bi][bj], 0, 0, 0);
+#pragma unroll
+        for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 4; ++bj)
+                acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[bi], bh[bj], acc[bi][bj], 0, 0, 0);
+        if (more) {
+            __syncthreads();  // every wave has read this chunk's operands
+            stash();
+            __syncthreads();
+        }
+    }
+    big_store(M, ld, mp, ce, r0, c0, acc);
+}
+
+// FP32 trailing update: operands are the panel columns [pc, pc+kw) of M.
+// LDS rows hold a K-chunk of 16 permuted so that lane group g = l/16 reads
+// k = g, g+4, g+8, g+12 (its four MFMA steps) as one float4.  The next
+// chunk is loaded into registers during the current chunk's MFMAs.
+__global__ __launch_bounds__(256) void k_gemm_f32(float* __restrict__ M, size_t ld, int rs, int cs, int ce, int mp,
+                                                  int pc, int kw) {
+    __shared__ __attribute__((aligned(16))) float sa[GT][LPF];
+    __shared__ __attribute__((aligned(16))) float sb[GT][LPF];
+    size_t r0, c0;
+    if (!big_tile(rs, cs, r0, c0)) return;
+    const int tid = threadIdx.x, l = tid & 63, w = wave_of();
+    const int lr = l & 15, lg = l >> 4;
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // loader: thread t moves 8 floats (two float4, k = 8h..8h+7) of row t/2
+    const int ldr = tid >> 1, h = tid & 1;
+    const size_t ga = r0 + ldr, gb = c0 + ldr;
+    const bool va = ga < (size_t)mp, vb = gb < (size_t)ce;
+    float4 a0, a1, b0, b1;
+    auto fetch = [&](int kc) {
+        a0 = a1 = b0 = b1 = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (va) {
+            const float4* pa = reinterpret_cast<const float4*>(M + ga * ld + pc + kc + 8 * h);
+            a0 = pa[0], a1 = pa[1];
+        }
+        if (vb) {
+            const float4* pb = reinterpret_cast<const float4*>(M + gb * ld + pc + kc + 8 * h);
+            b0 = pb[0], b1 = pb[1];
+        }
+    };
+    // element k (0..15) goes to position (k%4)*4 + k/4
+    auto stash = [&]() {
+        const int q0 = 2 * h, q1 = 2 * h + 1;  // k/4 of a0 and a1
+        sa[ldr][0 + q0] = a0.x, sa[ldr][4 + q0] = a0.y, sa[ldr][8 + q0] = a0.z, sa[ldr][12 + q0] = a0.w;
+        sa[ldr][0 + q1] = a1.x, sa[ldr][4 + q1] = a1.y, sa[ldr][8 + q1] = a1.z, sa[ldr][12 + q1] = a1.w;
+        sb[ldr][0 + q0] = b0.x, sb[ldr][4 + q0] = b0.y, sb[ldr][8 + q0] = b0.z, sb[ldr][12 + q0] = b0.w;
+        sb[ldr][0 + q1] = b1.x, sb[ldr][4 + q1] = b1.y, sb[ldr][8 + q1] = b1.z, sb[ldr][12 + q1] = b1.w;
+    };
+    fetch(0);
+    stash();
+    __syncthreads();
+    const int ra = 64 * (w >> 1) + lr, rbb = 64 * (w & 1) + lr;
+    for (int kc = 0; kc < kw; kc += KCF) {
+        const bool more = kc + KCF < kw;
+        if (more) fetch(kc + KCF);
+        float4 av[4], bv[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            av[b] = *reinterpret_cast<const float4*>(&sa[ra + 16 * b][4 * lg]);
+            bv[b] = *reinterpret_cast<const float4*>(&sb[rbb + 16 * b][4 * lg]);
+        }
+#pragma unroll
+        for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 4; ++bj)
+                acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[bi].x, bv[bj].x, acc[bi][bj], 0, 0, 0);
+#pragma unroll
+        for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 4; ++bj)
+                acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[bi].y, bv[bj].y, acc[bi][bj], 0, 0, 0);
+#pragma unroll
+        for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 4; ++bj)
+                acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[bi].z, bv[bj].z, acc[bi][bj], 0, 0, 0);
+#pragma unroll
+        for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 4; ++bj)
+                acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[bi].w, bv[bj].w, acc[bi][bj], 0, 0, 0);
+        if (more) {
+            __syncthreads();
+            stash();
+            __syncthreads();
+        }
+    }
+    big_store(M, ld, mp, ce, r0, c0, acc);
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -552,21 +805,72 @@ hipError_t launch_hi_tea_step(const DevJob* jobs, const DevJobState* st, const i
     return hipGetLastError();
 }
 
-hipError_t launch_chol_factor(float* M, size_t ld, int mp, int mode, double* linv, unsigned short* phi,
-                              unsigned short* plo, const int* flag, hipStream_t stream) {
+// Rows [rs, mp) x columns [cs, ce) of the trailing matrix minus the outer
+// panel (columns [pc, pc+kw)) times its transpose.
+static void trailing_update(float* M, size_t ld, int rs, int cs, int ce, int mp, int pc, int kw, int mode,
+                            const unsigned short* phi, const unsigned short* plo, hipStream_t stream) {
+    if (rs >= mp || cs >= ce) return;
+    const dim3 g((mp - rs + GT - 1) / GT, (ce - cs + GT - 1) / GT);
+    if (mode == SOP_CHOL_F32)
+        hipLaunchKernelGGL(k_gemm_f32, g, dim3(256), 0, stream, M, ld, rs, cs, ce, mp, pc, kw);
+    else
+        hipLaunchKernelGGL(k_gemm_bf16x3, g, dim3(256), 0, stream, M, ld, rs, cs, ce, mp, kw, phi, plo);
+}
+
+// Right-looking blocked factorisation with a lookahead of one outer panel.
+// After panel p is factored, the columns of panel p+1 are updated on the
+// main stream (T1) and the rest of the trailing matrix on `aux` (T2), so
+// panel p+1 is factored while T2(p) runs.  T1(p+1) and T2(p) both write
+// panel p+2's columns, so the main stream waits for T2(p) before T1(p+1).
+// With aux == nullptr everything runs on `stream`.
+hipError_t launch_chol_factor(float* M, size_t ld, int mp, int mode, double* ldiag, unsigned short* phi,
+                              unsigned short* plo, const int* flag, hipStream_t stream, hipStream_t aux,
+                              hipEvent_t ev_f, hipEvent_t ev_t2) {
     const int nblk = mp / NB;
-    for (int k = 0; k < nblk; ++k) {
-        hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(256), 0, stream, M, ld, k, linv, flag);
-        const int below = mp - (k + 1) * NB;
-        if (below <= 0) break;
-        hipLaunchKernelGGL(k_trsm, dim3((below + 255) / 256), dim3(256), 0, stream, M, ld, k, mp, linv, phi, plo,
-                           flag);
-        const int T = below / NB;
-        const int tiles = T * (T + 1) / 2;
-        if (mode == SOP_CHOL_F32)
-            hipLaunchKernelGGL(k_syrk_f32, dim3(tiles), dim3(256), 0, stream, M, ld, k, flag);
-        else
-            hipLaunchKernelGGL(k_syrk_bf16x3, dim3(tiles), dim3(256), 0, stream, M, ld, k, phi, plo, flag);
+    const bool two = aux != nullptr && ev_f != nullptr && ev_t2 != nullptr;
+    bool t2_pending = false;
+    for (int c0 = 0; c0 < mp; c0 += NB2) {
+        const int kw = mp - c0 < NB2 ? mp - c0 : NB2;
+        const int kb = c0 / NB, ke = (c0 + kw) / NB;  // inner blocks of this panel
+        // split-BF16 copies of consecutive panels alternate between two
+        // buffers: T2 of panel p still reads its copy while panel p+1 is factored
+        const size_t pbuf = (size_t)((c0 / NB2) & 1) * mp * NB2;
+        unsigned short* const ph = phi + pbuf;
+        unsigned short* const pl = plo + pbuf;
+        for (int k = kb; k < ke; ++k) {
+            hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(256), 0, stream, M, ld, k, ldiag, flag);
+            const int below = mp - (k + 1) * NB;
+            if (below <= 0) break;
+            hipLaunchKernelGGL(k_trsm, dim3((below + 63) / 64), dim3(256), 0, stream, M, ld, k, mp, ldiag, ph, pl,
+                               k * NB - c0, flag);
+            if (k + 1 < ke) {  // the panel's remaining columns
+                const dim3 g(nblk - (k + 1), ke - (k + 1));
+                if (mode == SOP_CHOL_F32)
+                    hipLaunchKernelGGL(k_syrk_f32, g, dim3(256), 0, stream, M, ld, k, nblk, flag);
+                else
+                    hipLaunchKernelGGL(k_syrk_bf16x3, g, dim3(256), 0, stream, M, ld, k, nblk, ph, pl,
+                                       k * NB - c0, flag);
+            }
+        }
+        const int s0 = c0 + kw;
+        if (s0 >= mp) break;
+        const int s1 = s0 + NB2 < mp ? s0 + NB2 : mp;
+        if (two) {
+            hipError_t e = hipEventRecord(ev_f, stream);  // panel p factored
+            if (e != hipSuccess) return e;
+            if (t2_pending && (e = hipStreamWaitEvent(stream, ev_t2, 0)) != hipSuccess) return e;
+            trailing_update(M, ld, s0, s0, s1, mp, c0, kw, mode, ph, pl, stream);
+            if ((e = hipStreamWaitEvent(aux, ev_f, 0)) != hipSuccess) return e;
+            trailing_update(M, ld, s1, s1, mp, mp, c0, kw, mode, ph, pl, aux);
+            if ((e = hipEventRecord(ev_t2, aux)) != hipSuccess) return e;
+            t2_pending = true;
+        } else {
+            trailing_update(M, ld, s0, s0, mp, mp, c0, kw, mode, ph, pl, stream);
+        }
+    }
+    if (t2_pending) {
+        hipError_t e = hipStreamWaitEvent(stream, ev_t2, 0);
+        if (e != hipSuccess) return e;
     }
     return hipGetLastError();
 }
@@ -574,7 +878,9 @@ hipError_t launch_chol_factor(float* M, size_t ld, int mp, int mode, double* lin
 hipError_t launch_chol_step(const DevJob* jobs, const DevJobState* st, const int2* tiles, int ntiles, int kstep,
                             const HiExactWork* w, hipStream_t stream) {
     hipLaunchKernelGGL(k_chol_D, dim3(grid_tiles(ntiles)), dim3(256), 0, stream, jobs, st, tiles, ntiles, kstep);
-    if (w) (void)launch_chol_factor(w->M, w->ld, w->mp, w->mode, w->linv, w->phi, w->plo, nullptr, stream);
+    if (w)
+        (void)launch_chol_factor(w->M, w->ld, w->mp, w->mode, w->linv, w->phi, w->plo, nullptr, stream, w->aux,
+                                 w->ev_f, w->ev_t2);
     hipLaunchKernelGGL(k_chol_L, dim3(grid_tiles(ntiles)), dim3(256), 0, stream, jobs, st, tiles, ntiles, kstep);
     return hipGetLastError();
 }

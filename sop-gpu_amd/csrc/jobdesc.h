@@ -48,9 +48,17 @@ struct DevJob {
     void* pair_j;   // slot-major: entry k of bead i at [k*n + i]
     int* pp_cnt;
     void* pp_j;     // bead-major: entry k of bead i at [i*pp_cap + k]
-    void* pair_tmp; // bead-major scratch of the Verlet build
     int* excl_cnt;  // sorted covalent+native partners (possible-pair exclusions)
     int* excl_j;
+    // cell grid of the possible-pairs build, for jobs of kCellMinBeads
+    // beads and more (null otherwise): per-cell counts and exclusive starts,
+    // the beads grouped by cell, each bead's cell and slot within it
+    int* cell_cnt;
+    int* cell_start;
+    int* cell_beads;
+    int* bead_cell;
+    int* bead_slot;
+    int cell_cap;
     int* pulled;    // bead indices of pulled beads
     float* energies; // 3n per-bead energies (output cadence)
     int cov_cap, nat_cap, pair_cap, pp_cap, excl_cap;
@@ -102,7 +110,14 @@ struct DevJobState {
     float tipDr;      // |chip - tip| of the last tip update, before the tip moved
     float beta;       // TEA beta' (bdhitea.cu:162-170)
     double eps_traj;  // TEA epsilon of the last update
+    float cell_org[3];  // possible-pairs cell grid of the last rebuild
+    float cell_inv;     // 1 / cell edge
+    int cell_dim[3];
 };
+
+// Jobs of this many beads and more build their possible-pairs list from a
+// cell grid instead of the all-pairs scan (same list, same order).
+constexpr int kCellMinBeads = 8192;
 
 // List entry k of the Verlet list (slot-major) / possible list (bead-major).
 __device__ __forceinline__ int list_at(const void* p, int idx16, size_t at) {

@@ -11,10 +11,15 @@ namespace sop {
 hipError_t launch_step(const DevJob* jobs, const DevJobState* st, const int2* tiles, int ntiles, int kstep,
                        int max_blocks, hipStream_t stream);
 // List rebuilds for jobs whose cadence is due (force != 0: rebuild all).
-hipError_t launch_possible(const DevJob* jobs, DevJobState* st, const int2* tiles, int ntiles, int force,
-                           hipStream_t stream);
-hipError_t launch_verlet(const DevJob* jobs, DevJobState* st, const int2* tiles, int ntiles, int force,
-                         hipStream_t stream);
+// cell_words: bitmap words (ceil(n/64)) of the largest job with a cell grid,
+// 0 to build every job's possible list by the all-pairs scan.
+hipError_t launch_possible(const DevJob* jobs, DevJobState* st, const int2* tiles, int ntiles, int njobs,
+                           int cell_words, int force, hipStream_t stream);
+// Largest cell_words launch_possible accepts (the LDS bitmap of one wavefront).
+constexpr int kMaxCellWords = 8192;
+// any_idx32: some job of the launch stores its lists as int32 (n >= 65536).
+hipError_t launch_verlet(const DevJob* jobs, DevJobState* st, const int2* tiles, int ntiles, int any_idx32,
+                         int force, hipStream_t stream);
 // Per-job updaters at a block start, and the step counter advance.
 hipError_t launch_job_update(const DevJob* jobs, DevJobState* st, int njobs, hipStream_t stream, int mask = 7);
 hipError_t launch_job_advance(const DevJob* jobs, DevJobState* st, int njobs, int nsteps, hipStream_t stream);

@@ -35,6 +35,7 @@ namespace sop {
 
 static constexpr int kTilesPerBlock = 4;  // 256-thread workgroups
 
+
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
 
 struct TileRange {
@@ -181,12 +182,11 @@ __global__ __launch_bounds__(256) void k_step(const DevJob* __restrict__ jobs, c
 #undef SOP_NAT
         }
         // repulsive LJ over the Verlet list (pairs_kernel.cu:41-56)
+        const int np = J.pair_cnt[i];
         if (J.idx16)
-            pair_loop(static_cast<const unsigned short*>(J.pair_j) + i, J.pair_cnt[i], n, cin, c, J.rc2, J.a2, J.m6,
-                      fx, fy, fz);
+            pair_loop(static_cast<const unsigned short*>(J.pair_j) + i, np, n, cin, c, J.rc2, J.a2, J.m6, fx, fy, fz);
         else
-            pair_loop(static_cast<const int*>(J.pair_j) + i, J.pair_cnt[i], n, cin, c, J.rc2, J.a2, J.m6, fx, fy,
-                      fz);
+            pair_loop(static_cast<const int*>(J.pair_j) + i, np, n, cin, c, J.rc2, J.a2, J.m6, fx, fy, fz);
         const DevJobState& S = st[td.x];
         // indentation tip and substrate plane (indentation_kernel.cu:71-128)
         if (J.flags & JF_INDENTATION) {
@@ -289,12 +289,14 @@ __device__ __forceinline__ unsigned long long lanes_below(int lane) {
 // bead's sorted covalent/native partners, turned into a 64-bit mask per
 // chunk by a scalar walk.
 __global__ __launch_bounds__(256) void k_possible(const DevJob* __restrict__ jobs, DevJobState* __restrict__ st,
-                                                  const int2* __restrict__ tiles, int ntiles, int force) {
+                                                  const int2* __restrict__ tiles, int ntiles, int force,
+                                                  int use_cells) {
     const int lane = threadIdx.x & 63;
     const TileRange tr = xcd_tiles(ntiles);
     for (int t = tr.begin; t < tr.end; t += tr.step) {
         const int2 td = tiles[t];
         const DevJob& J = jobs[td.x];
+        if (use_cells && J.cell_cnt) continue;  // built by k_possible_cells
         const long long s = st[td.x].step;
         if (!force && !due(J, s, J.pp_freq)) continue;
         const int n = J.n;
@@ -339,84 +341,401 @@ __global__ __launch_bounds__(256) void k_possible(const DevJob* __restrict__ job
     }
 }
 
-// Verlet list, one wavefront per bead: the bead's possible row filtered in
-// stored order by the same ballot compaction into a bead-major scratch row;
-// k_pairs_transpose then lays the rows out slot-major for k_step.
-__global__ __launch_bounds__(256) void k_verlet(const DevJob* __restrict__ jobs, DevJobState* __restrict__ st,
-                                                const int2* __restrict__ tiles, int ntiles, int force) {
+// ---------------------------------------------------------------------------
+// cell-binned possible pairs (large jobs)
+// ---------------------------------------------------------------------------
+// The list of k_possible -- the same predicate on the same operands, the
+// same ascending-j order -- from the candidates in the 27 cells around the
+// bead instead of the whole job.  Cells are cubes of edge thr*(1+1e-4) +
+// 0.01 A: a pair with sqrtf(d^2) < thr has every |coordinate difference|
+// within thr*(1 + 2^-21), and the cell index (x - x0) / edge is computed
+// with an error far below the margin, so such a pair always lies in
+// adjacent cells.  Accepted j are set in a per-wave LDS bitmap over the
+// job's beads, the bead's exclusions cleared, and the bitmap read back in
+// index order with a wave prefix sum of popcounts.
+
+__device__ __forceinline__ int cell_axis(float x, float org, float inv, int dim) {
+    const float q = (x - org) * inv;
+    if (!(q >= 0.0f)) return 0;  // below the grid, or NaN
+    if (q >= (float)dim) return dim - 1;
+    return min((int)q, dim - 1);
+}
+
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ int wave_inclusive_sum(int v, int lane) {
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(v, o);
+        if (lane >= o) v += u;
+    }
+    return v;
+}
+
+// Per job (one 1024-thread block each): bounding box of the coordinates,
+// the grid (edge grown until the cell count fits cell_cap), zeroed counts.
+__global__ __launch_bounds__(1024) void k_cells_setup(const DevJob* __restrict__ jobs, DevJobState* __restrict__ st,
+                                                      int njobs, int force) {
+    const int jb = blockIdx.x;
+    if (jb >= njobs) return;
+    const DevJob& J = jobs[jb];
+    if (!J.cell_cnt) return;
+    DevJobState& S = st[jb];
+    const long long s = S.step;
+    if (!force && !due(J, s, J.pp_freq)) return;
+    const float4* __restrict__ cur = J.coord[s & 1];
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int i = threadIdx.x; i < J.n; i += blockDim.x) {
+        const float4 r = cur[i];
+        lo[0] = fminf(lo[0], r.x), hi[0] = fmaxf(hi[0], r.x);
+        lo[1] = fminf(lo[1], r.y), hi[1] = fmaxf(hi[1], r.y);
+        lo[2] = fminf(lo[2], r.z), hi[2] = fmaxf(hi[2], r.z);
+    }
+    __shared__ float red[6][16];
+    __shared__ int dims[4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        float a = lo[d], b = hi[d];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            a = fminf(a, __shfl_xor(a, o));
+            b = fmaxf(b, __shfl_xor(b, o));
+        }
+        if (lane == 0) {
+            red[d][w] = a;
+            red[3 + d][w] = b;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = blockDim.x >> 6;
+        float org[3], ext[3];
+        for (int d = 0; d < 3; ++d) {
+            float a = red[d][0], b = red[3 + d][0];
+            for (int k = 1; k < nw; ++k) a = fminf(a, red[d][k]), b = fmaxf(b, red[3 + d][k]);
+            org[d] = a;
+            ext[d] = b - a;
+        }
+        float edge = J.pp_thr * 1.0001f + 0.01f;
+        int dm[3] = {1, 1, 1};
+        const bool finite = isfinite(ext[0]) && isfinite(ext[1]) && isfinite(ext[2]);
+        if (finite) {
+            for (int it = 0; it < 200; ++it) {
+                double cells = 1.0;
+                for (int d = 0; d < 3; ++d) cells *= floor((double)ext[d] / edge) + 1.0;
+                if (cells <= (double)J.cell_cap) break;
+                edge *= 1.1f;
+            }
+            for (int d = 0; d < 3; ++d) dm[d] = (int)floor((double)ext[d] / edge) + 1;
+            if ((long long)dm[0] * dm[1] * dm[2] > J.cell_cap) dm[0] = dm[1] = dm[2] = 1;
+        } else {
+            org[0] = org[1] = org[2] = 0.0f;
+        }
+        for (int d = 0; d < 3; ++d) {
+            S.cell_org[d] = org[d];
+            S.cell_dim[d] = dm[d];
+        }
+        S.cell_inv = 1.0f / edge;
+        dims[0] = dm[0] * dm[1] * dm[2];
+    }
+    __syncthreads();
+    const int nc = dims[0];
+    for (int c = threadIdx.x; c < nc; c += blockDim.x) J.cell_cnt[c] = 0;
+}
+
+// Each bead's cell and its slot there (the order within a cell is free:
+// the bitmap restores index order).
+__global__ __launch_bounds__(256) void k_cells_count(const DevJob* __restrict__ jobs, DevJobState* __restrict__ st,
+                                                     const int2* __restrict__ tiles, int ntiles, int force) {
     const int lane = threadIdx.x & 63;
     const TileRange tr = xcd_tiles(ntiles);
     for (int t = tr.begin; t < tr.end; t += tr.step) {
         const int2 td = tiles[t];
         const DevJob& J = jobs[td.x];
-        const long long s = st[td.x].step;
-        if (!force && !due(J, s, J.pairs_freq)) continue;
+        if (!J.cell_cnt) continue;
+        const DevJobState& S = st[td.x];
+        const long long s = S.step;
+        if (!force && !due(J, s, J.pp_freq)) continue;
+        const int i = td.y + lane;
+        if (i >= J.n) continue;
+        const float4 r = J.coord[s & 1][i];
+        const int cx = cell_axis(r.x, S.cell_org[0], S.cell_inv, S.cell_dim[0]);
+        const int cy = cell_axis(r.y, S.cell_org[1], S.cell_inv, S.cell_dim[1]);
+        const int cz = cell_axis(r.z, S.cell_org[2], S.cell_inv, S.cell_dim[2]);
+        const int c = (cz * S.cell_dim[1] + cy) * S.cell_dim[0] + cx;
+        J.bead_cell[i] = c;
+        J.bead_slot[i] = atomicAdd(&J.cell_cnt[c], 1);
+    }
+}
+
+// Exclusive scan of the cell counts (one 1024-thread block per job).
+__global__ __launch_bounds__(1024) void k_cells_scan(const DevJob* __restrict__ jobs, DevJobState* __restrict__ st,
+                                                     int njobs, int force) {
+    const int jb = blockIdx.x;
+    if (jb >= njobs) return;
+    const DevJob& J = jobs[jb];
+    if (!J.cell_cnt) return;
+    const DevJobState& S = st[jb];
+    if (!force && !due(J, S.step, J.pp_freq)) return;
+    const int nc = S.cell_dim[0] * S.cell_dim[1] * S.cell_dim[2];
+    const int per = (nc + blockDim.x - 1) / blockDim.x;
+    const int b = threadIdx.x * per, e = min(b + per, nc);
+    int sum = 0;
+    for (int c = b; c < e; ++c) sum += J.cell_cnt[c];
+    __shared__ int part[1024];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int k = 0; k < (int)blockDim.x; ++k) {
+            const int v = part[k];
+            part[k] = acc;
+            acc += v;
+        }
+    }
+    __syncthreads();
+    int acc = part[threadIdx.x];
+    for (int c = b; c < e; ++c) {
+        J.cell_start[c] = acc;
+        acc += J.cell_cnt[c];
+    }
+    if (threadIdx.x == 0) J.cell_start[nc] = J.n;
+}
+
+__global__ __launch_bounds__(256) void k_cells_fill(const DevJob* __restrict__ jobs, DevJobState* __restrict__ st,
+                                                    const int2* __restrict__ tiles, int ntiles, int force) {
+    const int lane = threadIdx.x & 63;
+    const TileRange tr = xcd_tiles(ntiles);
+    for (int t = tr.begin; t < tr.end; t += tr.step) {
+        const int2 td = tiles[t];
+        const DevJob& J = jobs[td.x];
+        if (!J.cell_cnt) continue;
+        if (!force && !due(J, st[td.x].step, J.pp_freq)) continue;
+        const int i = td.y + lane;
+        if (i >= J.n) continue;
+        J.cell_beads[J.cell_start[J.bead_cell[i]] + J.bead_slot[i]] = i;
+    }
+}
+
+// One 64-thread workgroup (one wavefront) per tile; dynamic LDS holds the
+// bitmap of ceil(n/64) words for the largest cell-binned job.
+__global__ __launch_bounds__(64) void k_possible_cells(const DevJob* __restrict__ jobs, DevJobState* __restrict__ st,
+                                                       const int2* __restrict__ tiles, int ntiles, int force) {
+    extern __shared__ unsigned long long bm[];
+    const int lane = threadIdx.x;
+    const int nb = gridDim.x, nxcd = nb >= 8 ? 8 : 1;
+    const int slice = (ntiles + nxcd - 1) / nxcd;
+    const int xcd = blockIdx.x % nxcd;
+    const int tend = min((xcd + 1) * slice, ntiles);
+    for (int t = xcd * slice + blockIdx.x / nxcd; t < tend; t += nb / nxcd) {
+        const int2 td = tiles[t];
+        const DevJob& J = jobs[td.x];
+        if (!J.cell_cnt) continue;
+        const DevJobState& S = st[td.x];
+        const long long s = S.step;
+        if (!force && !due(J, s, J.pp_freq)) continue;
         const int n = J.n;
+        const int nwords = (n + 63) >> 6;
+        for (int w = lane; w < nwords; w += 64) bm[w] = 0ull;
+        __syncthreads();
         const float4* __restrict__ cur = J.coord[s & 1];
-        const float cut = J.pl_cut;
-        const int cap = J.pair_cap;
+        const float thr = J.pp_thr;
+        const int cap = J.pp_cap;
+        const int dx = S.cell_dim[0], dy = S.cell_dim[1], dz = S.cell_dim[2];
         const int iend = min(td.y + 64, n);
         for (int i = td.y; i < iend; ++i) {
             const float4 r1 = cur[i];
-            const int npp = J.pp_cnt[i];
-            const size_t inrow = (size_t)i * J.pp_cap, outrow = (size_t)i * cap;
-            int cnt = 0;
-            for (int base = 0; base < npp; base += 64) {
-                const int k = base + lane;
-                bool ok = k < npp;
-                int j = 0;
-                if (ok) {
-                    j = list_at(J.pp_j, J.idx16, inrow + k);
-                    const float4 r2 = cur[j];
-                    const float x = r2.x - r1.x, y = r2.y - r1.y, z = r2.z - r1.z;
-                    ok = sqrtf(x * x + y * y + z * z) < cut;
+            const int cx = cell_axis(r1.x, S.cell_org[0], S.cell_inv, dx);
+            const int cy = cell_axis(r1.y, S.cell_org[1], S.cell_inv, dy);
+            const int cz = cell_axis(r1.z, S.cell_org[2], S.cell_inv, dz);
+            int jlo = INT_MAX, jhi = -1;
+            for (int z = max(cz - 1, 0); z <= min(cz + 1, dz - 1); ++z)
+                for (int y = max(cy - 1, 0); y <= min(cy + 1, dy - 1); ++y) {
+                    const int row = (z * dy + y) * dx;
+                    // the cells x-1..x+1 of one row are contiguous in cell_beads
+                    const int b = J.cell_start[row + max(cx - 1, 0)];
+                    const int e = J.cell_start[row + min(cx + 1, dx - 1) + 1];
+                    for (int k = b + lane; k < e; k += 64) {
+                        const int j = J.cell_beads[k];
+                        if (j == i) continue;
+                        const float4 r2 = cur[j];
+                        const float x = r2.x - r1.x, y2 = r2.y - r1.y, z2 = r2.z - r1.z;
+                        if (sqrtf(x * x + y2 * y2 + z2 * z2) < thr) {
+                            atomicOr(&bm[j >> 6], 1ull << (j & 63));
+                            jlo = min(jlo, j);
+                            jhi = max(jhi, j);
+                        }
+                    }
                 }
-                const unsigned long long m = __ballot(ok);
-                if (ok) {
-                    const int pos = cnt + __popcll(m & lanes_below(lane));
-                    if (pos < cap) list_put(J.pair_tmp, J.idx16, outrow + pos, j);
-                }
-                cnt += __popcll(m);
+            __syncthreads();
+            const int ne = J.excl_cnt[i];
+            for (int k = lane; k < ne; k += 64) {
+                const int e = J.excl_j[(size_t)k * n + i];
+                atomicAnd(&bm[e >> 6], ~(1ull << (e & 63)));
             }
+            __syncthreads();
+            jlo = wave_min(jlo);
+            jhi = wave_max(jhi);
+            int cnt = 0;
+            const size_t rowp = (size_t)i * cap;
+            if (jhi >= 0) {
+                const int wlo = jlo >> 6, whi = jhi >> 6;
+                for (int wb = wlo; wb <= whi; wb += 64) {
+                    const int w = wb + lane;
+                    unsigned long long word = 0ull;
+                    if (w <= whi) {
+                        word = bm[w];
+                        bm[w] = 0ull;
+                    }
+                    const int pc = __popcll(word);
+                    const int incl = wave_inclusive_sum(pc, lane);
+                    int pos = cnt + incl - pc;
+                    while (word) {
+                        const int j = (w << 6) + __builtin_ctzll(word);
+                        if (pos < cap) list_put(J.pp_j, J.idx16, rowp + pos, j);
+                        ++pos;
+                        word &= word - 1ull;
+                    }
+                    cnt += __shfl(incl, 63);
+                }
+            }
+            __syncthreads();
             if (lane == 0) {
-                J.pair_cnt[i] = cnt < cap ? cnt : cap;
-                if (cnt > cap) atomicOr(&st[td.x].overflow, 2);
+                J.pp_cnt[i] = cnt < cap ? cnt : cap;
+                if (cnt > cap) atomicOr(&st[td.x].overflow, 1);
             }
         }
     }
 }
 
-// Bead-major scratch rows -> slot-major Verlet list (pair_j[k*n + i]), one
-// 64-bead tile per 64-thread workgroup, 64x64 blocks through LDS so both the
-// reads and the writes are whole rows.
-__global__ __launch_bounds__(64) void k_pairs_transpose(const DevJob* __restrict__ jobs,
-                                                        const DevJobState* __restrict__ st,
-                                                        const int2* __restrict__ tiles, int ntiles, int force) {
-    __shared__ int buf[64][65];
-    const int lane = threadIdx.x;
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+// Verlet list: one 256-thread workgroup per 64-bead tile.  Wave w filters
+// the possible rows of beads 16w..16w+15 in stored order with the ballot
+// compaction; the accepted entries of the tile collect in LDS rows, and the
+// workgroup then writes them out slot-major (pair_j[k*n + i]) so each store
+// covers the 64 beads of one slot.  A bead's row is read eight 64-entry
+// chunks at a time: all eight index loads, then all eight coordinate
+// gathers, are in flight before the first compare.  Entries past the LDS
+// row (VerletLds::entries) are rare and stored to their slot directly.
+template <typename IdxT>
+struct VerletLds;
+template <>
+struct VerletLds<unsigned short> {  // 256 entries per bead, rows padded by one bank
+    static constexpr int entries = 256, stride = 258;
+};
+template <>
+struct VerletLds<int> {
+    static constexpr int entries = 255, stride = 256;
+};
+
+template <typename IdxT>
+__device__ __forceinline__ void verlet_tile(const DevJob& J, DevJobState& S, const float4* __restrict__ cur, int i0,
+                                            IdxT* __restrict__ rows, int* __restrict__ cnt_s) {
+    constexpr int E = VerletLds<IdxT>::entries, RS = VerletLds<IdxT>::stride;
+    constexpr int CH = 8;  // chunks of 64 entries whose loads are in flight together
+    const int lane = threadIdx.x & 63, w = wave_id();
+    const int n = J.n;
+    const int nrow = min(64, n - i0);
+    const int b_lo = 16 * w, b_hi = min(b_lo + 16, nrow);
+    const float cut = J.pl_cut;
+    const int cap = J.pair_cap, ppcap = J.pp_cap;
+    const IdxT* __restrict__ pp = static_cast<const IdxT*>(J.pp_j);
+    IdxT* __restrict__ pj = static_cast<IdxT*>(J.pair_j);
+    const int cl = lane < nrow ? J.pp_cnt[i0 + lane] : 0;  // possible counts of the tile
+    for (int b = b_lo; b < b_hi; ++b) {
+        const int npp = __shfl(cl, b);
+        const float4 r1 = cur[i0 + b];
+        const IdxT* __restrict__ row = pp + (size_t)(i0 + b) * ppcap;
+        int cnt = 0;
+        for (int base = 0; base < npp; base += 64 * CH) {
+            int j[CH];
+#pragma unroll
+            for (int u = 0; u < CH; ++u) {
+                const int k = base + 64 * u + lane;
+                j[u] = k < npp ? (int)row[k] : -1;
+            }
+            float4 c[CH];
+#pragma unroll
+            for (int u = 0; u < CH; ++u) c[u] = j[u] >= 0 ? cur[j[u]] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int u = 0; u < CH; ++u) {
+                if (base + 64 * u >= npp) break;
+                bool ok = false;
+                if (j[u] >= 0) {
+                    const float x = c[u].x - r1.x, y = c[u].y - r1.y, z = c[u].z - r1.z;
+                    ok = sqrtf(x * x + y * y + z * z) < cut;
+                }
+                const unsigned long long m = __ballot(ok);
+                if (ok) {
+                    const int pos = cnt + __popcll(m & lanes_below(lane));
+                    if (pos < E) rows[b * RS + pos] = (IdxT)j[u];
+                    else if (pos < cap) pj[(size_t)pos * n + i0 + b] = (IdxT)j[u];
+                }
+                cnt += __popcll(m);
+            }
+        }
+        if (lane == 0) {
+            cnt_s[b] = cnt;
+            if (cnt > cap) atomicOr(&S.overflow, 2);
+        }
+    }
+}
+
+template <typename IdxT>
+__device__ __forceinline__ void verlet_store(const DevJob& J, int i0, const IdxT* __restrict__ rows,
+                                             const int* __restrict__ cnt_s) {
+    constexpr int E = VerletLds<IdxT>::entries, RS = VerletLds<IdxT>::stride;
+    const int n = J.n, cap = J.pair_cap;
+    const int nrow = min(64, n - i0);
+    const int b = threadIdx.x & 63, kq = threadIdx.x >> 6;
+    const int c = b < nrow ? cnt_s[b] : 0;
+    if (b < nrow) {
+        if (kq == 0) J.pair_cnt[i0 + b] = c < cap ? c : cap;
+    }
+    int maxc = min(c, E);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) maxc = max(maxc, __shfl_xor(maxc, o));
+    // the four waves' maxima differ only in which beads they saw: all waves
+    // see all 64 beads here (b = lane), so maxc is the tile's
+    IdxT* __restrict__ pj = static_cast<IdxT*>(J.pair_j);
+    const int lim = min(c, E);
+    for (int k = kq; k < maxc; k += 4)
+        if (k < lim) pj[(size_t)k * n + i0 + b] = rows[b * RS + k];
+}
+
+__global__ __launch_bounds__(256) void k_verlet(const DevJob* __restrict__ jobs, DevJobState* __restrict__ st,
+                                                const int2* __restrict__ tiles, int ntiles, int force) {
+    extern __shared__ __attribute__((aligned(16))) int lds_rows[];  // 64 rows of uint16 or int32 entries
+    __shared__ int cnt_s[64];
+    const int nb = gridDim.x, nxcd = nb >= 8 ? 8 : 1;
+    const int slice = (ntiles + nxcd - 1) / nxcd;
+    const int xcd = blockIdx.x % nxcd;
+    const int tend = min((xcd + 1) * slice, ntiles);
+    for (int t = xcd * slice + blockIdx.x / nxcd; t < tend; t += nb / nxcd) {
         const int2 td = tiles[t];
         const DevJob& J = jobs[td.x];
-        const long long s = st[td.x].step;
+        DevJobState& S = st[td.x];
+        const long long s = S.step;
         if (!force && !due(J, s, J.pairs_freq)) continue;
-        const int n = J.n;
-        const int cap = J.pair_cap;
-        const int rows = min(64, n - td.y);
-        int c = lane < rows ? J.pair_cnt[td.y + lane] : 0;
-        int maxc = c;
-        for (int off = 32; off > 0; off >>= 1) maxc = max(maxc, __shfl_xor(maxc, off));
-        for (int kc = 0; kc < maxc; kc += 64) {
-            __syncthreads();
-            for (int b = 0; b < rows; ++b) {
-                const int k = kc + lane;
-                const int cb = __shfl(c, b);
-                buf[b][lane] = k < cb ? list_at(J.pair_tmp, J.idx16, (size_t)(td.y + b) * cap + k) : 0;
-            }
-            __syncthreads();
-            const int kend = min(64, maxc - kc);
-            for (int k = 0; k < kend; ++k)
-                if (lane < rows && kc + k < c) list_put(J.pair_j, J.idx16, (size_t)(kc + k) * n + td.y + lane, buf[lane][k]);
-        }
+        const float4* __restrict__ cur = J.coord[s & 1];
+        if (J.idx16)
+            verlet_tile(J, S, cur, td.y, reinterpret_cast<unsigned short*>(lds_rows), cnt_s);
+        else
+            verlet_tile(J, S, cur, td.y, lds_rows, cnt_s);
+        __syncthreads();
+        if (J.idx16)
+            verlet_store(J, td.y, reinterpret_cast<const unsigned short*>(lds_rows), cnt_s);
+        else
+            verlet_store(J, td.y, static_cast<const int*>(lds_rows), cnt_s);
+        __syncthreads();
     }
 }
 
@@ -674,18 +993,35 @@ hipError_t launch_step(const DevJob* jobs, const DevJobState* st, const int2* ti
     return hipGetLastError();
 }
 
-hipError_t launch_possible(const DevJob* jobs, DevJobState* st, const int2* tiles, int ntiles, int force,
-                           hipStream_t stream) {
+hipError_t launch_possible(const DevJob* jobs, DevJobState* st, const int2* tiles, int ntiles, int njobs,
+                           int cell_words, int force, hipStream_t stream) {
     if (ntiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_possible, dim3(grid_for(ntiles, 0)), dim3(256), 0, stream, jobs, st, tiles, ntiles, force);
+    const int use_cells = cell_words > 0;
+    hipLaunchKernelGGL(k_possible, dim3(grid_for(ntiles, 0)), dim3(256), 0, stream, jobs, st, tiles, ntiles, force,
+                       use_cells);
+    if (use_cells) {
+        hipLaunchKernelGGL(k_cells_setup, dim3(njobs), dim3(1024), 0, stream, jobs, st, njobs, force);
+        hipLaunchKernelGGL(k_cells_count, dim3(grid_for(ntiles, 0)), dim3(256), 0, stream, jobs, st, tiles, ntiles,
+                           force);
+        hipLaunchKernelGGL(k_cells_scan, dim3(njobs), dim3(1024), 0, stream, jobs, st, njobs, force);
+        hipLaunchKernelGGL(k_cells_fill, dim3(grid_for(ntiles, 0)), dim3(256), 0, stream, jobs, st, tiles, ntiles,
+                           force);
+        int blocks = ntiles;
+        if (blocks >= 8) blocks = (blocks + 7) / 8 * 8;
+        hipLaunchKernelGGL(k_possible_cells, dim3(blocks), dim3(64), (size_t)cell_words * 8, stream, jobs, st, tiles,
+                           ntiles, force);
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_verlet(const DevJob* jobs, DevJobState* st, const int2* tiles, int ntiles, int force,
-                         hipStream_t stream) {
+hipError_t launch_verlet(const DevJob* jobs, DevJobState* st, const int2* tiles, int ntiles, int any_idx32,
+                         int force, hipStream_t stream) {
     if (ntiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_verlet, dim3(grid_for(ntiles, 0)), dim3(256), 0, stream, jobs, st, tiles, ntiles, force);
-    hipLaunchKernelGGL(k_pairs_transpose, dim3(ntiles), dim3(64), 0, stream, jobs, st, tiles, ntiles, force);
+    int blocks = ntiles;
+    if (blocks >= 8) blocks = (blocks + 7) / 8 * 8;
+    const size_t lds = any_idx32 ? 64 * VerletLds<int>::stride * sizeof(int)
+                                 : 64 * VerletLds<unsigned short>::stride * sizeof(unsigned short);
+    hipLaunchKernelGGL(k_verlet, dim3(blocks), dim3(256), lds, stream, jobs, st, tiles, ntiles, force);
     return hipGetLastError();
 }
 
