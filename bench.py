@@ -258,6 +258,50 @@ def hi_phase(S, n=4096, reps=5, steps=50):
     return out
 
 
+def capsid_phase(S, steps=2000):
+    """Continuous indentation of a 100,000-bead capsid-like shell (200 chains
+    x 500 beads on a sphere of R = 140 A), tip R = 50 A moving at 1 um/s,
+    Verlet/possible rebuilds every 10/100 steps: stepping rate with the
+    cell-binned possible-pairs build, and one possible-pairs rebuild with
+    cells vs the all-pairs scan."""
+    sysm = S.synth_shell(200, 500, seed=2026, radius=140.0)
+    c = sysm.coords[:, :3]
+    top, bot = float(c[:, 2].max()), float(c[:, 2].min())
+    tip_r = 50.0
+    dx = 1.0 * 1e-8 * 1000 * H_TAU * TAU_H_PS  # 1 um/s, tip moved every nav = 1000 steps
+    p = S.make_params(kspring_cov=14.0, R_limit=2.0, pairs_cutoff=20.0, pairlist_cutoff=20.0, pairs_threshold=32.0,
+                      pairs_freq=10, possiblepairs_freq=100, max_possiblePairs=1024, timestep=H_TAU,
+                      temperature=0.6, seed=2026, nav=1000, indentation=1,
+                      indTip=(0.0, 0.0, top + tip_r + 4.0), indChip=(0.0, 0.0, top + tip_r + 4.0),
+                      indDirection=(0.0, 0.0, -1.0), indSurfR0=(0.0, 0.0, bot - 4.0), indSurfN=(0.0, 0.0, 1.0),
+                      indTipR=tip_r, indKs=0.07, indDeltaX=dx, indTipZeta=5000.0, indTipA=0.0, indTipB=1.0)
+    eng = S.Engine(0, 32 << 30)
+    jid = eng.add_job(p, sysm, steps + 100)
+    eng.set_tuning(0, True)
+    eng.run_blocks(10)  # warm-up: 100 steps incl. one possible-pairs rebuild
+    eng.synchronize()
+    L = eng.block_steps()
+    nb = steps // L
+    t0 = time.perf_counter()
+    eng.run_blocks(nb)
+    eng.synchronize()
+    dt = time.perf_counter() - t0
+    out = {"beads": sysm.n, "natives": int(len(sysm.nat_r0)), "steps": nb * L,
+           "bead_steps_per_s": sysm.n * nb * L / dt, "ms_per_step": dt / (nb * L) * 1e3}
+    for cells, name in ((True, "possible_rebuild_ms_cells"), (False, "possible_rebuild_ms_all_pairs")):
+        eng.set_cells(enabled=cells)
+        eng.rebuild(jid, 1)
+        eng.synchronize()
+        t0 = time.perf_counter()
+        eng.rebuild(jid, 1)
+        eng.synchronize()
+        out[name] = (time.perf_counter() - t0) * 1e3
+    st = eng.status(jid)
+    out["overflow"] = st[2]
+    eng.close()
+    return out
+
+
 def bytes_per_bead_step(avg):
     """Algorithmic HBM bytes per bead-step of the fused step kernel:
     coordinates read+write 32, RNG state read+write 32, three list counts 12,
@@ -305,6 +349,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-hi", action="store_true")
+    ap.add_argument("--no-capsid", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=60)
     ap.add_argument("--serve-steps-lo", type=int, default=2000)
     ap.add_argument("--serve-steps-hi", type=int, default=20000)
@@ -396,6 +441,11 @@ def main():
         print(f"# serving: p50 {serving['p50_latency_s']:.3f} s p99 {serving['p99_latency_s']:.3f} s",
               file=sys.stderr, flush=True)
 
+    capsid = None
+    if not args.no_capsid:
+        print("# capsid indentation phase", file=sys.stderr, flush=True)
+        capsid = capsid_phase(S)
+
     hi = None
     if not args.no_hi:
         print("# exact-HI phase", file=sys.stderr, flush=True)
@@ -435,6 +485,7 @@ def main():
                      "avg_lists": {"covalent": avg_lists[0], "native": avg_lists[1], "verlet": avg_lists[2],
                                    "possible": avg_lists[3]}},
         "serving": serving,
+        "capsid_indentation": capsid,
         "hi_cholesky": hi,
         "cpu_baseline": cpu,
     }

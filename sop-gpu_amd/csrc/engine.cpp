@@ -2,6 +2,7 @@
 #include "engine.h"
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -92,6 +93,22 @@ void Arena::release(void* p) {
         }
     }
     c.free[off] = len;
+}
+
+// Largest float x with sqrtf(x) < t (-1 when no x >= 0 qualifies).
+// IEEE sqrtf is correctly rounded on the host and in the kernels (built
+// with correctly rounded divide/sqrt) and monotone, so for any float d2,
+// sqrtf(d2) < t holds exactly when d2 <= sqrt_below_bound(t).
+float sqrt_below_bound(float t) {
+    if (!(t > 0.0f) || !std::isfinite(t)) return t > 0.0f ? FLT_MAX : -1.0f;
+    float x = t * t;
+    while (x > 0.0f && !(std::sqrt(x) < t)) x = std::nextafter(x, 0.0f);
+    for (;;) {
+        const float y = std::nextafter(x, INFINITY);
+        if (!(std::sqrt(y) < t)) break;
+        x = y;
+    }
+    return x;
 }
 
 // ---------------------------------------------------------------------------
@@ -282,6 +299,8 @@ static int build_job(const sop_params& p, const sop_system& s, long long numstep
     J.el = p.el;
     J.pl_cut = p.pairlist_cutoff;
     J.pp_thr = p.pairs_threshold;
+    J.pl_d2 = sqrt_below_bound(J.pl_cut);
+    J.pp_d2 = sqrt_below_bound(J.pp_thr);
     J.h = p.timestep;
     J.zeta = p.zeta;
     J.hOverZeta = p.timestep / p.zeta;

@@ -170,4 +170,6 @@ class Engine {
     size_t scratch4_n_ = 0;
 };
 
+float sqrt_below_bound(float t);
+
 }  // namespace sop

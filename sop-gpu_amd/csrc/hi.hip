@@ -370,12 +370,37 @@ __device__ __forceinline__ unsigned short f2bf(float x) {  // round to nearest e
 }
 __device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float(((unsigned int)h) << 16); }
 
-// Factor the k-th diagonal block in FP64 in LDS, 256 threads.  Step j
-// subtracts (a_rj/sqrt(p_j)) (a_cj/sqrt(p_j)) from the entries j < c <= r,
-// reading the unscaled column j, then column j is scaled; the scaling
-// touches no element the next step reads, so each step needs one barrier.
-// Writes L to M (FP32) and to ldiag (FP64, zero above the diagonal) for the
-// panel solve.
+// A lane's double from another lane of the wavefront (uniform lane index).
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// 16-column steps of the diagonal factorisation, run by one wavefront on
+// registers: lanes 0..15 hold the rows of the 16x16 diagonal sub-block,
+// lanes 16..63 the rows below it.  Step J (a template parameter, so every
+// register index is a constant) scales column J by 1/L[J][J] and subtracts
+// L[r][J] L[c][J] from the columns c > J, with L[c][J] read from lane c:
+// for lanes below the sub-block this is the forward substitution of the
+// panel solve, so both run as one pass.  Lanes 0..15 accumulate scratch
+// above their diagonal, which is never written back.
+template <int J>
+__device__ __forceinline__ void chol16_step(double (&v)[16], int r) {
+    const double d = sqrt(readlane_d(v[J], J));
+    const double rd = 1.0 / d;
+    v[J] = r == J ? d : v[J] * rd;
+#pragma unroll
+    for (int c = J + 1; c < 16; ++c) v[c] = fma(-v[J], readlane_d(v[J], c), v[c]);
+    if constexpr (J + 1 < 16) chol16_step<J + 1>(v, r);
+}
+
+// Factor the k-th diagonal block in FP64: four 16-column sub-blocks, each
+// factored together with the rows below it by wavefront 0 in registers
+// (no barrier inside), then a rank-16 update of the rest of the block by
+// all 256 threads.  Writes L to M (FP32) and to ldiag (FP64, zero above
+// the diagonal) for the panel solve.
 __global__ __launch_bounds__(256) void k_potrf_diag(float* __restrict__ M, size_t ld, int k, double* __restrict__ ldiag,
                                                     const int* __restrict__ flag) {
     if (flag && !*flag) return;
@@ -387,37 +412,53 @@ __global__ __launch_bounds__(256) void k_potrf_diag(float* __restrict__ M, size_
         s[r][c] = c <= r ? (double)M[(o + r) * ld + o + c] : 0.0;
     }
     __syncthreads();
-    for (int j = 0; j < NB; ++j) {
-        const double d = sqrt(s[j][j]);
-        const double rd = 1.0 / d;
-        // every operand of the thread's (up to) 4x4 entries is read before any
-        // entry is written, so the reads are issued back to back
-        double lr[4], lc[4], t[4][4];
+#pragma unroll 1
+    for (int sb = 0; sb < NB / 16; ++sb) {
+        const int q0 = 16 * sb;
+        if (tid < 64) {
+            const int r = tid, row = q0 + r;  // lanes 0..15: sub-block rows; 16..63: rows below
+            const bool live = row < NB;
+            double v[16];
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            const int r = j + 1 + ty + 16 * a, c = j + 1 + tx + 16 * a;
-            lr[a] = r < NB ? s[r][j] : 0.0;
-            lc[a] = c < NB ? s[c][j] : 0.0;
+            for (int c = 0; c < 16; ++c) v[c] = live && (r >= 16 || c <= r) ? s[row][q0 + c] : 0.0;
+            chol16_step<0>(v, r);
+            if (live) {
+#pragma unroll
+                for (int c = 0; c < 16; ++c)
+                    if (r >= 16 || c <= r) s[row][q0 + c] = v[c];
+            }
+        }
+        __syncthreads();
+        // rank-16 update of rows/columns q0+16 .. 63
+        const int b0 = q0 + 16;
+        double acc[3][3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) acc[a][b] = 0.0;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            double lr[3], lc[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const int r = b0 + ty + 16 * a, c = b0 + tx + 16 * a;
+                lr[a] = r < NB ? s[r][q0 + q] : 0.0;
+                lc[a] = c < NB ? s[c][q0 + q] : 0.0;
+            }
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b) acc[a][b] = fma(lr[a], lc[b], acc[a][b]);
         }
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
+        for (int a = 0; a < 3; ++a)
 #pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const int r = j + 1 + ty + 16 * a, c = j + 1 + tx + 16 * b;
-                t[a][b] = (r < NB && c <= r) ? s[r][c] : 0.0;
-            }
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const int r = j + 1 + ty + 16 * a, c = j + 1 + tx + 16 * b;
-                if (r < NB && c <= r) s[r][c] = fma(-(lr[a] * rd), lc[b] * rd, t[a][b]);
+            for (int b = 0; b < 3; ++b) {
+                const int r = b0 + ty + 16 * a, c = b0 + tx + 16 * b;
+                if (r < NB && c <= r) s[r][c] -= acc[a][b];
             }
         __syncthreads();
-        if (tid > j && tid < NB) s[tid][j] *= rd;
-        if (tid == j) s[j][j] = d;
     }
-    __syncthreads();
     for (int e = tid; e < NB * NB; e += 256) {
         const int r = e / NB, c = e % NB;
         if (c <= r) M[(o + r) * ld + o + c] = (float)s[r][c];

@@ -68,6 +68,9 @@ struct DevJob {
     float ks, R2;                     // FENE: kspring_cov, R_limit^2
     float a2, m6, rc2, el;            // repulsive LJ: a^2, -6 el/a^2, cutoff^2
     float pl_cut, pp_thr;             // Verlet cutoff, possible-pairs threshold
+    // largest float d2 with sqrtf(d2) < pl_cut (pp_thr): sqrtf is monotone, so
+    // the lists' predicate sqrtf(d2) < cut is exactly d2 <= pl_d2 (pp_d2)
+    float pl_d2, pp_d2;
     float hOverZeta, tempNorm;
     float h, zeta;
     // cadence

@@ -283,11 +283,15 @@ __device__ __forceinline__ unsigned long long lanes_below(int lane) {
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
-// Possible pairs, one wavefront per bead: lanes test 64 consecutive j at a
-// time; an order-preserving ballot/popcount compaction appends the accepted
-// j to the bead's row (bead-major, pp_j[i*pp_cap + k]).  Exclusions are the
-// bead's sorted covalent/native partners, turned into a 64-bit mask per
-// chunk by a scalar walk.
+// Possible pairs by the all-pairs scan, one wavefront per 64-bead tile:
+// the job's beads are visited in chunks of 64 (one coordinate per lane,
+// loaded once), and every bead i of the tile tests the chunk with its own
+// coordinates read from the lane holding them.  The order-preserving
+// ballot compaction appends the accepted j to bead i's row (bead-major,
+// pp_j[i*pp_cap + k]); lane i keeps bead i's running count.  Exclusions
+// (the sorted covalent/native partners) become, per chunk, a 64-bit mask
+// built by lane i from its own list.  The predicate sqrtf(d^2) < threshold
+// is evaluated as d^2 <= pp_d2 (the same set of floats).
 __global__ __launch_bounds__(256) void k_possible(const DevJob* __restrict__ jobs, DevJobState* __restrict__ st,
                                                   const int2* __restrict__ tiles, int ntiles, int force,
                                                   int use_cells) {
@@ -301,42 +305,49 @@ __global__ __launch_bounds__(256) void k_possible(const DevJob* __restrict__ job
         if (!force && !due(J, s, J.pp_freq)) continue;
         const int n = J.n;
         const float4* __restrict__ cur = J.coord[s & 1];
-        const float thr = J.pp_thr;
+        const float d2max = J.pp_d2;
         const int cap = J.pp_cap;
-        const int iend = min(td.y + 64, n);
-        for (int i = td.y; i < iend; ++i) {
-            const float4 r1 = cur[i];
-            const int ne = J.excl_cnt[i];
-            int pe = 0;
-            int cnt = 0;
-            const size_t row = (size_t)i * cap;
-            for (int base = 0; base < n; base += 64) {
-                const int j = base + lane;
-                bool ok = j < n && j != i;
-                if (ok) {
-                    const float4 r2 = cur[j];
-                    const float x = r2.x - r1.x, y = r2.y - r1.y, z = r2.z - r1.z;
-                    ok = sqrtf(x * x + y * y + z * z) < thr;
-                }
-                unsigned long long ex = 0ull;
-                while (pe < ne) {
-                    const int e = J.excl_j[(size_t)pe * n + i];
-                    if (e >= base + 64) break;
-                    ex |= 1ull << (e - base);
-                    ++pe;
-                }
-                ok = ok && !((ex >> lane) & 1ull);
+        const int i0 = td.y, nrow = min(64, n - i0);
+        const bool own = lane < nrow;  // lane owns bead i0 + lane
+        const float4 ci = own ? cur[i0 + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const int ne = own ? J.excl_cnt[i0 + lane] : 0;
+        int pe = 0;
+        int e_next = pe < ne ? J.excl_j[(size_t)pe * n + i0 + lane] : INT_MAX;
+        int cnt = 0;  // lane i: entries of bead i0 + i so far
+        for (int base = 0; base < n; base += 64) {
+            const int j = base + lane;
+            const float4 cj = j < n ? cur[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+            // this lane's bead: exclusions falling in [base, base + 64)
+            unsigned long long ex = 0ull;
+            while (e_next < base + 64) {
+                ex |= 1ull << (e_next - base);
+                ++pe;
+                e_next = pe < ne ? J.excl_j[(size_t)pe * n + i0 + lane] : INT_MAX;
+            }
+            const int exlo = (int)(unsigned)ex, exhi = (int)(unsigned)(ex >> 32);
+            for (int b = 0; b < nrow; ++b) {
+                // bead b's coordinates, exclusion mask and count: v_readlane with
+                // a uniform lane index
+                const float xi = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ci.x), b));
+                const float yi = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ci.y), b));
+                const float zi = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ci.z), b));
+                const unsigned long long exb =
+                    ((unsigned long long)(unsigned)__builtin_amdgcn_readlane(exhi, b) << 32) |
+                    (unsigned)__builtin_amdgcn_readlane(exlo, b);
+                const float x = cj.x - xi, y = cj.y - yi, z = cj.z - zi;
+                const bool ok = j < n && j != i0 + b && !((exb >> lane) & 1ull) && x * x + y * y + z * z <= d2max;
                 const unsigned long long m = __ballot(ok);
+                const int c0 = __builtin_amdgcn_readlane(cnt, b);
                 if (ok) {
-                    const int pos = cnt + __popcll(m & lanes_below(lane));
-                    if (pos < cap) list_put(J.pp_j, J.idx16, row + pos, j);
+                    const int pos = c0 + __popcll(m & lanes_below(lane));
+                    if (pos < cap) list_put(J.pp_j, J.idx16, (size_t)(i0 + b) * cap + pos, j);
                 }
-                cnt += __popcll(m);
+                if (lane == b) cnt = c0 + __popcll(m);
             }
-            if (lane == 0) {
-                J.pp_cnt[i] = cnt < cap ? cnt : cap;
-                if (cnt > cap) atomicOr(&st[td.x].overflow, 1);
-            }
+        }
+        if (own) {
+            J.pp_cnt[i0 + lane] = cnt < cap ? cnt : cap;
+            if (cnt > cap) atomicOr(&st[td.x].overflow, 1);
         }
     }
 }
@@ -548,7 +559,7 @@ __global__ __launch_bounds__(64) void k_possible_cells(const DevJob* __restrict_
         for (int w = lane; w < nwords; w += 64) bm[w] = 0ull;
         __syncthreads();
         const float4* __restrict__ cur = J.coord[s & 1];
-        const float thr = J.pp_thr;
+        const float d2max = J.pp_d2;
         const int cap = J.pp_cap;
         const int dx = S.cell_dim[0], dy = S.cell_dim[1], dz = S.cell_dim[2];
         const int iend = min(td.y + 64, n);
@@ -569,7 +580,7 @@ __global__ __launch_bounds__(64) void k_possible_cells(const DevJob* __restrict_
                         if (j == i) continue;
                         const float4 r2 = cur[j];
                         const float x = r2.x - r1.x, y2 = r2.y - r1.y, z2 = r2.z - r1.z;
-                        if (sqrtf(x * x + y2 * y2 + z2 * z2) < thr) {
+                        if (x * x + y2 * y2 + z2 * z2 <= d2max) {
                             atomicOr(&bm[j >> 6], 1ull << (j & 63));
                             jlo = min(jlo, j);
                             jhi = max(jhi, j);
@@ -645,7 +656,7 @@ __device__ __forceinline__ void verlet_tile(const DevJob& J, DevJobState& S, con
     const int n = J.n;
     const int nrow = min(64, n - i0);
     const int b_lo = 16 * w, b_hi = min(b_lo + 16, nrow);
-    const float cut = J.pl_cut;
+    const float d2max = J.pl_d2;  // sqrtf(d2) < pl_cut, exactly
     const int cap = J.pair_cap, ppcap = J.pp_cap;
     const IdxT* __restrict__ pp = static_cast<const IdxT*>(J.pp_j);
     IdxT* __restrict__ pj = static_cast<IdxT*>(J.pair_j);
@@ -671,7 +682,7 @@ __device__ __forceinline__ void verlet_tile(const DevJob& J, DevJobState& S, con
                 bool ok = false;
                 if (j[u] >= 0) {
                     const float x = c[u].x - r1.x, y = c[u].y - r1.y, z = c[u].z - r1.z;
-                    ok = sqrtf(x * x + y * y + z * z) < cut;
+                    ok = x * x + y * y + z * z <= d2max;
                 }
                 const unsigned long long m = __ballot(ok);
                 if (ok) {
@@ -742,20 +753,30 @@ __global__ __launch_bounds__(256) void k_verlet(const DevJob* __restrict__ jobs,
 // ---------------------------------------------------------------------------
 // per-job updaters at stride boundaries
 // ---------------------------------------------------------------------------
+// One wavefront per job.  The reaction on the tip is summed in bead order,
+// as the host loop of the reference sums it: the wavefront loads 64 beads'
+// forces at a time (coalesced, and clears them), and lane 0 adds them one
+// by one from the other lanes' registers.
 __device__ void tip_update(const DevJob& J, DevJobState& S, long long s) {
-    if (s == J.retraction) S.V = -S.V;
+    const int lane = threadIdx.x & 63;
+    if (s == J.retraction && lane == 0) S.V = -S.V;
     if (s % J.nav != 0) return;
     float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
-    for (int i = 0; i < J.n; ++i) {
-        float4 v = J.tipF[i];
-        f0 += v.x;
-        f1 += v.y;
-        f2 += v.z;
-        v.x = 0.0f;
-        v.y = 0.0f;
-        v.z = 0.0f;
-        J.tipF[i] = v;
+    for (int base = 0; base < J.n; base += 64) {
+        const int i = base + lane;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (i < J.n) {
+            v = J.tipF[i];
+            J.tipF[i] = make_float4(0.0f, 0.0f, 0.0f, v.w);
+        }
+        const int m = min(64, J.n - base);
+        for (int b = 0; b < m; ++b) {
+            f0 += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.x), b));
+            f1 += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.y), b));
+            f2 += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.z), b));
+        }
     }
+    if (lane != 0) return;
     const float fn = (float)J.nav;
     f0 /= fn;
     f1 /= fn;
@@ -794,15 +815,18 @@ __device__ void pull_update(const DevJob& J, DevJobState& S, long long s) {
     for (int k = 0; k < J.npulled; ++k) J.ext[J.pulled[k]] = make_float4(S.extF[0], S.extF[1], S.extF[2], 2.0f);
 }
 
-// mask: 1 indentation tip, 2 pulling cantilever, 4 heating schedule
-__global__ void k_job_update(const DevJob* __restrict__ jobs, DevJobState* __restrict__ st, int njobs, int mask) {
-    const int jb = blockIdx.x * blockDim.x + threadIdx.x;
+// mask: 1 indentation tip, 2 pulling cantilever, 4 heating schedule.
+// One 64-thread workgroup per job; lane 0 does the scalar work.
+__global__ __launch_bounds__(64) void k_job_update(const DevJob* __restrict__ jobs, DevJobState* __restrict__ st,
+                                                   int njobs, int mask) {
+    const int jb = blockIdx.x;
     if (jb >= njobs) return;
     const DevJob& J = jobs[jb];
     DevJobState& S = st[jb];
     const long long s = S.step;
     if (s >= J.end_step || s % J.stride != 0) return;
     if ((mask & 1) && (J.flags & JF_INDENTATION)) tip_update(J, S, s);
+    if (threadIdx.x != 0) return;
     if ((mask & 2) && (J.flags & JF_PULL_UPDATER)) pull_update(J, S, s);
     if ((mask & 4) && (J.flags & JF_HEATING) && s % J.tempFreq == 0) {
         const float temp = J.initialT + (float)(s / J.tempFreq) * J.deltaT;
@@ -1027,7 +1051,7 @@ hipError_t launch_verlet(const DevJob* jobs, DevJobState* st, const int2* tiles,
 
 hipError_t launch_job_update(const DevJob* jobs, DevJobState* st, int njobs, hipStream_t stream, int mask) {
     if (njobs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_job_update, dim3((njobs + 63) / 64), dim3(64), 0, stream, jobs, st, njobs, mask);
+    hipLaunchKernelGGL(k_job_update, dim3(njobs), dim3(64), 0, stream, jobs, st, njobs, mask);
     return hipGetLastError();
 }
 

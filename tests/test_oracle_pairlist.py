@@ -1,6 +1,7 @@
 """Pins the oracle's possible-pairs and Verlet builds against the set
 definition written independently in numpy (float32 arithmetic, no FMA)."""
 import numpy as np
+import pytest
 
 import oracle as O
 from helpers import numpy_chain
@@ -66,3 +67,32 @@ def test_overflow_is_flagged():
     s = O.Sim(O.make_params(pairs_threshold=30.0, max_possiblePairs=5), system)
     s.build_possible()
     assert s.overflow() == 1
+
+
+def _sqrt_below_bound(t):
+    """Largest float32 x with sqrtf(x) < t, found as the engine finds it."""
+    t = np.float32(t)
+    x = np.float32(t * t)
+    while x > 0 and not np.sqrt(x) < t:
+        x = np.nextafter(x, np.float32(0))
+    while True:
+        y = np.nextafter(x, np.float32(np.inf))
+        if not np.sqrt(y) < t:
+            return x
+        x = y
+
+
+@pytest.mark.parametrize("t", [20.0, 32.0, 12.0, 7.3, 200.0, 0.5, 3.8, 1e-3])
+def test_list_predicate_as_squared_bound(t):
+    # The kernels test sqrtf(d2) < cut as d2 <= bound: sqrtf is correctly
+    # rounded and monotone, so the two agree on every float d2.  Checked on
+    # every float within 4000 ulps of the bound and on a random sample.
+    b = _sqrt_below_bound(t)
+    t32 = np.float32(t)
+    bi = np.array([b], np.float32).view(np.int32)[0]
+    window = np.arange(bi - 4000, bi + 4000, dtype=np.int32).view(np.float32)
+    window = window[window >= 0]
+    np.testing.assert_array_equal(np.sqrt(window) < t32, window <= b)
+    rng = np.random.default_rng(1)
+    d2 = (rng.random(200000, dtype=np.float32) * np.float32(4.0 * t * t)).astype(np.float32)
+    np.testing.assert_array_equal(np.sqrt(d2) < t32, d2 <= b)
