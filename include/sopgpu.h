@@ -141,6 +141,11 @@ int sop_engine_set_tuning(sop_engine* e, int max_blocks, int use_graph);
  * cells around each bead.  enabled = 0 makes every job use the all-pairs
  * scan again (the cell grids stay allocated). */
 int sop_engine_set_cells(sop_engine* e, int min_beads, int enabled);
+/* Step kernel variant (the reference's potentials + integrator,
+ * src/gsop.cu:136-142): jobs of at most 5120 beads are stepped with their
+ * coordinates staged in LDS (default, enabled = 1) or, enabled = 0, with
+ * every neighbour gather from global memory.  Results are identical. */
+int sop_engine_set_lds_step(sop_engine* e, int enabled);
 /* Pick max_blocks by timing candidate grids on the admitted jobs. */
 int sop_engine_autotune(sop_engine* e, int* max_blocks_out);
 

@@ -84,6 +84,12 @@ int sop_engine_set_tuning(sop_engine* e, int max_blocks, int use_graph) {
     return 0;
 }
 
+int sop_engine_set_lds_step(sop_engine* e, int enabled) {
+    ENG(e);
+    E.set_lds_step(enabled);
+    return 0;
+}
+
 int sop_engine_set_cells(sop_engine* e, int min_beads, int enabled) {
     ENG(e);
     E.set_cells(min_beads, enabled);
@@ -261,7 +267,7 @@ int sop_chol_factor(int device, int n, const float* A, int mode, float* L, int r
     if (err == hipSuccess) err = hipEventCreateWithFlags(&et, hipEventDisableTiming);
     if (err == hipSuccess) err = hipMalloc(&dA, mp * mp * 4);
     if (err == hipSuccess) err = hipMalloc(&dM, mp * mp * 4);
-    if (err == hipSuccess) err = hipMalloc(&linv, 64 * 64 * 8);
+    if (err == hipSuccess) err = hipMalloc(&linv, sop::kCholLinvBytes);
     if (err == hipSuccess) err = hipMalloc(&phi, 2 * mp * sop::kCholPanel * 2);
     if (err == hipSuccess) err = hipMalloc(&plo, 2 * mp * sop::kCholPanel * 2);
     if (err == hipSuccess) err = hipEventCreate(&e0);

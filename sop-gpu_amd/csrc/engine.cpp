@@ -134,6 +134,8 @@ Engine::~Engine() {
     (void)hipFree(d_state_);
     (void)hipFree(d_tiles_);
     (void)hipFree(d_tiles1_);
+    (void)hipFree(d_units_);
+    (void)hipFree(d_units1_);
     (void)hipFree(d_scratch4_);
     if (aux_stream_) {
         (void)hipStreamSynchronize(aux_stream_);
@@ -460,7 +462,7 @@ static unsigned long long built_bytes(const BuiltJob& b, bool pulling, bool inde
         t += 3 * round_up(n * 16) + round_up(n * 4);  // rforce, mforce, ci, eps
         if (b.dj.flags & JF_HI_EXACT) {
             const unsigned long long mp = (3 * n + 63) / 64 * 64;
-            t += round_up(n * 16) + round_up(mp * mp * 4) + round_up(64 * 64 * 8) + 2 * round_up(2 * mp * kCholPanel * 2);
+            t += round_up(n * 16) + round_up(mp * mp * 4) + round_up(kCholLinvBytes) + 2 * round_up(2 * mp * kCholPanel * 2);
         }
     }
     return t;
@@ -514,6 +516,7 @@ int Engine::add_job(const sop_params& p, const sop_system& s, long long numsteps
     J.pulled = (int*)A(std::max<size_t>(1, b.pulled.size()) * 4);
     J.npulled = (int)b.pulled.size();
     J.energies = (float*)A(n * 12);
+    J.lds_step = use_lds_step_ && n <= (size_t)kLdsStepBeads ? 1 : 0;
     if (uses_cells(n)) {
         J.cell_cap = (int)n + 64;
         J.cell_cnt = (int*)A((size_t)J.cell_cap * 4);
@@ -537,7 +540,7 @@ int Engine::add_job(const sop_params& p, const sop_system& s, long long numsteps
             hw.ld = mp;
             hw.mp = (int)mp;
             hw.mode = exact_mode_;
-            hw.linv = (double*)A(64 * 64 * 8);
+            hw.linv = (double*)A(kCholLinvBytes);
             hw.phi = (unsigned short*)A(2 * mp * kCholPanel * 2);
             hw.plo = (unsigned short*)A(2 * mp * kCholPanel * 2);
             hw.chol_freq = J.hi_chol_freq;
@@ -708,6 +711,33 @@ int Engine::sync_tables() {
                            "tile upload"))
             return rc;
     }
+    // units of the LDS step kernel: (job slot, first bead) per kLdsUnit beads
+    std::vector<int2> units;
+    lds_beads_ = 0;
+    any_global_ = 0;
+    for (int k = 0; k < nj; ++k) {
+        const DevJob& J = jobs_[k].dj;
+        if (!J.lds_step) {
+            any_global_ = 1;
+            continue;
+        }
+        lds_beads_ = std::max(lds_beads_, J.n);
+        for (int b = 0; b < J.n; b += kLdsUnit) units.push_back(make_int2(k, b));
+    }
+    nunits_ = (int)units.size();
+    if (nunits_ > cap_units_) {
+        (void)hipFree(d_units_);
+        d_units_ = nullptr;
+        cap_units_ = 0;
+        const int cap = std::max(64, nunits_ * 2);
+        if (int rc = check(hipMalloc(&d_units_, cap * sizeof(int2)), "unit table")) return rc;
+        cap_units_ = cap;
+    }
+    if (nunits_) {
+        if (int rc = check(hipMemcpyAsync(d_units_, units.data(), nunits_ * sizeof(int2), hipMemcpyHostToDevice, stream_),
+                           "unit upload"))
+            return rc;
+    }
     if (int rc = check(hipStreamSynchronize(stream_), "table sync")) return rc;
     tables_dirty_ = false;
     synced_jobs_ = nj;
@@ -734,6 +764,13 @@ void Engine::set_cells(int min_beads, int enabled) {
     drop_graph();
 }
 
+void Engine::set_lds_step(int enabled) {
+    use_lds_step_ = enabled != 0;
+    for (auto& j : jobs_) j.dj.lds_step = use_lds_step_ && j.n <= kLdsStepBeads ? 1 : 0;
+    tables_dirty_ = true;
+    drop_graph();
+}
+
 void Engine::set_tuning(int max_blocks, int use_graph) {
     max_blocks_ = max_blocks < 0 ? 0 : max_blocks;
     use_graph_ = use_graph != 0;
@@ -753,7 +790,7 @@ void Engine::flush_profile() {
         float ms = 0.0f;
         if (hipEventElapsedTime(&ms, ev_pool_[k].first, ev_pool_[k].second) == hipSuccess) {
             prof_ms_ += ms;
-            prof_launches_++;
+            prof_launches_ += ev_launches_[k];
         }
     }
     ev_used_ = 0;
@@ -792,8 +829,12 @@ int Engine::launch_block(int slot_only) {
     const bool exact = slot_only >= 0 ? (jobs_[slot_only].dj.flags & JF_HI_EXACT) != 0 : any_exact_;
     if (tea)
         if (int rc = check(launch_hi_tea_update(jobs, st, tiles, nt, nj, stream_), "TEA update")) return rc;
+    // profiling: HIP events around the step launches; when nothing else is
+    // launched between them, one pair spans the block's L launches
+    const bool span = !tea && !exact;
     for (int k = 0; k < L; ++k) {
-        const bool prof = profiling_;
+        const bool prof = profiling_ && (!span || k == 0);
+        const bool prof_end = profiling_ && (!span || k == L - 1);
         if (prof) {
             if (ev_used_ == ev_pool_.size()) {
                 if (ev_pool_.size() >= 4096) flush_profile();
@@ -802,13 +843,23 @@ int Engine::launch_block(int slot_only) {
                     if (int rc = check(hipEventCreate(&a), "event")) return rc;
                     if (int rc = check(hipEventCreate(&b), "event")) return rc;
                     ev_pool_.push_back({a, b});
+                    ev_launches_.push_back(1);
                 }
             }
             (void)hipEventRecord(ev_pool_[ev_used_].first, stream_);
         }
-        if (int rc = check(launch_step(jobs, st, tiles, nt, k, max_blocks_, stream_), "step")) return rc;
-        if (prof) {
+        if (int rc = check(slot_only >= 0
+                               ? launch_step(jobs, st, tiles, nt, !jobs_[slot_only].dj.lds_step, d_units1_,
+                                             jobs_[slot_only].dj.lds_step ? (jobs_[slot_only].n + kLdsUnit - 1) / kLdsUnit
+                                                                          : 0,
+                                             jobs_[slot_only].n, k, max_blocks_, stream_)
+                               : launch_step(jobs, st, tiles, nt, any_global_, d_units_, nunits_, lds_beads_, k,
+                                             max_blocks_, stream_),
+                           "step"))
+            return rc;
+        if (prof_end) {
             (void)hipEventRecord(ev_pool_[ev_used_].second, stream_);
+            ev_launches_[ev_used_] = span ? L : 1;
             ev_used_++;
         }
         if (tea)
@@ -852,6 +903,18 @@ int Engine::tiles_for_slot(int slot, int* nt) {
     if (int rc = synchronize()) return rc;
     if (int rc = check(hipMemcpy(d_tiles1_, t1.data(), need * sizeof(int2), hipMemcpyHostToDevice), "tile upload"))
         return rc;
+    const int nu = (n + kLdsUnit - 1) / kLdsUnit;
+    if (nu > cap_units1_) {
+        (void)hipFree(d_units1_);
+        d_units1_ = nullptr;
+        cap_units1_ = 0;
+        if (int rc = check(hipMalloc(&d_units1_, nu * sizeof(int2)), "unit table")) return rc;
+        cap_units1_ = nu;
+    }
+    std::vector<int2> u1(nu);
+    for (int u = 0; u < nu; ++u) u1[u] = make_int2(0, kLdsUnit * u);
+    if (int rc = check(hipMemcpy(d_units1_, u1.data(), nu * sizeof(int2), hipMemcpyHostToDevice), "unit upload"))
+        return rc;
     tiles1_n_ = n;
     return 0;
 }
@@ -876,7 +939,10 @@ int Engine::stage_tea_update() {
 }
 int Engine::stage_step(int k) {
     if (int rc = sync_tables()) return rc;
-    if (int rc = check(launch_step(d_jobs_, d_state_, d_tiles_, ntiles_, k, max_blocks_, stream_), "step")) return rc;
+    if (int rc = check(launch_step(d_jobs_, d_state_, d_tiles_, ntiles_, any_global_, d_units_, nunits_, lds_beads_, k,
+                                   max_blocks_, stream_),
+                       "step"))
+        return rc;
     if (any_tea_)
         if (int rc = check(launch_hi_tea_step(d_jobs_, d_state_, d_tiles_, ntiles_, k, stream_), "TEA step")) return rc;
     if (any_exact_)

@@ -108,6 +108,9 @@ class Engine {
     // from now on get one (0 keeps the threshold, < 0 gives none); enabled = 0
     // builds every list by the all-pairs scan.
     void set_cells(int min_beads, int enabled);
+    // Step jobs of at most kLdsStepBeads beads with their coordinates in
+    // LDS (default) or, enabled = 0, all from global memory.
+    void set_lds_step(int enabled);
     void set_exact_mode(int mode) { exact_mode_ = mode; }
     int get_hi(int id, double* eps, float* beta);
     int get_hi_factor(int id, float* L);
@@ -152,7 +155,12 @@ class Engine {
     int cell_min_ = kCellMinBeads;
     bool use_cells_ = true;
     int cell_words_ = 0;
-    int any_idx32_ = 0;  // some job stores int32 list entries  // bitmap words of the largest job with a cell grid (0: none or disabled)
+    int any_idx32_ = 0;  // some job stores int32 list entries
+    bool use_lds_step_ = true;
+    int2* d_units_ = nullptr;  // LDS step units (job slot, first bead)
+    int cap_units_ = 0, nunits_ = 0, lds_beads_ = 0, any_global_ = 1;
+    int2* d_units1_ = nullptr;  // units of the single-job path
+    int cap_units1_ = 0;  // bitmap words of the largest job with a cell grid (0: none or disabled)
     bool uses_cells(size_t n) const;
     int cell_words_of(const DevJob& J) const;
     int exact_mode_ = SOP_CHOL_BF16X3;
@@ -162,6 +170,7 @@ class Engine {
     // profiling
     bool profiling_ = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool_;
+    std::vector<int> ev_launches_;  // step launches each event pair spans
     size_t ev_used_ = 0;
     long long prof_launches_ = 0;
     double prof_ms_ = 0.0;

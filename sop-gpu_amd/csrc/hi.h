@@ -13,6 +13,8 @@ enum { SOP_CHOL_BF16X3 = 0, SOP_CHOL_F32 = 1 };
 // alternate, so a panel's trailing update may still read its copy while
 // the next panel is factored).
 constexpr int kCholPanel = 256;
+// FP64 inverses of an outer panel's 64x64 diagonal blocks.
+constexpr size_t kCholLinvBytes = (size_t)(kCholPanel / 64) * 64 * 64 * sizeof(double);
 
 // Device workspace of one exact-HI job's blocked factorisation.
 struct HiExactWork {
@@ -20,7 +22,7 @@ struct HiExactWork {
     size_t ld;
     int mp;                // padded order (multiple of 64)
     int mode;              // SOP_CHOL_*
-    double* linv;          // 64x64 FP64 factor of the current diagonal block
+    double* linv;          // kCholLinvBytes: inverses of the current panel's diagonal blocks
     unsigned short* phi;   // split-BF16 outer panels, two buffers of mp x kCholPanel
     unsigned short* plo;
     int chol_freq;         // refactorisation interval in steps
@@ -40,7 +42,7 @@ hipError_t launch_chol_step(const DevJob* jobs, const DevJobState* st, const int
 // Standalone factorisation of M (mp x mp, row-major, lower triangle used).
 // aux, ev_f, ev_t2 (optional): a second stream and two events for the
 // lookahead schedule; the factor is complete in `stream` order on return.
-hipError_t launch_chol_factor(float* M, size_t ld, int mp, int mode, double* ldiag, unsigned short* phi,
+hipError_t launch_chol_factor(float* M, size_t ld, int mp, int mode, double* linv, unsigned short* phi,
                               unsigned short* plo, const int* flag, hipStream_t stream, hipStream_t aux = nullptr,
                               hipEvent_t ev_f = nullptr, hipEvent_t ev_t2 = nullptr);
 

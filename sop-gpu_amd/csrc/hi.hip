@@ -13,12 +13,19 @@
 // from the step-start buffer, so no kernel reads what it writes.
 //
 // The exact path factorises the RPY matrix (3N x 3N, padded to a multiple
-// of 64 with an identity block) by a right-looking blocked Cholesky:
-//   diagonal 64x64 blocks in FP64 in LDS, with their inverse;
-//   the panel below as A L^-T with FP64 accumulation, also emitted as
-//   split-BF16 pairs (hi = bf16(x), lo = bf16(x - hi));
+// of 64 with an identity block) by a blocked Cholesky over outer panels of
+// kCholPanel columns:
+//   the panel's diagonal region, left-looking over its 64x64 blocks in one
+//   workgroup: each diagonal block factored in FP64 in LDS, with its
+//   inverse (FP64), and the blocks below it in the region solved against
+//   it in FP64;
+//   the panel's rows below the region, 64 per workgroup, solved the same
+//   way with the inverses (X = (A - X' L'^T) Li^T, FP64 accumulation),
+//   also emitted as split-BF16 pairs (hi = bf16(x), lo = bf16(x - hi));
 //   the trailing update C -= P_I P_J^T on MFMA, either split-BF16 with three
-//   products (hi*hi + hi*lo + lo*hi, FP32 accumulate) or FP32 MFMA.
+//   products (hi*hi + hi*lo + lo*hi, FP32 accumulate) or FP32 MFMA, the
+//   next panel's columns first and the rest on a second stream, overlapped
+//   with the next panel's factorisation.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -378,40 +385,48 @@ __device__ __forceinline__ double readlane_d(double v, int lane) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// 1/sqrt(p) in FP64: the hardware estimate refined by two Newton steps
+// (y <- y (3 - p y^2) / 2), which leaves it within a few ulp of FP64.
+__device__ __forceinline__ double rsqrt_d(double p) {
+    double y = __builtin_amdgcn_rsq(p);
+    y = y * fma(-0.5 * p * y, y, 1.5);
+    y = y * fma(-0.5 * p * y, y, 1.5);
+    return y;
+}
+
 // 16-column steps of the diagonal factorisation, run by one wavefront on
 // registers: lanes 0..15 hold the rows of the 16x16 diagonal sub-block,
 // lanes 16..63 the rows below it.  Step J (a template parameter, so every
 // register index is a constant) scales column J by 1/L[J][J] and subtracts
-// L[r][J] L[c][J] from the columns c > J, with L[c][J] read from lane c:
-// for lanes below the sub-block this is the forward substitution of the
-// panel solve, so both run as one pass.  Lanes 0..15 accumulate scratch
-// above their diagonal, which is never written back.
+// L[r][J] L[c][J] from the columns c > J, with L[c][J] broadcast from the
+// wavefront's LDS row `col`: for lanes below the sub-block this is the
+// forward substitution of the panel solve, so both run as one pass.  Lanes
+// 0..15 accumulate scratch above their diagonal, which is never written
+// back.  rinv[J] receives 1/L[J][J].
 template <int J>
-__device__ __forceinline__ void chol16_step(double (&v)[16], int r) {
-    const double d = sqrt(readlane_d(v[J], J));
-    const double rd = 1.0 / d;
-    v[J] = r == J ? d : v[J] * rd;
+__device__ __forceinline__ void chol16_step(double (&v)[16], int r, double* __restrict__ col,
+                                            double* __restrict__ rinv) {
+    const double y = rsqrt_d(readlane_d(v[J], J));
+    v[J] *= y;  // lane J: L[J][J] = p / sqrt(p); below: a[r][J] / L[J][J]
+    if (r == J) rinv[J] = y;
+    if constexpr (J + 1 < 16) {
+        col[r] = v[J];
 #pragma unroll
-    for (int c = J + 1; c < 16; ++c) v[c] = fma(-v[J], readlane_d(v[J], c), v[c]);
-    if constexpr (J + 1 < 16) chol16_step<J + 1>(v, r);
+        for (int c = J + 1; c < 16; ++c) v[c] = fma(-v[J], col[c], v[c]);
+        chol16_step<J + 1>(v, r, col, rinv);
+    }
 }
 
-// Factor the k-th diagonal block in FP64: four 16-column sub-blocks, each
-// factored together with the rows below it by wavefront 0 in registers
-// (no barrier inside), then a rank-16 update of the rest of the block by
-// all 256 threads.  Writes L to M (FP32) and to ldiag (FP64, zero above
-// the diagonal) for the panel solve.
-__global__ __launch_bounds__(256) void k_potrf_diag(float* __restrict__ M, size_t ld, int k, double* __restrict__ ldiag,
-                                                    const int* __restrict__ flag) {
-    if (flag && !*flag) return;
-    __shared__ double s[NB][NB + 1];
+static constexpr int LDS_ROW = NB + 1;  // row stride of the FP64 block being factored
+static constexpr int LDT = NB + 2;      // row stride of the transposed FP64 blocks (16-byte rows)
+
+// Factor the 64x64 FP64 block s (lower triangle, row-major) in place with
+// 256 threads: four 16-column sub-blocks, each factored together with the
+// rows below it by wavefront 0 in registers (no barrier inside), then a
+// rank-16 update of the rest of the block by all threads.  dinv receives
+// the reciprocals of the diagonal; col is a 64-double LDS scratch row.
+__device__ __forceinline__ void potrf64(double (*s)[LDS_ROW], double* __restrict__ dinv, double* __restrict__ col) {
     const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-    const size_t o = (size_t)k * NB;
-    for (int e = tid; e < NB * NB; e += 256) {
-        const int r = e / NB, c = e % NB;
-        s[r][c] = c <= r ? (double)M[(o + r) * ld + o + c] : 0.0;
-    }
-    __syncthreads();
 #pragma unroll 1
     for (int sb = 0; sb < NB / 16; ++sb) {
         const int q0 = 16 * sb;
@@ -421,7 +436,7 @@ __global__ __launch_bounds__(256) void k_potrf_diag(float* __restrict__ M, size_
             double v[16];
 #pragma unroll
             for (int c = 0; c < 16; ++c) v[c] = live && (r >= 16 || c <= r) ? s[row][q0 + c] : 0.0;
-            chol16_step<0>(v, r);
+            chol16_step<0>(v, r, col, dinv + q0);
             if (live) {
 #pragma unroll
                 for (int c = 0; c < 16; ++c)
@@ -429,194 +444,285 @@ __global__ __launch_bounds__(256) void k_potrf_diag(float* __restrict__ M, size_
             }
         }
         __syncthreads();
-        // rank-16 update of rows/columns q0+16 .. 63
-        const int b0 = q0 + 16;
-        double acc[3][3];
+        const int b0 = q0 + 16;  // rank-16 update of rows/columns b0 .. 63
+        if (b0 < NB) {
+            double acc[3][3];
 #pragma unroll
-        for (int a = 0; a < 3; ++a)
+            for (int a = 0; a < 3; ++a)
 #pragma unroll
-            for (int b = 0; b < 3; ++b) acc[a][b] = 0.0;
+                for (int b = 0; b < 3; ++b) acc[a][b] = 0.0;
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            double lr[3], lc[3];
+            for (int q = 0; q < 16; ++q) {
+                double lr[3], lc[3];
 #pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                const int r = b0 + ty + 16 * a, c = b0 + tx + 16 * a;
-                lr[a] = r < NB ? s[r][q0 + q] : 0.0;
-                lc[a] = c < NB ? s[c][q0 + q] : 0.0;
+                for (int a = 0; a < 3; ++a) {
+                    const int r = b0 + ty + 16 * a, c = b0 + tx + 16 * a;
+                    lr[a] = r < NB ? s[r][q0 + q] : 0.0;
+                    lc[a] = c < NB ? s[c][q0 + q] : 0.0;
+                }
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+#pragma unroll
+                    for (int b = 0; b < 3; ++b) acc[a][b] = fma(lr[a], lc[b], acc[a][b]);
             }
 #pragma unroll
             for (int a = 0; a < 3; ++a)
 #pragma unroll
-                for (int b = 0; b < 3; ++b) acc[a][b] = fma(lr[a], lc[b], acc[a][b]);
+                for (int b = 0; b < 3; ++b) {
+                    const int r = b0 + ty + 16 * a, c = b0 + tx + 16 * b;
+                    if (r < NB && c <= r) s[r][c] -= acc[a][b];
+                }
+            __syncthreads();
         }
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-#pragma unroll
-            for (int b = 0; b < 3; ++b) {
-                const int r = b0 + ty + 16 * a, c = b0 + tx + 16 * b;
-                if (r < NB && c <= r) s[r][c] -= acc[a][b];
-            }
-        __syncthreads();
-    }
-    for (int e = tid; e < NB * NB; e += 256) {
-        const int r = e / NB, c = e % NB;
-        if (c <= r) M[(o + r) * ld + o + c] = (float)s[r][c];
-        ldiag[e] = c <= r ? s[r][c] : 0.0;
     }
 }
 
-// Steps T of the panel solve (a template parameter, so every register index
-// below is a constant): the owner of column T solves x_T, the four threads
-// of the row exchange it, and each updates its columns c = 4u + p > T.
-template <int T>
-__device__ __forceinline__ void trsm_step(double (&a)[16], const double (*lp)[4][16], const double* inv, int p,
-                                          int g0) {
-    constexpr int owner = T & 3, ut = T >> 2;
-    const double xo = p == owner ? a[ut] * inv[T] : 0.0;
-    const double xt = __shfl(xo, g0 | owner);
-    if (p == owner) a[ut] = xt;
-    const double2* __restrict__ cf = reinterpret_cast<const double2*>(&lp[T][p][0]);
+// Off-diagonal 16x16 blocks of the inverse at distance D = I - J:
+// Li_IJ = -Li_II (sum_{J<=K<I} L_IK Li_KJ), thread (ty, tx) one element of
+// each block, sums split over two accumulators.
+template <int D>
+__device__ __forceinline__ void inverse_level(const double (*s)[LDS_ROW], double* __restrict__ LiT,
+                                              double (*T)[16][17], int ty, int tx) {
 #pragma unroll
-    for (int h = ut / 2; h < 8; ++h) {
-        const double2 q = cf[h];
-        const int u0 = 2 * h, u1 = 2 * h + 1;
-        if (4 * u0 + p > T) a[u0] = fma(-xt, q.x, a[u0]);
-        if (4 * u1 + p > T) a[u1] = fma(-xt, q.y, a[u1]);
+    for (int J = 0; J + D < 4; ++J) {
+        const int I = J + D;
+        double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+        for (int t = 16 * J; t < 16 * I; t += 2) {
+            a0 = fma(s[16 * I + ty][t], LiT[(16 * J + tx) * LDT + t], a0);
+            a1 = fma(s[16 * I + ty][t + 1], LiT[(16 * J + tx) * LDT + t + 1], a1);
+        }
+        T[J][ty][tx] = a0 + a1;
     }
-    if constexpr (T + 1 < NB) trsm_step<T + 1>(a, lp, inv, p, g0);
-}
-
-// Panel below block k: X L^T = A for the rows below the diagonal block, in
-// FP64 by forward substitution.  Four threads share a row, thread p owning
-// the columns c = 4u + p; at step t the owner of column t solves x_t and
-// hands it to the other three, which update their columns c > t.  The
-// factor sits in LDS permuted so that a thread's sixteen coefficients of a
-// step are contiguous.  Writes X to M (FP32) and its split-BF16 copy into
-// the outer panel buffers (row stride NB2, block k at column poff).
-__global__ __launch_bounds__(256) void k_trsm(float* __restrict__ M, size_t ld, int k, int rows_end,
-                                              const double* __restrict__ ldiag, unsigned short* __restrict__ phi,
-                                              unsigned short* __restrict__ plo, int poff,
-                                              const int* __restrict__ flag) {
-    if (flag && !*flag) return;
-    __shared__ __attribute__((aligned(16))) double lp[NB][4][16];  // lp[t][p][u] = L[4u+p][t]
-    __shared__ double inv[NB];
-    for (int e = threadIdx.x; e < NB * NB; e += 256) {
-        const int c = e / NB, t = e % NB;  // L[c][t]
-        lp[t][c & 3][c >> 2] = ldiag[e];
-    }
-    if (threadIdx.x < NB) inv[threadIdx.x] = 1.0 / ldiag[threadIdx.x * NB + threadIdx.x];
     __syncthreads();
-    const int lane = threadIdx.x & 63, p = threadIdx.x & 3;
-    const int rl = threadIdx.x >> 2;
-    const size_t o = (size_t)k * NB;
-    const size_t row = o + NB + (size_t)blockIdx.x * 64 + rl;
-    if (row >= (size_t)rows_end) return;  // the four threads of a row leave together
-    float* __restrict__ Ar = M + row * ld + o;
-    double a[16];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) a[u] = (double)Ar[4 * u + p];
-    trsm_step<0>(a, lp, inv, p, lane & ~3);
+    for (int J = 0; J + D < 4; ++J) {
+        const int I = J + D;
+        double a0 = 0.0, a1 = 0.0;
 #pragma unroll
-    for (int u = 0; u < 16; ++u) {
-        const int c = 4 * u + p;
-        const float v = (float)a[u];
-        Ar[c] = v;
-        const unsigned short h = f2bf(v);
-        phi[row * NB2 + poff + c] = h;
-        plo[row * NB2 + poff + c] = f2bf(v - bf2f(h));
+        for (int t = 0; t < 16; t += 2) {
+            a0 = fma(LiT[(16 * I + t) * LDT + 16 * I + ty], T[J][t][tx], a0);
+            a1 = fma(LiT[(16 * I + t + 1) * LDT + 16 * I + ty], T[J][t + 1][tx], a1);
+        }
+        LiT[(16 * J + tx) * LDT + 16 * I + ty] = -(a0 + a1);
     }
+    __syncthreads();
 }
 
-__device__ __forceinline__ void tri_index(int idx, int& I, int& J) {
-    int i = (int)((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
-    while ((i + 1) * (i + 2) / 2 <= idx) ++i;
-    while (i * (i + 1) / 2 > idx) --i;
-    I = i;
-    J = idx - i * (i + 1) / 2;
+// Inverse of the lower-triangular 64x64 factor L (FP64, in s; dinv holds
+// the reciprocals of its diagonal) by 16x16 blocks, 256 threads: the four
+// diagonal blocks by forward substitution (wavefront I, lane c: column c of
+// inv(L_II)), then the blocks at distance d = I - J = 1, 2, 3.  Output
+// transposed: LiT[a*LDT + b] = Li[b][a], zero for a > b.
+__device__ __forceinline__ void tri_inverse64(const double (*s)[LDS_ROW], double* __restrict__ LiT,
+                                              double (*T)[16][17], const double* __restrict__ dinv) {
+    const int tid = threadIdx.x;
+    for (int e = tid; e < NB * LDT; e += 256) LiT[e] = 0.0;
+    __syncthreads();
+    {
+        const int I = tid >> 6, c = tid & 63, q0 = 16 * I;
+        if (c < 16) {
+            double y[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                double acc = 0.0;
+#pragma unroll
+                for (int t = 0; t < r; ++t) acc = fma(s[q0 + r][q0 + t], y[t], acc);
+                y[r] = r < c ? 0.0 : (r == c ? dinv[q0 + r] : -acc * dinv[q0 + r]);
+            }
+#pragma unroll
+            for (int r = 0; r < 16; ++r) LiT[(q0 + c) * LDT + q0 + r] = y[r];
+        }
+    }
+    __syncthreads();
+    const int ty = tid >> 4, tx = tid & 15;
+    inverse_level<1>(s, LiT, T, ty, tx);
+    inverse_level<2>(s, LiT, T, ty, tx);
+    inverse_level<3>(s, LiT, T, ty, tx);
 }
 
-// Update of the outer panel's remaining columns by inner block k: tile
-// (I, J) of 64x64 with J = k+1+blockIdx.y (a column block of the panel)
-// and I = J + blockIdx.x (rows down to the end), C_IJ -= L_Ik L_Jk^T.
-// Split-BF16 MFMA, three products per K-chunk, FP32 accumulation; wave w
-// owns rows 16w..16w+15 of the tile.
+// The 64x64 block of M at rows r0.., columns kr.. (column block k of the
+// panel starting at column c0) minus sum_{j<k} X_j L_j^T over the panel's
+// first 64k columns, where X are rows r0.. and L rows kr.. of the panel.
+// Wave w computes rows 16w..16w+15 on MFMA, from the split-BF16 copy of the
+// panel (three products) or from M in FP32.  The result goes to LDS in
+// FP64, row-major (dst[r*dld + c]) or transposed (dst[c*dld + r]).
 // v_mfma_f32_16x16x32_bf16 lane layout: A[m=l%16][k=8(l/16)+0..7],
-// B[k=8(l/16)+0..7][n=l%16], C[4(l/16)+0..3][l%16].
-__global__ __launch_bounds__(256) void k_syrk_bf16x3(float* __restrict__ M, size_t ld, int k, int ntiles,
-                                                     const unsigned short* __restrict__ phi,
-                                                     const unsigned short* __restrict__ plo, int poff,
-                                                     const int* __restrict__ flag) {
-    if (flag && !*flag) return;
-    const int Jt = k + 1 + (int)blockIdx.y, I = Jt + (int)blockIdx.x;
-    if (I >= ntiles) return;
-    const size_t r0 = (size_t)I * NB, c0 = (size_t)Jt * NB;
+// B[k=8(l/16)+0..7][n=l%16], C[4(l/16)+0..3][l%16]; v_mfma_f32_16x16x4_f32:
+// A[m=l%16][k=l/16], B[k=l/16][n=l%16], C as above.
+template <int MODE>
+__device__ __forceinline__ void block_update(const float* __restrict__ M, size_t ld, size_t r0, size_t kr,
+                                             size_t c0, int k, const unsigned short* __restrict__ phi,
+                                             const unsigned short* __restrict__ plo, double* __restrict__ dst,
+                                             int dld, bool trans) {
     const int l = threadIdx.x & 63, w = wave_of();
-    const int lr = l & 15, lk = (l >> 4) * 8;
+    const int lr = l & 15;
     f32x4 acc[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const size_t arow = r0 + 16 * w + lr;
+    const int K = NB * k;
+    if constexpr (MODE == SOP_CHOL_BF16X3) {
+        const int lk = (l >> 4) * 8;
+        const size_t arow = (r0 + 16 * w + lr) * NB2 + lk;
+#pragma unroll 1
+        for (int jc = 0; jc < K; jc += NB)
 #pragma unroll
-    for (int kc = 0; kc < NB; kc += 32) {
-        const bf16x8 ah = *reinterpret_cast<const bf16x8*>(phi + arow * NB2 + poff + kc + lk);
-        const bf16x8 al = *reinterpret_cast<const bf16x8*>(plo + arow * NB2 + poff + kc + lk);
+        for (int kc = jc; kc < jc + NB; kc += 32) {
+            const bf16x8 ah = *reinterpret_cast<const bf16x8*>(phi + arow + kc);
+            const bf16x8 al = *reinterpret_cast<const bf16x8*>(plo + arow + kc);
+            bf16x8 bh[4], bl[4];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const size_t brow = c0 + 16 * t + lr;
-            const bf16x8 bh = *reinterpret_cast<const bf16x8*>(phi + brow * NB2 + poff + kc + lk);
-            const bf16x8 bl = *reinterpret_cast<const bf16x8*>(plo + brow * NB2 + poff + kc + lk);
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[t], 0, 0, 0);
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[t], 0, 0, 0);
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[t], 0, 0, 0);
+            for (int t = 0; t < 4; ++t) {
+                const size_t brow = (kr + 16 * t + lr) * NB2 + lk + kc;
+                bh[t] = *reinterpret_cast<const bf16x8*>(phi + brow);
+                bl[t] = *reinterpret_cast<const bf16x8*>(plo + brow);
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[t], acc[t], 0, 0, 0);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[t], acc[t], 0, 0, 0);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[t], acc[t], 0, 0, 0);
+        }
+    } else {
+        const int lk = l >> 4;
+        const float* __restrict__ arow = M + (r0 + 16 * w + lr) * ld + c0 + lk;
+#pragma unroll 1
+        for (int jc = 0; jc < K; jc += NB)
+#pragma unroll 4
+        for (int kc = jc; kc < jc + NB; kc += 4) {
+            const float av = arow[kc];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const float bv = M[(kr + 16 * t + lr) * ld + c0 + lk + kc];
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[t], 0, 0, 0);
+            }
         }
     }
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
+    for (int t = 0; t < 4; ++t)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const size_t row = r0 + 16 * w + 4 * (l >> 4) + q;
-            const size_t col = c0 + 16 * t + lr;
-            if (I == Jt && col > row) continue;
-            M[row * ld + col] -= acc[t][q];
+            const int row = 16 * w + 4 * (l >> 4) + q, col = 16 * t + lr;
+            const double v = (double)(M[(r0 + row) * ld + kr + col] - acc[t][q]);
+            dst[trans ? col * dld + row : row * dld + col] = v;
+        }
+}
+
+// X = A Li^T for one 64x64 block in FP64: At[t*LDT + r] = A[r][t] and
+// LiT[t*LDT + c] = Li[c][t] (zero for t > c).  Thread (rb, cb) computes
+// rows 4rb..4rb+3 and columns 4cb..4cb+3.  X goes to M (rows r0..,
+// columns col0..) in FP32 and, in split-BF16 mode, to the panel copy
+// (columns pcol0.. of the panel).
+template <int MODE>
+__device__ __forceinline__ void trsm_block(const double* __restrict__ At, const double* __restrict__ LiT,
+                                           float* __restrict__ M, size_t ld, size_t r0, size_t col0,
+                                           unsigned short* __restrict__ phi, unsigned short* __restrict__ plo,
+                                           int pcol0) {
+    const int tid = threadIdx.x, rb = tid >> 4, cb = tid & 15;
+    double x[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) x[i][j] = 0.0;
+    const int tend = 4 * cb + 4;
+    for (int t = 0; t < tend; ++t) {
+        const double2 a01 = *reinterpret_cast<const double2*>(At + t * LDT + 4 * rb);
+        const double2 a23 = *reinterpret_cast<const double2*>(At + t * LDT + 4 * rb + 2);
+        const double2 b01 = *reinterpret_cast<const double2*>(LiT + t * LDT + 4 * cb);
+        const double2 b23 = *reinterpret_cast<const double2*>(LiT + t * LDT + 4 * cb + 2);
+        const double a[4] = {a01.x, a01.y, a23.x, a23.y}, b[4] = {b01.x, b01.y, b23.x, b23.y};
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) x[i][j] = fma(a[i], b[j], x[i][j]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const size_t row = r0 + 4 * rb + i;
+        const float4 v = make_float4((float)x[i][0], (float)x[i][1], (float)x[i][2], (float)x[i][3]);
+        *reinterpret_cast<float4*>(M + row * ld + col0 + 4 * cb) = v;
+        if constexpr (MODE == SOP_CHOL_BF16X3) {
+            const float vv[4] = {v.x, v.y, v.z, v.w};
+            ushort4 h, lo;
+            unsigned short hh[4], ll[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                hh[j] = f2bf(vv[j]);
+                ll[j] = f2bf(vv[j] - bf2f(hh[j]));
+            }
+            h = make_ushort4(hh[0], hh[1], hh[2], hh[3]);
+            lo = make_ushort4(ll[0], ll[1], ll[2], ll[3]);
+            *reinterpret_cast<ushort4*>(phi + row * NB2 + pcol0 + 4 * cb) = h;
+            *reinterpret_cast<ushort4*>(plo + row * NB2 + pcol0 + 4 * cb) = lo;
         }
     }
 }
 
-// FP32 path of the same update: v_mfma_f32_16x16x4_f32, lane layout
-// A[m=l%16][k=l/16], B[k=l/16][n=l%16], C as above; operands read from the
-// panel in M itself.
-__global__ __launch_bounds__(256) void k_syrk_f32(float* __restrict__ M, size_t ld, int k, int ntiles,
-                                                  const int* __restrict__ flag) {
+// Diagonal region of the outer panel at column c0 (kw columns, nb = kw/64
+// blocks), one workgroup, left-looking over its blocks k: A_kk minus the
+// panel's earlier columns, its FP64 factor and inverse (to linv, block k,
+// transposed), then the blocks I > k below it in the region as
+// (A_Ik - sum_{j<k} X_Ij L_kj^T) Li_kk^T.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_panel_diag(float* __restrict__ M, size_t ld, int c0, int kw,
+                                                    double* __restrict__ linv, unsigned short* __restrict__ phi,
+                                                    unsigned short* __restrict__ plo, const int* __restrict__ flag) {
     if (flag && !*flag) return;
-    const int Jt = k + 1 + (int)blockIdx.y, I = Jt + (int)blockIdx.x;
-    if (I >= ntiles) return;
-    const size_t pc = (size_t)k * NB;
-    const size_t r0 = (size_t)I * NB, c0 = (size_t)Jt * NB;
-    const int l = threadIdx.x & 63, w = wave_of();
-    const int lr = l & 15, lk = l >> 4;
-    f32x4 acc[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const float* __restrict__ arow = M + (r0 + 16 * w + lr) * ld + pc;
-#pragma unroll 4
-    for (int kc = 0; kc < NB; kc += 4) {
-        const float av = arow[kc + lk];
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const float bv = M[(c0 + 16 * t + lr) * ld + pc + kc + lk];
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[t], 0, 0, 0);
+    // the diagonal block being factored (row stride LDS_ROW) and, once it
+    // is inverted, the transposed block being solved (row stride LDT)
+    __shared__ __attribute__((aligned(16))) double SA[NB * LDT];
+    __shared__ __attribute__((aligned(16))) double LiT[NB * LDT];
+    __shared__ double T[3][16][17];
+    __shared__ double dinv[NB];
+    __shared__ double col[NB];
+    double(*S)[LDS_ROW] = reinterpret_cast<double(*)[LDS_ROW]>(SA);
+    double* const At = SA;
+    const int nb = kw / NB, tid = threadIdx.x;
+#pragma unroll 1
+    for (int k = 0; k < nb; ++k) {
+        const size_t kr = (size_t)c0 + (size_t)NB * k;
+        block_update<MODE>(M, ld, kr, kr, c0, k, phi, plo, &S[0][0], LDS_ROW, false);
+        __syncthreads();
+        potrf64(S, dinv, col);
+        for (int e = tid; e < NB * NB; e += 256) {
+            const int r = e / NB, c = e % NB;
+            if (c <= r) M[(kr + r) * ld + kr + c] = (float)S[r][c];
+        }
+        tri_inverse64(S, LiT, T, dinv);
+        for (int e = tid; e < NB * NB; e += 256) linv[(size_t)k * NB * NB + e] = LiT[(e / NB) * LDT + e % NB];
+#pragma unroll 1
+        for (int I = k + 1; I < nb; ++I) {
+            const size_t r0 = (size_t)c0 + (size_t)NB * I;
+            block_update<MODE>(M, ld, r0, kr, c0, k, phi, plo, At, LDT, true);
+            __syncthreads();
+            trsm_block<MODE>(At, LiT, M, ld, r0, kr, phi, plo, NB * k);
+            __syncthreads();
         }
     }
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const size_t row = r0 + 16 * w + 4 * (l >> 4) + q;
-            const size_t col = c0 + 16 * t + lr;
-            if (I == Jt && col > row) continue;
-            M[row * ld + col] -= acc[t][q];
-        }
+}
+
+// Rows of the outer panel below its diagonal region, 64 per workgroup, with
+// the diagonal blocks' inverses from linv: for k = 0..nb-1,
+// X_Rk = (A_Rk - sum_{j<k} X_Rj L_kj^T) Li_kk^T.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_panel_rows(float* __restrict__ M, size_t ld, int c0, int kw, int mp,
+                                                    const double* __restrict__ linv, unsigned short* __restrict__ phi,
+                                                    unsigned short* __restrict__ plo, const int* __restrict__ flag) {
+    if (flag && !*flag) return;
+    __shared__ __attribute__((aligned(16))) double At[NB * LDT];
+    __shared__ __attribute__((aligned(16))) double LiT[NB * LDT];
+    const size_t r0 = (size_t)c0 + kw + (size_t)blockIdx.x * NB;
+    if (r0 >= (size_t)mp) return;
+    const int nb = kw / NB;
+#pragma unroll 1
+    for (int k = 0; k < nb; ++k) {
+        const size_t kr = (size_t)c0 + (size_t)NB * k;
+        for (int e = threadIdx.x; e < NB * NB; e += 256) LiT[(e / NB) * LDT + e % NB] = linv[(size_t)k * NB * NB + e];
+        block_update<MODE>(M, ld, r0, kr, c0, k, phi, plo, At, LDT, true);
+        __syncthreads();
+        trsm_block<MODE>(At, LiT, M, ld, r0, kr, phi, plo, NB * k);
+        __syncthreads();
     }
 }
 
@@ -828,6 +934,112 @@ __global__ __launch_bounds__(256) void k_gemm_f32(float* __restrict__ M, size_t 
     big_store(M, ld, mp, ce, r0, c0, acc);
 }
 
+// Update of the next panel's columns (on the critical path, so built for
+// latency): C -= P_I P_J^T on 64x64 tiles, the four waves of a workgroup
+// splitting K (wave w takes the panel's columns 64w..64w+63, nothing when
+// beyond kw) and loading their MFMA operands straight from memory, all at
+// once; the four partial tiles are summed through LDS in wave order.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_gemm_ks(float* __restrict__ M, size_t ld, int rs, int cs, int ce, int mp,
+                                                 int pc, int kw, const unsigned short* __restrict__ phi,
+                                                 const unsigned short* __restrict__ plo) {
+    __shared__ float red[4][NB][NB + 1];
+    const size_t r0 = (size_t)rs + (size_t)blockIdx.x * NB, c0 = (size_t)cs + (size_t)blockIdx.y * NB;
+    if (c0 > r0 + NB - 1) return;  // wholly above the diagonal
+    const int tid = threadIdx.x, l = tid & 63, w = wave_of(), lr = l & 15;
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int k0 = NB * w;
+    if (k0 < kw) {
+        if constexpr (MODE == SOP_CHOL_BF16X3) {
+            const int lk = (l >> 4) * 8;
+            bf16x8 ah[2][4], al[2][4], bh[2][4], bl[2][4];
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const size_t ao = (r0 + 16 * b + lr) * NB2 + k0 + 32 * h + lk;
+                    const size_t bo = (c0 + 16 * b + lr) * NB2 + k0 + 32 * h + lk;
+                    ah[h][b] = *reinterpret_cast<const bf16x8*>(phi + ao);
+                    al[h][b] = *reinterpret_cast<const bf16x8*>(plo + ao);
+                    bh[h][b] = *reinterpret_cast<const bf16x8*>(phi + bo);
+                    bl[h][b] = *reinterpret_cast<const bf16x8*>(plo + bo);
+                }
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+#pragma unroll
+                for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < 4; ++bj)
+                        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[h][bi], bh[h][bj], acc[bi][bj], 0, 0, 0);
+#pragma unroll
+                for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < 4; ++bj)
+                        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[h][bi], bl[h][bj], acc[bi][bj], 0, 0, 0);
+#pragma unroll
+                for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < 4; ++bj)
+                        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[h][bi], bh[h][bj], acc[bi][bj], 0, 0, 0);
+            }
+        } else {
+            // lane group g = l/16 holds k = kc+4g..kc+4g+3; MFMA step i uses
+            // component i, so step i covers k = kc+4g+i over the groups g
+            const int lg = l >> 4;
+            float4 av[4][4], bv[4][4];
+#pragma unroll
+            for (int h = 0; h < 4; ++h)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    av[h][b] = *reinterpret_cast<const float4*>(M + (r0 + 16 * b + lr) * ld + pc + k0 + 16 * h + 4 * lg);
+                    bv[h][b] = *reinterpret_cast<const float4*>(M + (c0 + 16 * b + lr) * ld + pc + k0 + 16 * h + 4 * lg);
+                }
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+#pragma unroll
+                for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < 4; ++bj)
+                        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[h][bi].x, bv[h][bj].x, acc[bi][bj], 0, 0, 0);
+#pragma unroll
+                for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < 4; ++bj)
+                        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[h][bi].y, bv[h][bj].y, acc[bi][bj], 0, 0, 0);
+#pragma unroll
+                for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < 4; ++bj)
+                        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[h][bi].z, bv[h][bj].z, acc[bi][bj], 0, 0, 0);
+#pragma unroll
+                for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+                    for (int bj = 0; bj < 4; ++bj)
+                        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[h][bi].w, bv[h][bj].w, acc[bi][bj], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) red[w][16 * bi + 4 * (l >> 4) + q][16 * bj + lr] = acc[bi][bj][q];
+    __syncthreads();
+#pragma unroll 4
+    for (int e = tid; e < NB * NB; e += 256) {
+        const int r = e / NB, c = e % NB;
+        const size_t row = r0 + r, col = c0 + c;
+        if (row >= (size_t)mp || col >= (size_t)ce || col > row) continue;
+        const float s = ((red[0][r][c] + red[1][r][c]) + red[2][r][c]) + red[3][r][c];
+        M[row * ld + col] -= s;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -849,8 +1061,19 @@ hipError_t launch_hi_tea_step(const DevJob* jobs, const DevJobState* st, const i
 // Rows [rs, mp) x columns [cs, ce) of the trailing matrix minus the outer
 // panel (columns [pc, pc+kw)) times its transpose.
 static void trailing_update(float* M, size_t ld, int rs, int cs, int ce, int mp, int pc, int kw, int mode,
-                            const unsigned short* phi, const unsigned short* plo, hipStream_t stream) {
+                            const unsigned short* phi, const unsigned short* plo, hipStream_t stream,
+                            bool latency = false) {
     if (rs >= mp || cs >= ce) return;
+    if (latency) {
+        const dim3 g((mp - rs) / NB, (ce - cs) / NB);
+        if (mode == SOP_CHOL_F32)
+            hipLaunchKernelGGL(k_gemm_ks<SOP_CHOL_F32>, g, dim3(256), 0, stream, M, ld, rs, cs, ce, mp, pc, kw, phi,
+                               plo);
+        else
+            hipLaunchKernelGGL(k_gemm_ks<SOP_CHOL_BF16X3>, g, dim3(256), 0, stream, M, ld, rs, cs, ce, mp, pc, kw,
+                               phi, plo);
+        return;
+    }
     const dim3 g((mp - rs + GT - 1) / GT, (ce - cs + GT - 1) / GT);
     if (mode == SOP_CHOL_F32)
         hipLaunchKernelGGL(k_gemm_f32, g, dim3(256), 0, stream, M, ld, rs, cs, ce, mp, pc, kw);
@@ -864,34 +1087,31 @@ static void trailing_update(float* M, size_t ld, int rs, int cs, int ce, int mp,
 // panel p+1 is factored while T2(p) runs.  T1(p+1) and T2(p) both write
 // panel p+2's columns, so the main stream waits for T2(p) before T1(p+1).
 // With aux == nullptr everything runs on `stream`.
-hipError_t launch_chol_factor(float* M, size_t ld, int mp, int mode, double* ldiag, unsigned short* phi,
+hipError_t launch_chol_factor(float* M, size_t ld, int mp, int mode, double* linv, unsigned short* phi,
                               unsigned short* plo, const int* flag, hipStream_t stream, hipStream_t aux,
                               hipEvent_t ev_f, hipEvent_t ev_t2) {
-    const int nblk = mp / NB;
     const bool two = aux != nullptr && ev_f != nullptr && ev_t2 != nullptr;
     bool t2_pending = false;
     for (int c0 = 0; c0 < mp; c0 += NB2) {
         const int kw = mp - c0 < NB2 ? mp - c0 : NB2;
-        const int kb = c0 / NB, ke = (c0 + kw) / NB;  // inner blocks of this panel
         // split-BF16 copies of consecutive panels alternate between two
         // buffers: T2 of panel p still reads its copy while panel p+1 is factored
         const size_t pbuf = (size_t)((c0 / NB2) & 1) * mp * NB2;
         unsigned short* const ph = phi + pbuf;
         unsigned short* const pl = plo + pbuf;
-        for (int k = kb; k < ke; ++k) {
-            hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(256), 0, stream, M, ld, k, ldiag, flag);
-            const int below = mp - (k + 1) * NB;
-            if (below <= 0) break;
-            hipLaunchKernelGGL(k_trsm, dim3((below + 63) / 64), dim3(256), 0, stream, M, ld, k, mp, ldiag, ph, pl,
-                               k * NB - c0, flag);
-            if (k + 1 < ke) {  // the panel's remaining columns
-                const dim3 g(nblk - (k + 1), ke - (k + 1));
-                if (mode == SOP_CHOL_F32)
-                    hipLaunchKernelGGL(k_syrk_f32, g, dim3(256), 0, stream, M, ld, k, nblk, flag);
-                else
-                    hipLaunchKernelGGL(k_syrk_bf16x3, g, dim3(256), 0, stream, M, ld, k, nblk, ph, pl,
-                                       k * NB - c0, flag);
-            }
+        const int below = mp - c0 - kw;
+        if (mode == SOP_CHOL_F32) {
+            hipLaunchKernelGGL(k_panel_diag<SOP_CHOL_F32>, dim3(1), dim3(256), 0, stream, M, ld, c0, kw, linv, ph, pl,
+                               flag);
+            if (below > 0)
+                hipLaunchKernelGGL(k_panel_rows<SOP_CHOL_F32>, dim3(below / NB), dim3(256), 0, stream, M, ld, c0, kw,
+                                   mp, linv, ph, pl, flag);
+        } else {
+            hipLaunchKernelGGL(k_panel_diag<SOP_CHOL_BF16X3>, dim3(1), dim3(256), 0, stream, M, ld, c0, kw, linv, ph,
+                               pl, flag);
+            if (below > 0)
+                hipLaunchKernelGGL(k_panel_rows<SOP_CHOL_BF16X3>, dim3(below / NB), dim3(256), 0, stream, M, ld, c0,
+                                   kw, mp, linv, ph, pl, flag);
         }
         const int s0 = c0 + kw;
         if (s0 >= mp) break;
@@ -900,7 +1120,7 @@ hipError_t launch_chol_factor(float* M, size_t ld, int mp, int mode, double* ldi
             hipError_t e = hipEventRecord(ev_f, stream);  // panel p factored
             if (e != hipSuccess) return e;
             if (t2_pending && (e = hipStreamWaitEvent(stream, ev_t2, 0)) != hipSuccess) return e;
-            trailing_update(M, ld, s0, s0, s1, mp, c0, kw, mode, ph, pl, stream);
+            trailing_update(M, ld, s0, s0, s1, mp, c0, kw, mode, ph, pl, stream, true);
             if ((e = hipStreamWaitEvent(aux, ev_f, 0)) != hipSuccess) return e;
             trailing_update(M, ld, s1, s1, mp, mp, c0, kw, mode, ph, pl, aux);
             if ((e = hipEventRecord(ev_t2, aux)) != hipSuccess) return e;

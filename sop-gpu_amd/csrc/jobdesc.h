@@ -59,6 +59,7 @@ struct DevJob {
     int* bead_cell;
     int* bead_slot;
     int cell_cap;
+    int lds_step;   // stepped by k_step_lds (coordinates staged in LDS)
     int* pulled;    // bead indices of pulled beads
     float* energies; // 3n per-bead energies (output cadence)
     int cov_cap, nat_cap, pair_cap, pp_cap, excl_cap;
@@ -121,6 +122,11 @@ struct DevJobState {
 // Jobs of this many beads and more build their possible-pairs list from a
 // cell grid instead of the all-pairs scan (same list, same order).
 constexpr int kCellMinBeads = 8192;
+// Jobs of at most this many beads are stepped with their coordinates staged
+// in LDS (16 bytes per bead: 80 KB, two workgroups per CU).
+constexpr int kLdsStepBeads = 5120;
+// Beads per LDS-step unit (one 1024-thread workgroup).
+constexpr int kLdsUnit = 1024;
 
 // List entry k of the Verlet list (slot-major) / possible list (bead-major).
 __device__ __forceinline__ int list_at(const void* p, int idx16, size_t at) {

@@ -105,149 +105,118 @@ __device__ __forceinline__ void pair_loop(const IdxT* __restrict__ pj, int np, i
     }
 #undef SOP_PAIR
 }
-__global__ __launch_bounds__(256) void k_step(const DevJob* __restrict__ jobs, const DevJobState* __restrict__ st,
-                                              const int2* __restrict__ tiles, int ntiles, int kstep) {
-    const int lane = threadIdx.x & 63;
-    const TileRange tr = xcd_tiles(ntiles);
-    for (int t = tr.begin; t < tr.end; t += tr.step) {
-        const int2 td = tiles[t];
-        const DevJob& J = jobs[td.x];
-        const long long s = st[td.x].step + kstep;
-        if (s >= J.end_step) continue;
-        const int i = td.y + lane;
-        const int n = J.n;
-        if (i >= n) continue;
-        const float4* __restrict__ cin = J.coord[s & 1];
-        float4* __restrict__ cout = J.coord[(s + 1) & 1];
-        const float4 c = cin[i];
-        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-        if (J.flags & JF_EXTRA_FORCE) {
-            const float4 e = J.fextra[i];
-            fx = e.x; fy = e.y; fz = e.z;
-            J.fextra[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        }
-        // FENE (covalent_kernel.cu:36-48)
-        {
-            const int nc = J.cov_cnt[i];
-            const int* __restrict__ cj = J.cov_j + i;
-            const float* __restrict__ cr = J.cov_r0 + i;
-            for (int k = 0; k < nc; ++k) {
-                const int j = cj[(size_t)k * n];
-                const float r0 = cr[(size_t)k * n];
-                const float4 o = cin[j];
-                const float x = o.x - c.x, y = o.y - c.y, z = o.z - c.z;
-                float w = sqrtf(x * x + y * y + z * z);
-                w = w - r0;
-                w = J.ks * w / ((w + r0) * (1.0f - w * w / J.R2));
-                fx += w * x;
-                fy += w * y;
-                fz += w * z;
-            }
-        }
-        // native LJ 12-6 (native_kernel.cu:36-49), gathers four ahead
-        {
-            const int nn = J.nat_cnt[i];
-            const int* __restrict__ nj = J.nat_j + i;
-            const float2* __restrict__ npar = J.nat_par + i;
-#define SOP_NAT(o, par)                                                 \
-    {                                                                   \
-        const float x = (o).x - c.x, y = (o).y - c.y, z = (o).z - c.z;     \
-        float q = (par).x / (x * x + y * y + z * z);                    \
-        const float p6 = q * q * q;                                     \
-        q = q * (p6 * p6 - p6) * (par).y;                               \
-        fx += x * q;                                                    \
-        fy += y * q;                                                    \
-        fz += z * q;                                                    \
+// One bead's step: FENE, native and repulsive terms gathered from `cin`
+// (global memory, or the job's coordinates staged in LDS), indentation
+// walls, then noise, the fixed/pulled rules and the Euler update, written to
+// `cout` in global memory.  Hydrodynamic jobs stop after the forces.
+__device__ __forceinline__ void step_bead(const DevJob& J, const DevJobState& S, long long s, int i, int n,
+                                          const float4* __restrict__ cin, float4* __restrict__ cout) {
+    const float4 c = cin[i];
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    if (J.flags & JF_EXTRA_FORCE) {
+        const float4 e = J.fextra[i];
+        fx = e.x; fy = e.y; fz = e.z;
+        J.fextra[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
-            int k = 0;
-            for (; k + 4 <= nn; k += 4) {
-                int jj[4];
-                float2 pp[4];
+    // FENE (covalent_kernel.cu:36-48)
+    {
+        const int nc = J.cov_cnt[i];
+        const int* __restrict__ cj = J.cov_j + i;
+        const float* __restrict__ cr = J.cov_r0 + i;
+        for (int k = 0; k < nc; ++k) {
+            const int j = cj[(size_t)k * n];
+            const float r0 = cr[(size_t)k * n];
+            const float4 o = cin[j];
+            const float x = o.x - c.x, y = o.y - c.y, z = o.z - c.z;
+            float w = sqrtf(x * x + y * y + z * z);
+            w = w - r0;
+            w = J.ks * w / ((w + r0) * (1.0f - w * w / J.R2));
+            fx += w * x;
+            fy += w * y;
+            fz += w * z;
+        }
+    }
+    // native LJ 12-6 (native_kernel.cu:36-49), gathers four ahead
+    {
+        const int nn = J.nat_cnt[i];
+        const int* __restrict__ nj = J.nat_j + i;
+        const float2* __restrict__ npar = J.nat_par + i;
+#define SOP_NAT(o, par)                                                 \
+{                                                                   \
+    const float x = (o).x - c.x, y = (o).y - c.y, z = (o).z - c.z;     \
+    float q = (par).x / (x * x + y * y + z * z);                    \
+    const float p6 = q * q * q;                                     \
+    q = q * (p6 * p6 - p6) * (par).y;                               \
+    fx += x * q;                                                    \
+    fy += y * q;                                                    \
+    fz += z * q;                                                    \
+}
+        int k = 0;
+        for (; k + 4 <= nn; k += 4) {
+            int jj[4];
+            float2 pp[4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    jj[u] = nj[(size_t)(k + u) * n];
-                    pp[u] = npar[(size_t)(k + u) * n];
-                }
-                float4 o[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) o[u] = cin[jj[u]];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) SOP_NAT(o[u], pp[u]);
+            for (int u = 0; u < 4; ++u) {
+                jj[u] = nj[(size_t)(k + u) * n];
+                pp[u] = npar[(size_t)(k + u) * n];
             }
-            for (; k < nn; ++k) {
-                const float2 par = npar[(size_t)k * n];
-                const float4 o = cin[nj[(size_t)k * n]];
-                SOP_NAT(o, par);
-            }
+            float4 o[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) o[u] = cin[jj[u]];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) SOP_NAT(o[u], pp[u]);
+        }
+        for (; k < nn; ++k) {
+            const float2 par = npar[(size_t)k * n];
+            const float4 o = cin[nj[(size_t)k * n]];
+            SOP_NAT(o, par);
+        }
 #undef SOP_NAT
-        }
-        // repulsive LJ over the Verlet list (pairs_kernel.cu:41-56)
-        const int np = J.pair_cnt[i];
-        if (J.idx16)
-            pair_loop(static_cast<const unsigned short*>(J.pair_j) + i, np, n, cin, c, J.rc2, J.a2, J.m6, fx, fy, fz);
-        else
-            pair_loop(static_cast<const int*>(J.pair_j) + i, np, n, cin, c, J.rc2, J.a2, J.m6, fx, fy, fz);
-        const DevJobState& S = st[td.x];
-        // indentation tip and substrate plane (indentation_kernel.cu:71-128)
-        if (J.flags & JF_INDENTATION) {
-            float4 fin = J.tipF[i];
-            float x = c.x - S.tip[0], y = c.y - S.tip[1], z = c.z - S.tip[2];
-            const float rr = sqrtf(x * x + y * y + z * z);
-            float r2 = rr - J.tipR;
-            r2 = 1.0f / r2;
-            r2 = r2 * r2;
-            float r6 = r2 * r2 * r2;
-            float dw = J.tipAp * r6 + J.tipBp;
-            dw = dw * r6 / (rr * (rr - J.tipR));
-            const float dfx = x * dw, dfy = y * dw, dfz = z * dw;
-            fin.x += dfx;
-            fin.y += dfy;
-            fin.z += dfz;
-            J.tipF[i] = fin;
-            x = c.x - S.micaR[0];
-            y = c.y - S.micaR[1];
-            z = c.z - S.micaR[2];
-            float h = x * J.micaN[0] + y * J.micaN[1] + z * J.micaN[2];
-            h = 1.0f / h;
-            h = h * h;
-            r6 = h * h * h;
-            float sw = J.surfAp * r6 + J.surfBp;
-            sw = sw * r6 * h;
-            fx += J.micaN[0] * sw + dfx;
-            fy += J.micaN[1] * sw + dfy;
-            fz += J.micaN[2] * sw + dfz;
-        }
-        // hydrodynamic jobs: noise and forces for the HI integrator
-        // (integrateTea_prepare, bdhitea_kernel.cu:33-65); the update itself
-        // runs in hi.hip once every bead's forces are known
-        if (J.flags & JF_HI) {
-            uint4 rs = J.rng[i];
-            const float var = S.var;
-            const float3 g = gauss3(rs);
-            J.rng[i] = rs;
-            J.rforce[i] = make_float4(g.x * var, g.y * var, g.z * var, 0.0f);
-            if (J.flags & JF_PULLING) {
-                const float4 e = J.ext[i];
-                if (e.w == 1.0f) {
-                    fx = 0.0f; fy = 0.0f; fz = 0.0f;
-                }
-                if (e.w == 2.0f) {
-                    fx += e.x; fy += e.y; fz += e.z;
-                }
-            }
-            J.mforce[i] = make_float4(fx, fy, fz, 0.0f);
-            continue;
-        }
-        // Langevin step (langevin_kernel.cu:24-56)
-        if (!(J.flags & JF_MINIMIZATION)) {
-            uint4 rs = J.rng[i];
-            const float var = S.var;
-            const float3 g = gauss3(rs);
-            J.rng[i] = rs;
-            fx += g.x * var;
-            fy += g.y * var;
-            fz += g.z * var;
-        }
+    }
+    // repulsive LJ over the Verlet list (pairs_kernel.cu:41-56)
+    const int np = J.pair_cnt[i];
+    if (J.idx16)
+        pair_loop(static_cast<const unsigned short*>(J.pair_j) + i, np, n, cin, c, J.rc2, J.a2, J.m6, fx, fy, fz);
+    else
+        pair_loop(static_cast<const int*>(J.pair_j) + i, np, n, cin, c, J.rc2, J.a2, J.m6, fx, fy, fz);
+    // indentation tip and substrate plane (indentation_kernel.cu:71-128)
+    if (J.flags & JF_INDENTATION) {
+        float4 fin = J.tipF[i];
+        float x = c.x - S.tip[0], y = c.y - S.tip[1], z = c.z - S.tip[2];
+        const float rr = sqrtf(x * x + y * y + z * z);
+        float r2 = rr - J.tipR;
+        r2 = 1.0f / r2;
+        r2 = r2 * r2;
+        float r6 = r2 * r2 * r2;
+        float dw = J.tipAp * r6 + J.tipBp;
+        dw = dw * r6 / (rr * (rr - J.tipR));
+        const float dfx = x * dw, dfy = y * dw, dfz = z * dw;
+        fin.x += dfx;
+        fin.y += dfy;
+        fin.z += dfz;
+        J.tipF[i] = fin;
+        x = c.x - S.micaR[0];
+        y = c.y - S.micaR[1];
+        z = c.z - S.micaR[2];
+        float h = x * J.micaN[0] + y * J.micaN[1] + z * J.micaN[2];
+        h = 1.0f / h;
+        h = h * h;
+        r6 = h * h * h;
+        float sw = J.surfAp * r6 + J.surfBp;
+        sw = sw * r6 * h;
+        fx += J.micaN[0] * sw + dfx;
+        fy += J.micaN[1] * sw + dfy;
+        fz += J.micaN[2] * sw + dfz;
+    }
+    // hydrodynamic jobs: noise and forces for the HI integrator
+    // (integrateTea_prepare, bdhitea_kernel.cu:33-65); the update itself
+    // runs in hi.hip once every bead's forces are known
+    if (J.flags & JF_HI) {
+        uint4 rs = J.rng[i];
+        const float var = S.var;
+        const float3 g = gauss3(rs);
+        J.rng[i] = rs;
+        J.rforce[i] = make_float4(g.x * var, g.y * var, g.z * var, 0.0f);
         if (J.flags & JF_PULLING) {
             const float4 e = J.ext[i];
             if (e.w == 1.0f) {
@@ -257,18 +226,78 @@ __global__ __launch_bounds__(256) void k_step(const DevJob* __restrict__ jobs, c
                 fx += e.x; fy += e.y; fz += e.z;
             }
         }
-        const float m = J.hOverZeta;
-        const float dx = m * fx, dy = m * fy, dz = m * fz;
-        float4 cn = c;
-        cn.x += dx;
-        cn.y += dy;
-        cn.z += dz;
-        // the temperature accumulator is cleared at updater passes that fall
-        // on a multiple of nav (output_manager.cu:124-126)
-        const bool reset = (s % J.stride == 0) && (s % J.nav == 0);
-        const float T0 = reset ? 0.0f : J.T[i];
-        J.T[i] = T0 + J.tempNorm * (dx * dx + dy * dy + dz * dz);
-        cout[i] = cn;
+        J.mforce[i] = make_float4(fx, fy, fz, 0.0f);
+        return;
+    }
+    // Langevin step (langevin_kernel.cu:24-56)
+    if (!(J.flags & JF_MINIMIZATION)) {
+        uint4 rs = J.rng[i];
+        const float var = S.var;
+        const float3 g = gauss3(rs);
+        J.rng[i] = rs;
+        fx += g.x * var;
+        fy += g.y * var;
+        fz += g.z * var;
+    }
+    if (J.flags & JF_PULLING) {
+        const float4 e = J.ext[i];
+        if (e.w == 1.0f) {
+            fx = 0.0f; fy = 0.0f; fz = 0.0f;
+        }
+        if (e.w == 2.0f) {
+            fx += e.x; fy += e.y; fz += e.z;
+        }
+    }
+    const float m = J.hOverZeta;
+    const float dx = m * fx, dy = m * fy, dz = m * fz;
+    float4 cn = c;
+    cn.x += dx;
+    cn.y += dy;
+    cn.z += dz;
+    // the temperature accumulator is cleared at updater passes that fall
+    // on a multiple of nav (output_manager.cu:124-126)
+    const bool reset = (s % J.stride == 0) && (s % J.nav == 0);
+    const float T0 = reset ? 0.0f : J.T[i];
+    J.T[i] = T0 + J.tempNorm * (dx * dx + dy * dy + dz * dz);
+    cout[i] = cn;
+}
+
+__global__ __launch_bounds__(256) void k_step(const DevJob* __restrict__ jobs, const DevJobState* __restrict__ st,
+                                              const int2* __restrict__ tiles, int ntiles, int kstep) {
+    const int lane = threadIdx.x & 63;
+    const TileRange tr = xcd_tiles(ntiles);
+    for (int t = tr.begin; t < tr.end; t += tr.step) {
+        const int2 td = tiles[t];
+        const DevJob& J = jobs[td.x];
+        if (J.lds_step) continue;  // stepped by k_step_lds
+        const long long s = st[td.x].step + kstep;
+        if (s >= J.end_step) continue;
+        const int i = td.y + lane;
+        const int n = J.n;
+        if (i >= n) continue;
+        step_bead(J, st[td.x], s, i, n, J.coord[s & 1], J.coord[(s + 1) & 1]);
+    }
+}
+
+// The same step for jobs small enough to keep their coordinates in LDS:
+// one 1024-thread workgroup per unit of up to kLdsUnit beads of one job; the
+// workgroup stages all of the job's coordinates (coalesced) and every
+// neighbour gather of the step reads LDS.
+__global__ __launch_bounds__(1024) void k_step_lds(const DevJob* __restrict__ jobs, const DevJobState* __restrict__ st,
+                                                   const int2* __restrict__ units, int nunits, int kstep) {
+    extern __shared__ float4 lds_coord[];
+    for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
+        const int2 ud = units[u];
+        const DevJob& J = jobs[ud.x];
+        const long long s = st[ud.x].step + kstep;
+        if (s >= J.end_step) continue;
+        const int n = J.n;
+        const float4* __restrict__ g = J.coord[s & 1];
+        __syncthreads();  // the previous unit's gathers are done
+        for (int k = threadIdx.x; k < n; k += blockDim.x) lds_coord[k] = g[k];
+        __syncthreads();
+        const int i = ud.y + threadIdx.x;
+        if (i < n && i < ud.y + kLdsUnit) step_bead(J, st[ud.x], s, i, n, lds_coord, J.coord[(s + 1) & 1]);
     }
 }
 
@@ -1010,10 +1039,15 @@ static int grid_for(int ntiles, int max_blocks) {
     return b < 1 ? 1 : b;
 }
 
-hipError_t launch_step(const DevJob* jobs, const DevJobState* st, const int2* tiles, int ntiles, int kstep,
-                       int max_blocks, hipStream_t stream) {
+hipError_t launch_step(const DevJob* jobs, const DevJobState* st, const int2* tiles, int ntiles, int any_global,
+                       const int2* units, int nunits, int lds_beads, int kstep, int max_blocks, hipStream_t stream) {
     if (ntiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_step, dim3(grid_for(ntiles, max_blocks)), dim3(256), 0, stream, jobs, st, tiles, ntiles, kstep);
+    if (any_global)
+        hipLaunchKernelGGL(k_step, dim3(grid_for(ntiles, max_blocks)), dim3(256), 0, stream, jobs, st, tiles, ntiles,
+                           kstep);
+    if (nunits > 0)
+        hipLaunchKernelGGL(k_step_lds, dim3(nunits), dim3(kLdsUnit), (size_t)lds_beads * sizeof(float4), stream, jobs,
+                           st, units, nunits, kstep);
     return hipGetLastError();
 }
 

@@ -198,6 +198,7 @@ _sig("sop_engine_remove_job", C.c_int, VP, C.c_int)
 _sig("sop_engine_job_bytes", C.c_int, C.POINTER(Params), C.POINTER(SystemStruct), C.POINTER(C.c_ulonglong))
 _sig("sop_engine_set_tuning", C.c_int, VP, C.c_int, C.c_int)
 _sig("sop_engine_set_cells", C.c_int, VP, C.c_int, C.c_int)
+_sig("sop_engine_set_lds_step", C.c_int, VP, C.c_int)
 _sig("sop_engine_autotune", C.c_int, VP, C.POINTER(C.c_int))
 _sig("sop_engine_run_blocks", C.c_int, VP, C.c_int)
 _sig("sop_engine_run_job", C.c_int, VP, C.c_int)
@@ -248,7 +249,7 @@ _sig("sop_run_config", C.c_int, C.c_char_p, C.c_int, C.POINTER(C.c_char_p), C.PO
 
 EXPORTED = [
     "sop_last_error", "sop_version", "sop_engine_create", "sop_engine_destroy", "sop_engine_add_job",
-    "sop_engine_remove_job", "sop_engine_job_bytes", "sop_engine_set_tuning", "sop_engine_set_cells", "sop_engine_autotune",
+    "sop_engine_remove_job", "sop_engine_job_bytes", "sop_engine_set_tuning", "sop_engine_set_cells", "sop_engine_set_lds_step", "sop_engine_autotune",
     "sop_engine_run_blocks", "sop_engine_run_job", "sop_engine_block_steps", "sop_engine_synchronize",
     "sop_engine_job_status", "sop_engine_active_jobs", "sop_engine_get_coords", "sop_engine_set_coords",
     "sop_engine_get_temperature", "sop_engine_get_rng", "sop_engine_get_ext", "sop_engine_list_cap",
@@ -391,6 +392,9 @@ class Engine:
 
     def set_cells(self, min_beads: int = 0, enabled: bool = True):
         check(lib.sop_engine_set_cells(self.h, min_beads, int(enabled)))
+
+    def set_lds_step(self, enabled: bool = True):
+        check(lib.sop_engine_set_lds_step(self.h, int(enabled)))
 
     def autotune(self) -> int:
         b = C.c_int()

@@ -150,6 +150,42 @@ def test_batched_equals_plain_and_alone_equals_mixed():
             np.testing.assert_array_equal(e2.coords(jid), ref)
 
 
+def test_lds_resident_step_equals_global_gathers():
+    """Jobs of up to 5120 beads step from coordinates staged in LDS; the
+    result is bit-identical to the kernel that gathers from HBM, for sizes
+    on either side of the 1024-bead work unit and of the 5120-bead limit,
+    batched and alone."""
+    sizes = [64, 1000, 1024, 1025, 2049, 5120, 5121]
+    jobs = []
+    for k, n in enumerate(sizes):
+        sysm = _pull_system(n, 60 + k)
+        kw = c0_params(temperature=[0.0, 0.6][k % 2], seed=300 + k, pulling=1, fixedEnd=0, pulledEnd=n - 1,
+                       deltax=0.02, pullFreq=100, nav=100, outputtiming=100, dcdfreq=100)
+        jobs.append((sysm, kw))
+    steps = 290
+    out = {}
+    for lds in (True, False):
+        print("case: mixed batch %s, lds=%s" % (sizes, lds), flush=True)
+        with S.Engine(0, 4 << 30) as e:
+            e.set_lds_step(lds)
+            ids = [e.add_job(S.make_params(**kw), sysm, steps) for sysm, kw in jobs]
+            L = e.block_steps()
+            e.run_blocks((steps // 10 + 1) * 10 // L)
+            out[lds] = [e.coords(j) for j in ids]
+            for j in ids:
+                assert e.status(j)[0] == 300
+        print("case: 2049 beads alone, lds=%s" % lds, flush=True)
+        with S.Engine(0, 1 << 30) as e:
+            e.set_lds_step(lds)
+            j = e.add_job(S.make_params(**jobs[4][1]), jobs[4][0], steps)
+            e.run_job(j)
+            out[(lds, "alone")] = e.coords(j)
+    for a, b in zip(out[True], out[False]):
+        np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(out[(True, "alone")], out[(False, "alone")])
+    np.testing.assert_array_equal(out[(True, "alone")], out[True][4])
+
+
 def test_job_admitted_mid_run_matches_alone():
     n1, n2 = 300, 450
     s1, s2 = _pull_system(n1, 40), _pull_system(n2, 41)
