@@ -449,7 +449,7 @@ static unsigned long long built_bytes(const BuiltJob& b, bool pulling, bool inde
     t += round_up(n * 4);                 // T
     if (pulling) t += round_up(n * 16);   // ext
     if (indentation) t += round_up(n * 16);
-    t += round_up(n * 4) * 5;             // five count arrays
+    t += round_up(n * 4) * 6;             // five count arrays, Verlet row order
     t += round_up(n * b.cov_cap * 4) * 2; // cov j, r0
     t += round_up(n * b.nat_cap * 4) + round_up(n * b.nat_cap * 8);
     const unsigned long long es = b.dj.idx16 ? 2 : 4;
@@ -507,6 +507,7 @@ int Engine::add_job(const sop_params& p, const sop_system& s, long long numsteps
     J.nat_j = (int*)A(n * b.nat_cap * 4);
     J.nat_par = (float2*)A(n * b.nat_cap * 8);
     J.pair_cnt = (int*)A(n * 4);
+    J.perm = (int*)A(n * 4);
     const size_t es = J.idx16 ? 2 : 4;
     J.pair_j = A(n * b.pair_cap * es);
     J.pp_cnt = (int*)A(n * 4);
@@ -517,6 +518,7 @@ int Engine::add_job(const sop_params& p, const sop_system& s, long long numsteps
     J.npulled = (int)b.pulled.size();
     J.energies = (float*)A(n * 12);
     J.lds_step = use_lds_step_ && n <= (size_t)kLdsStepBeads ? 1 : 0;
+    J.sort_rows = J.lds_step && !(J.flags & JF_HI);
     if (uses_cells(n)) {
         J.cell_cap = (int)n + 64;
         J.cell_cnt = (int*)A((size_t)J.cell_cap * 4);
@@ -578,6 +580,9 @@ int Engine::add_job(const sop_params& p, const sop_system& s, long long numsteps
     H2D(J.nat_j, b.nat_j.data(), nbytes(b.nat_j));
     H2D(J.nat_par, b.nat_par.data(), nbytes(b.nat_par));
     H2D(J.pair_cnt, zeros.data(), n * 4);
+    std::vector<int> ident(n);
+    for (size_t k = 0; k < n; ++k) ident[k] = (int)k;
+    H2D(J.perm, ident.data(), n * 4);
     H2D(J.pp_cnt, b.pp_cnt.data(), nbytes(b.pp_cnt));
     std::vector<unsigned short> pp16;
     if (J.idx16) {
@@ -766,7 +771,10 @@ void Engine::set_cells(int min_beads, int enabled) {
 
 void Engine::set_lds_step(int enabled) {
     use_lds_step_ = enabled != 0;
-    for (auto& j : jobs_) j.dj.lds_step = use_lds_step_ && j.n <= kLdsStepBeads ? 1 : 0;
+    for (auto& j : jobs_) {
+        j.dj.lds_step = use_lds_step_ && j.n <= kLdsStepBeads ? 1 : 0;
+        j.dj.sort_rows = j.dj.lds_step && !(j.dj.flags & JF_HI);
+    }
     tables_dirty_ = true;
     drop_graph();
 }
@@ -823,7 +831,11 @@ int Engine::launch_block(int slot_only) {
     const int cw = slot_only >= 0 ? cell_words_of(jobs_[slot_only].dj) : cell_words_;
     if (int rc = check(launch_possible(jobs, st, tiles, nt, nj, cw, 0, stream_), "possible pairs")) return rc;
     const int i32 = slot_only >= 0 ? !jobs_[slot_only].dj.idx16 : any_idx32_;
-    if (int rc = check(launch_verlet(jobs, st, tiles, nt, i32, 0, stream_), "verlet")) return rc;
+    const int nu1 = slot_only >= 0 && jobs_[slot_only].dj.sort_rows ? (jobs_[slot_only].n + kLdsUnit - 1) / kLdsUnit : 0;
+    if (int rc = check(slot_only >= 0 ? launch_verlet(jobs, st, tiles, nt, i32, d_units1_, nu1, 0, stream_)
+                                      : launch_verlet(jobs, st, tiles, nt, i32, d_units_, nunits_, 0, stream_),
+                       "verlet"))
+        return rc;
     const bool tea = slot_only >= 0 ? ((jobs_[slot_only].dj.flags & JF_HI) && !(jobs_[slot_only].dj.flags & JF_HI_EXACT))
                                     : any_tea_;
     const bool exact = slot_only >= 0 ? (jobs_[slot_only].dj.flags & JF_HI_EXACT) != 0 : any_exact_;
@@ -930,7 +942,8 @@ int Engine::stage_possible() {
 }
 int Engine::stage_verlet() {
     if (int rc = sync_tables()) return rc;
-    return check(launch_verlet(d_jobs_, d_state_, d_tiles_, ntiles_, any_idx32_, 0, stream_), "verlet");
+    return check(launch_verlet(d_jobs_, d_state_, d_tiles_, ntiles_, any_idx32_, d_units_, nunits_, 0, stream_),
+                 "verlet");
 }
 int Engine::stage_tea_update() {
     if (int rc = sync_tables()) return rc;
@@ -1135,6 +1148,16 @@ int Engine::get_list(int id, int which, int* counts, int* entries) {
     } else {
         if (int rc = check(hipMemcpy(words.data(), e, n * cap * 4, hipMemcpyDeviceToHost), "list entries")) return rc;
     }
+    if (which == 2) {  // row r holds bead perm[r]'s list
+        std::vector<int> perm(n), rc(counts, counts + n);
+        if (int rv = check(hipMemcpy(perm.data(), J.perm, n * 4, hipMemcpyDeviceToHost), "row order")) return rv;
+        for (size_t r = 0; r < n; ++r) {
+            const size_t i = (size_t)perm[r];
+            counts[i] = rc[r];
+            for (int k = 0; k < cap; ++k) entries[(size_t)k * n + i] = words[(size_t)k * n + r];
+        }
+        return 0;
+    }
     if (which != 3) {
         std::memcpy(entries, words.data(), n * cap * 4);
         return 0;
@@ -1223,8 +1246,10 @@ int Engine::rebuild_lists(int id, int mask) {
                            "possible pairs"))
             return rc;
     if (mask & 2)
-        if (int rc = check(launch_verlet(d_jobs_ + slot, d_state_ + slot, d_tiles1_, nt, !jobs_[slot].dj.idx16, 1,
-                                         stream_),
+        if (int rc = check(launch_verlet(d_jobs_ + slot, d_state_ + slot, d_tiles1_, nt, !jobs_[slot].dj.idx16,
+                                         d_units1_,
+                                         jobs_[slot].dj.sort_rows ? (jobs_[slot].n + kLdsUnit - 1) / kLdsUnit : 0,
+                                         1, stream_),
                            "verlet"))
             return rc;
     return synchronize();

@@ -44,8 +44,16 @@ struct DevJob {
     float2* nat_par; // (r0^2, -12 eh / r0^2)
     // Verlet and possible lists hold job-local bead indices as uint16 when the
     // job has fewer than 65536 beads (idx16), as int32 otherwise
-    int* pair_cnt;
-    void* pair_j;   // slot-major: entry k of bead i at [k*n + i]
+    // Verlet list rows: row p holds the list of bead perm[p] (slot-major:
+    // entry k of row p at [k*n + p]) and the wavefront that steps rows
+    // 64g..64g+63 steps those beads.  For jobs stepped by k_step_lds (not
+    // hydrodynamic) the beads of each kLdsUnit-row unit are re-sorted by
+    // list length at every Verlet rebuild, so a wavefront's lists are of
+    // about equal length; elsewhere perm is the identity.
+    int* perm;
+    int sort_rows;
+    int* pair_cnt;  // per row
+    void* pair_j;
     int* pp_cnt;
     void* pp_j;     // bead-major: entry k of bead i at [i*pp_cap + k]
     int* excl_cnt;  // sorted covalent+native partners (possible-pair exclusions)

@@ -109,7 +109,7 @@ __device__ __forceinline__ void pair_loop(const IdxT* __restrict__ pj, int np, i
 // (global memory, or the job's coordinates staged in LDS), indentation
 // walls, then noise, the fixed/pulled rules and the Euler update, written to
 // `cout` in global memory.  Hydrodynamic jobs stop after the forces.
-__device__ __forceinline__ void step_bead(const DevJob& J, const DevJobState& S, long long s, int i, int n,
+__device__ __forceinline__ void step_bead(const DevJob& J, const DevJobState& S, long long s, int i, int r, int n,
                                           const float4* __restrict__ cin, float4* __restrict__ cout) {
     const float4 c = cin[i];
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
@@ -174,11 +174,11 @@ __device__ __forceinline__ void step_bead(const DevJob& J, const DevJobState& S,
 #undef SOP_NAT
     }
     // repulsive LJ over the Verlet list (pairs_kernel.cu:41-56)
-    const int np = J.pair_cnt[i];
+    const int np = J.pair_cnt[r];
     if (J.idx16)
-        pair_loop(static_cast<const unsigned short*>(J.pair_j) + i, np, n, cin, c, J.rc2, J.a2, J.m6, fx, fy, fz);
+        pair_loop(static_cast<const unsigned short*>(J.pair_j) + r, np, n, cin, c, J.rc2, J.a2, J.m6, fx, fy, fz);
     else
-        pair_loop(static_cast<const int*>(J.pair_j) + i, np, n, cin, c, J.rc2, J.a2, J.m6, fx, fy, fz);
+        pair_loop(static_cast<const int*>(J.pair_j) + r, np, n, cin, c, J.rc2, J.a2, J.m6, fx, fy, fz);
     // indentation tip and substrate plane (indentation_kernel.cu:71-128)
     if (J.flags & JF_INDENTATION) {
         float4 fin = J.tipF[i];
@@ -272,10 +272,10 @@ __global__ __launch_bounds__(256) void k_step(const DevJob* __restrict__ jobs, c
         if (J.lds_step) continue;  // stepped by k_step_lds
         const long long s = st[td.x].step + kstep;
         if (s >= J.end_step) continue;
-        const int i = td.y + lane;
+        const int r = td.y + lane;  // Verlet row; the lane steps bead perm[r]
         const int n = J.n;
-        if (i >= n) continue;
-        step_bead(J, st[td.x], s, i, n, J.coord[s & 1], J.coord[(s + 1) & 1]);
+        if (r >= n) continue;
+        step_bead(J, st[td.x], s, J.perm[r], r, n, J.coord[s & 1], J.coord[(s + 1) & 1]);
     }
 }
 
@@ -296,8 +296,8 @@ __global__ __launch_bounds__(1024) void k_step_lds(const DevJob* __restrict__ jo
         __syncthreads();  // the previous unit's gathers are done
         for (int k = threadIdx.x; k < n; k += blockDim.x) lds_coord[k] = g[k];
         __syncthreads();
-        const int i = ud.y + threadIdx.x;
-        if (i < n && i < ud.y + kLdsUnit) step_bead(J, st[ud.x], s, i, n, lds_coord, J.coord[(s + 1) & 1]);
+        const int r = ud.y + threadIdx.x;  // Verlet row; the thread steps bead perm[r]
+        if (r < n && r < ud.y + kLdsUnit) step_bead(J, st[ud.x], s, J.perm[r], r, n, lds_coord, J.coord[(s + 1) & 1]);
     }
 }
 
@@ -657,11 +657,12 @@ __global__ __launch_bounds__(64) void k_possible_cells(const DevJob* __restrict_
     }
 }
 
-// Verlet list: one 256-thread workgroup per 64-bead tile.  Wave w filters
-// the possible rows of beads 16w..16w+15 in stored order with the ballot
-// compaction; the accepted entries of the tile collect in LDS rows, and the
-// workgroup then writes them out slot-major (pair_j[k*n + i]) so each store
-// covers the 64 beads of one slot.  A bead's row is read eight 64-entry
+// Verlet list: one 256-thread workgroup per tile of 64 list rows (the
+// beads perm[row]).  Wave w filters the possible lists of the beads of rows
+// 16w..16w+15 in stored order with the ballot compaction; the accepted
+// entries of the tile collect in LDS rows, and the workgroup then writes
+// them out slot-major (pair_j[k*n + row]) so each store covers the 64 rows
+// of one slot.  A bead's row is read eight 64-entry
 // chunks at a time: all eight index loads, then all eight coordinate
 // gathers, are in flight before the first compare.  Entries past the LDS
 // row (VerletLds::entries) are rare and stored to their slot directly.
@@ -689,11 +690,13 @@ __device__ __forceinline__ void verlet_tile(const DevJob& J, DevJobState& S, con
     const int cap = J.pair_cap, ppcap = J.pp_cap;
     const IdxT* __restrict__ pp = static_cast<const IdxT*>(J.pp_j);
     IdxT* __restrict__ pj = static_cast<IdxT*>(J.pair_j);
-    const int cl = lane < nrow ? J.pp_cnt[i0 + lane] : 0;  // possible counts of the tile
+    const int bl = lane < nrow ? J.perm[i0 + lane] : 0;    // beads of the tile's rows
+    const int cl = lane < nrow ? J.pp_cnt[bl] : 0;         // their possible counts
     for (int b = b_lo; b < b_hi; ++b) {
         const int npp = __shfl(cl, b);
-        const float4 r1 = cur[i0 + b];
-        const IdxT* __restrict__ row = pp + (size_t)(i0 + b) * ppcap;
+        const int bead = __shfl(bl, b);
+        const float4 r1 = cur[bead];
+        const IdxT* __restrict__ row = pp + (size_t)bead * ppcap;
         int cnt = 0;
         for (int base = 0; base < npp; base += 64 * CH) {
             int j[CH];
@@ -749,6 +752,43 @@ __device__ __forceinline__ void verlet_store(const DevJob& J, int i0, const IdxT
     const int lim = min(c, E);
     for (int k = kq; k < maxc; k += 4)
         if (k < lim) pj[(size_t)k * n + i0 + b] = rows[b * RS + k];
+}
+
+// Re-sorts the rows of each kLdsUnit-row unit of a job by Verlet list
+// length (longest first, ties by bead), just before the rebuild that fills
+// the rows again, with the lengths of the last rebuild as keys: the 64 rows
+// of a wavefront then hold lists of about the same length, so its loop
+// over the longest list idles few lanes and reads few padding slots.  Only
+// which row and lane serve a bead changes, never the arithmetic, so the
+// trajectories are the same bits in any row order.  Bitonic sort in LDS.
+__global__ __launch_bounds__(1024) void k_sort_rows(const DevJob* __restrict__ jobs, const DevJobState* __restrict__ st,
+                                                    const int2* __restrict__ units, int nunits, int force) {
+    __shared__ unsigned long long key[kLdsUnit];
+    const int t = threadIdx.x;
+    for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
+        const int2 ud = units[u];
+        const DevJob& J = jobs[ud.x];
+        if (!J.sort_rows) continue;
+        if (!force && !due(J, st[ud.x].step, J.pairs_freq)) continue;
+        const int m = min(kLdsUnit, J.n - ud.y), r = ud.y + t;
+        __syncthreads();  // the previous unit's keys are read
+        key[t] = t < m ? ((unsigned long long)(unsigned)(J.pair_cap - J.pair_cnt[r]) << 32) | (unsigned)J.perm[r]
+                       : ~0ull;
+        __syncthreads();
+        for (int k = 2; k <= kLdsUnit; k <<= 1)
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                const int q = t ^ j;
+                if (q > t) {
+                    const unsigned long long a = key[t], b = key[q];
+                    if ((a > b) == ((t & k) == 0)) {
+                        key[t] = b;
+                        key[q] = a;
+                    }
+                }
+                __syncthreads();
+            }
+        if (t < m) J.perm[r] = (int)(unsigned)(key[t] & 0xffffffffull);
+    }
 }
 
 __global__ __launch_bounds__(256) void k_verlet(const DevJob* __restrict__ jobs, DevJobState* __restrict__ st,
@@ -877,8 +917,9 @@ __global__ __launch_bounds__(256) void k_energies(const DevJob* __restrict__ job
                                                   int job) {
     const DevJob& J = jobs[job];
     const int n = J.n;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;  // Verlet row of bead i
+    if (r >= n) return;
+    const int i = J.perm[r];
     const long long s = st[job].step;
     const float4* __restrict__ cur = J.coord[s & 1];
     const float4 c = cur[i];
@@ -908,8 +949,8 @@ __global__ __launch_bounds__(256) void k_energies(const DevJob* __restrict__ job
     }
     J.energies[n + i] = pot;
     pot = 0.0f;
-    for (int k = 0; k < J.pair_cnt[i]; ++k) {
-        const int j = list_at(J.pair_j, J.idx16, (size_t)k * n + i);
+    for (int k = 0; k < J.pair_cnt[r]; ++k) {
+        const int j = list_at(J.pair_j, J.idx16, (size_t)k * n + r);
         const float4 o = cur[j];
         const float x = o.x - c.x, y = o.y - c.y, z = o.z - c.z;
         const float d2 = x * x + y * y + z * z;
@@ -927,8 +968,9 @@ __global__ __launch_bounds__(256) void k_forces(const DevJob* __restrict__ jobs,
                                                 int job, float4* __restrict__ out) {
     const DevJob& J = jobs[job];
     const int n = J.n;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;  // Verlet row of bead i
+    if (r >= n) return;
+    const int i = J.perm[r];
     const long long s = st[job].step;
     const float4* __restrict__ cin = J.coord[s & 1];
     const float4 c = cin[i];
@@ -957,8 +999,8 @@ __global__ __launch_bounds__(256) void k_forces(const DevJob* __restrict__ jobs,
         fy += y * q;
         fz += z * q;
     }
-    for (int k = 0; k < J.pair_cnt[i]; ++k) {
-        const int j = list_at(J.pair_j, J.idx16, (size_t)k * n + i);
+    for (int k = 0; k < J.pair_cnt[r]; ++k) {
+        const int j = list_at(J.pair_j, J.idx16, (size_t)k * n + r);
         const float4 o = cin[j];
         const float x = o.x - c.x, y = o.y - c.y, z = o.z - c.z;
         const float d2 = x * x + y * y + z * z;
@@ -1073,8 +1115,9 @@ hipError_t launch_possible(const DevJob* jobs, DevJobState* st, const int2* tile
 }
 
 hipError_t launch_verlet(const DevJob* jobs, DevJobState* st, const int2* tiles, int ntiles, int any_idx32,
-                         int force, hipStream_t stream) {
+                         const int2* units, int nunits, int force, hipStream_t stream) {
     if (ntiles <= 0) return hipSuccess;
+    if (nunits > 0) hipLaunchKernelGGL(k_sort_rows, dim3(nunits), dim3(kLdsUnit), 0, stream, jobs, st, units, nunits, force);
     int blocks = ntiles;
     if (blocks >= 8) blocks = (blocks + 7) / 8 * 8;
     const size_t lds = any_idx32 ? 64 * VerletLds<int>::stride * sizeof(int)
