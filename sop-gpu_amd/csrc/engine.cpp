@@ -13,6 +13,22 @@
 
 namespace sop {
 
+// Device copy of a host table, growing the buffer when needed.
+template <typename T>
+static hipError_t upload_grow(T*& d, int& cap, const std::vector<T>& v, hipStream_t stream) {
+    const int m = (int)v.size();
+    if (m > cap) {
+        (void)hipFree(d);
+        d = nullptr;
+        cap = 0;
+        const int c = std::max(64, 2 * m);
+        if (hipError_t e = hipMalloc(&d, (size_t)c * sizeof(T))) return e;
+        cap = c;
+    }
+    return m ? hipMemcpyAsync(d, v.data(), (size_t)m * sizeof(T), hipMemcpyHostToDevice, stream) : hipSuccess;
+}
+
+
 static thread_local std::string g_error;
 static thread_local int g_code = 0;
 
@@ -136,6 +152,10 @@ Engine::~Engine() {
     (void)hipFree(d_tiles1_);
     (void)hipFree(d_units_);
     (void)hipFree(d_units1_);
+    (void)hipFree(d_sunits_);
+    (void)hipFree(d_segs_);
+    (void)hipFree(d_sunits1_);
+    (void)hipFree(d_segs1_);
     (void)hipFree(d_scratch4_);
     if (aux_stream_) {
         (void)hipStreamSynchronize(aux_stream_);
@@ -726,7 +746,6 @@ int Engine::sync_tables() {
             any_global_ = 1;
             continue;
         }
-        lds_beads_ = std::max(lds_beads_, J.n);
         for (int b = 0; b < J.n; b += kLdsUnit) units.push_back(make_int2(k, b));
     }
     nunits_ = (int)units.size();
@@ -743,10 +762,75 @@ int Engine::sync_tables() {
                            "unit upload"))
             return rc;
     }
+    // LDS step plan
+    std::vector<int2> su;
+    std::vector<int4> sg;
+    lds_beads_ = plan_lds_units(su, sg);
+    nsunits_ = (int)su.size();
+    if (int rc = check(upload_grow(d_sunits_, cap_sunits_, su, stream_), "step units")) return rc;
+    if (int rc = check(upload_grow(d_segs_, cap_segs_, sg, stream_), "step segments")) return rc;
     if (int rc = check(hipStreamSynchronize(stream_), "table sync")) return rc;
     tables_dirty_ = false;
     synced_jobs_ = nj;
     return 0;
+}
+
+// The LDS step plan of the LDS-step jobs: each whole kLdsUnit-row block of
+// a job is a unit; the rows left over (or a whole small job) are packed,
+// largest first, into units of at most kLdsUnit threads (segments rounded
+// up to 64 rows) and kLdsStepBeads staged beads.  Returns the largest
+// number of beads a unit stages.
+int Engine::plan_lds_units(std::vector<int2>& units, std::vector<int4>& segs) const {
+    struct Part {
+        int job, row, rows, n;
+    };
+    std::vector<Part> parts;
+    int lds = 0;
+    units.clear();
+    segs.clear();
+    for (int k = 0; k < (int)jobs_.size(); ++k) {
+        const DevJob& J = jobs_[k].dj;
+        if (!J.lds_step) continue;
+        int r = 0;
+        for (; r + kLdsUnit <= J.n; r += kLdsUnit) {
+            units.push_back(make_int2((int)segs.size(), 1));
+            segs.push_back(make_int4(k, r, kLdsUnit, 0));
+            lds = std::max(lds, J.n);
+        }
+        if (r < J.n) parts.push_back({k, r, J.n - r, J.n});
+    }
+    std::stable_sort(parts.begin(), parts.end(), [](const Part& a, const Part& b) { return a.rows > b.rows; });
+    struct Bin {
+        int threads, beads;
+        std::vector<int> items;
+    };
+    std::vector<Bin> bins;
+    for (int q = 0; q < (int)parts.size(); ++q) {
+        const int span = (parts[q].rows + 63) / 64 * 64;
+        Bin* fit = nullptr;
+        for (auto& b : bins)
+            if (b.threads + span <= kLdsUnit && b.beads + parts[q].n <= kLdsStepBeads) {
+                fit = &b;
+                break;
+            }
+        if (!fit) {
+            bins.push_back({0, 0, {}});
+            fit = &bins.back();
+        }
+        fit->threads += span;
+        fit->beads += parts[q].n;
+        fit->items.push_back(q);
+    }
+    for (auto& b : bins) {
+        units.push_back(make_int2((int)segs.size(), (int)b.items.size()));
+        int off = 0;
+        for (int q : b.items) {
+            segs.push_back(make_int4(parts[q].job, parts[q].row, parts[q].rows, off));
+            off += parts[q].n;
+        }
+        lds = std::max(lds, off);
+    }
+    return lds;
 }
 
 void Engine::drop_graph() {
@@ -861,12 +945,11 @@ int Engine::launch_block(int slot_only) {
             (void)hipEventRecord(ev_pool_[ev_used_].first, stream_);
         }
         if (int rc = check(slot_only >= 0
-                               ? launch_step(jobs, st, tiles, nt, !jobs_[slot_only].dj.lds_step, d_units1_,
-                                             jobs_[slot_only].dj.lds_step ? (jobs_[slot_only].n + kLdsUnit - 1) / kLdsUnit
-                                                                          : 0,
-                                             jobs_[slot_only].n, k, max_blocks_, stream_)
-                               : launch_step(jobs, st, tiles, nt, any_global_, d_units_, nunits_, lds_beads_, k,
-                                             max_blocks_, stream_),
+                               ? launch_step(jobs, st, tiles, nt, !jobs_[slot_only].dj.lds_step, d_sunits1_, d_segs1_,
+                                             jobs_[slot_only].dj.lds_step ? nsunits1_ : 0, jobs_[slot_only].n, k,
+                                             max_blocks_, stream_)
+                               : launch_step(jobs, st, tiles, nt, any_global_, d_sunits_, d_segs_, nsunits_, lds_beads_,
+                                             k, max_blocks_, stream_),
                            "step"))
             return rc;
         if (prof_end) {
@@ -923,10 +1006,19 @@ int Engine::tiles_for_slot(int slot, int* nt) {
         if (int rc = check(hipMalloc(&d_units1_, nu * sizeof(int2)), "unit table")) return rc;
         cap_units1_ = nu;
     }
-    std::vector<int2> u1(nu);
-    for (int u = 0; u < nu; ++u) u1[u] = make_int2(0, kLdsUnit * u);
+    std::vector<int2> u1(nu), su1(nu);
+    std::vector<int4> sg1(nu);
+    for (int u = 0; u < nu; ++u) {
+        u1[u] = make_int2(0, kLdsUnit * u);
+        su1[u] = make_int2(u, 1);
+        sg1[u] = make_int4(0, kLdsUnit * u, std::min(kLdsUnit, n - kLdsUnit * u), 0);
+    }
     if (int rc = check(hipMemcpy(d_units1_, u1.data(), nu * sizeof(int2), hipMemcpyHostToDevice), "unit upload"))
         return rc;
+    if (int rc = check(upload_grow(d_sunits1_, cap_sunits1_, su1, stream_), "step units")) return rc;
+    if (int rc = check(upload_grow(d_segs1_, cap_segs1_, sg1, stream_), "step segments")) return rc;
+    if (int rc = synchronize()) return rc;
+    nsunits1_ = nu;
     tiles1_n_ = n;
     return 0;
 }
@@ -952,8 +1044,8 @@ int Engine::stage_tea_update() {
 }
 int Engine::stage_step(int k) {
     if (int rc = sync_tables()) return rc;
-    if (int rc = check(launch_step(d_jobs_, d_state_, d_tiles_, ntiles_, any_global_, d_units_, nunits_, lds_beads_, k,
-                                   max_blocks_, stream_),
+    if (int rc = check(launch_step(d_jobs_, d_state_, d_tiles_, ntiles_, any_global_, d_sunits_, d_segs_, nsunits_,
+                                   lds_beads_, k, max_blocks_, stream_),
                        "step"))
         return rc;
     if (any_tea_)

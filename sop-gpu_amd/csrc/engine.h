@@ -150,6 +150,7 @@ class Engine {
     int cap_tiles1_ = 0;
     int tiles1_n_ = -1;  // bead count the single-job tile list was built for
     int tiles_for_slot(int slot, int* nt);
+    int plan_lds_units(std::vector<int2>& units, std::vector<int4>& segs) const;
     // tuning
     int max_blocks_ = 0;
     int cell_min_ = kCellMinBeads;
@@ -157,10 +158,18 @@ class Engine {
     int cell_words_ = 0;
     int any_idx32_ = 0;  // some job stores int32 list entries
     bool use_lds_step_ = true;
-    int2* d_units_ = nullptr;  // LDS step units (job slot, first bead)
+    int2* d_units_ = nullptr;  // row-sort units (job slot, first row) per kLdsUnit rows of LDS-step jobs
     int cap_units_ = 0, nunits_ = 0, lds_beads_ = 0, any_global_ = 1;
-    int2* d_units1_ = nullptr;  // units of the single-job path
-    int cap_units1_ = 0;  // bitmap words of the largest job with a cell grid (0: none or disabled)
+    int2* d_units1_ = nullptr;  // row-sort units of the single-job path
+    int cap_units1_ = 0;
+    // LDS step plan: units (first segment, segments) over segments (job
+    // slot, first row, rows, LDS offset), for all jobs and for one job
+    int2* d_sunits_ = nullptr;
+    int4* d_segs_ = nullptr;
+    int cap_sunits_ = 0, cap_segs_ = 0, nsunits_ = 0;
+    int2* d_sunits1_ = nullptr;
+    int4* d_segs1_ = nullptr;
+    int cap_sunits1_ = 0, cap_segs1_ = 0, nsunits1_ = 0;  // bitmap words of the largest job with a cell grid (0: none or disabled)
     bool uses_cells(size_t n) const;
     int cell_words_of(const DevJob& J) const;
     int exact_mode_ = SOP_CHOL_BF16X3;

@@ -8,11 +8,13 @@ namespace sop {
 
 // One Langevin step (local step = state.step + kstep) of every job that owns
 // a tile.  max_blocks caps the persistent grid (0 = one block per 4 tiles).
-// Jobs flagged lds_step are stepped by the LDS kernel over `units` (job
-// slot, first bead; lds_beads = the largest such job), the others by the
+// Jobs flagged lds_step are stepped by the LDS kernel over `units` (first
+// segment, segment count) of `segs` (job slot, first row, rows, LDS offset
+// in float4; lds_beads = the largest LDS use of a unit), the others by the
 // tile kernel (launched when any_global is set).
 hipError_t launch_step(const DevJob* jobs, const DevJobState* st, const int2* tiles, int ntiles, int any_global,
-                       const int2* units, int nunits, int lds_beads, int kstep, int max_blocks, hipStream_t stream);
+                       const int2* units, const int4* segs, int nunits, int lds_beads, int kstep, int max_blocks,
+                       hipStream_t stream);
 // List rebuilds for jobs whose cadence is due (force != 0: rebuild all).
 // cell_words: bitmap words (ceil(n/64)) of the largest job with a cell grid,
 // 0 to build every job's possible list by the all-pairs scan.

@@ -279,25 +279,44 @@ __global__ __launch_bounds__(256) void k_step(const DevJob* __restrict__ jobs, c
     }
 }
 
-// The same step for jobs small enough to keep their coordinates in LDS:
-// one 1024-thread workgroup per unit of up to kLdsUnit beads of one job; the
-// workgroup stages all of the job's coordinates (coalesced) and every
-// neighbour gather of the step reads LDS.
+// The same step for jobs small enough to keep their coordinates in LDS.
+// One 1024-thread workgroup per unit; a unit is up to 1024 Verlet rows in
+// segments (job, first row, rows, LDS offset): a job's whole 1024-row
+// blocks are units of their own, and the rows left over share units with
+// other jobs' left-over rows.  The workgroup stages each segment's job
+// coordinates in LDS (coalesced), so every neighbour gather of the step
+// reads LDS.  Segments start at multiples of 64 threads: no wavefront spans
+// two jobs.
 __global__ __launch_bounds__(1024) void k_step_lds(const DevJob* __restrict__ jobs, const DevJobState* __restrict__ st,
-                                                   const int2* __restrict__ units, int nunits, int kstep) {
+                                                   const int2* __restrict__ units, const int4* __restrict__ segs,
+                                                   int nunits, int kstep) {
     extern __shared__ float4 lds_coord[];
     for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
-        const int2 ud = units[u];
-        const DevJob& J = jobs[ud.x];
-        const long long s = st[ud.x].step + kstep;
-        if (s >= J.end_step) continue;
-        const int n = J.n;
-        const float4* __restrict__ g = J.coord[s & 1];
-        __syncthreads();  // the previous unit's gathers are done
-        for (int k = threadIdx.x; k < n; k += blockDim.x) lds_coord[k] = g[k];
+        const int2 ud = units[u];  // first segment, segments
+        __syncthreads();           // the previous unit's gathers are done
+        for (int q = 0; q < ud.y; ++q) {
+            const int4 sg = segs[ud.x + q];
+            const DevJob& J = jobs[sg.x];
+            const long long s = st[sg.x].step + kstep;
+            if (s >= J.end_step) continue;
+            const float4* __restrict__ g = J.coord[s & 1];
+            for (int k = threadIdx.x; k < J.n; k += blockDim.x) lds_coord[sg.w + k] = g[k];
+        }
         __syncthreads();
-        const int r = ud.y + threadIdx.x;  // Verlet row; the thread steps bead perm[r]
-        if (r < n && r < ud.y + kLdsUnit) step_bead(J, st[ud.x], s, J.perm[r], r, n, lds_coord, J.coord[(s + 1) & 1]);
+        int t = threadIdx.x;
+        for (int q = 0; q < ud.y; ++q) {
+            const int4 sg = segs[ud.x + q];
+            const int span = (sg.z + 63) & ~63;
+            if (t < span) {
+                const DevJob& J = jobs[sg.x];
+                const long long s = st[sg.x].step + kstep;
+                const int r = sg.y + t;  // Verlet row; the thread steps bead perm[r]
+                if (t < sg.z && s < J.end_step)
+                    step_bead(J, st[sg.x], s, J.perm[r], r, J.n, lds_coord + sg.w, J.coord[(s + 1) & 1]);
+                break;
+            }
+            t -= span;
+        }
     }
 }
 
@@ -1082,14 +1101,15 @@ static int grid_for(int ntiles, int max_blocks) {
 }
 
 hipError_t launch_step(const DevJob* jobs, const DevJobState* st, const int2* tiles, int ntiles, int any_global,
-                       const int2* units, int nunits, int lds_beads, int kstep, int max_blocks, hipStream_t stream) {
+                       const int2* units, const int4* segs, int nunits, int lds_beads, int kstep, int max_blocks,
+                       hipStream_t stream) {
     if (ntiles <= 0) return hipSuccess;
     if (any_global)
         hipLaunchKernelGGL(k_step, dim3(grid_for(ntiles, max_blocks)), dim3(256), 0, stream, jobs, st, tiles, ntiles,
                            kstep);
     if (nunits > 0)
         hipLaunchKernelGGL(k_step_lds, dim3(nunits), dim3(kLdsUnit), (size_t)lds_beads * sizeof(float4), stream, jobs,
-                           st, units, nunits, kstep);
+                           st, units, segs, nunits, kstep);
     return hipGetLastError();
 }
 
