@@ -711,43 +711,75 @@ __device__ __forceinline__ void verlet_tile(const DevJob& J, DevJobState& S, con
     IdxT* __restrict__ pj = static_cast<IdxT*>(J.pair_j);
     const int bl = lane < nrow ? J.perm[i0 + lane] : 0;    // beads of the tile's rows
     const int cl = lane < nrow ? J.pp_cnt[bl] : 0;         // their possible counts
+    // One row's chunk of candidates (64 CH entries from `base`): compare and
+    // compact the accepted ones into the row's LDS entries (or their slots).
+    auto filter = [&](int b, const float4 r1, int npp, int base, const int (&j)[CH], const float4 (&c)[CH],
+                      int& cnt) {
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+            if (base + 64 * u >= npp) break;
+            bool ok = false;
+            if (j[u] >= 0) {
+                const float x = c[u].x - r1.x, y = c[u].y - r1.y, z = c[u].z - r1.z;
+                ok = x * x + y * y + z * z <= d2max;
+            }
+            const unsigned long long m = __ballot(ok);
+            if (ok) {
+                const int pos = cnt + __popcll(m & lanes_below(lane));
+                if (pos < E) rows[b * RS + pos] = (IdxT)j[u];
+                else if (pos < cap) pj[(size_t)pos * n + i0 + b] = (IdxT)j[u];
+            }
+            cnt += __popcll(m);
+        }
+    };
+    auto load_idx = [&](int b, int base, int (&j)[CH]) {
+        const int npp = __shfl(cl, b);
+        const IdxT* __restrict__ row = pp + (size_t)__shfl(bl, b) * ppcap;
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+            const int k = base + 64 * u + lane;
+            j[u] = k < npp ? (int)row[k] : -1;
+        }
+    };
+    // rows in a software pipeline: the next row's first chunk of indices
+    // (and its centre) load while this row's gathers and compares run
+    int j[CH];
+    float4 r1 = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (b_lo < b_hi) {
+        load_idx(b_lo, 0, j);
+        r1 = cur[__shfl(bl, b_lo)];
+    }
     for (int b = b_lo; b < b_hi; ++b) {
         const int npp = __shfl(cl, b);
-        const int bead = __shfl(bl, b);
-        const float4 r1 = cur[bead];
-        const IdxT* __restrict__ row = pp + (size_t)bead * ppcap;
+        float4 c[CH];
+#pragma unroll
+        for (int u = 0; u < CH; ++u) c[u] = j[u] >= 0 ? cur[j[u]] : make_float4(0.f, 0.f, 0.f, 0.f);
+        int jn[CH];
+        float4 r1n = r1;
+        if (b + 1 < b_hi) {
+            load_idx(b + 1, 0, jn);
+            r1n = cur[__shfl(bl, b + 1)];
+        } else {
+#pragma unroll
+            for (int u = 0; u < CH; ++u) jn[u] = -1;
+        }
         int cnt = 0;
-        for (int base = 0; base < npp; base += 64 * CH) {
-            int j[CH];
+        filter(b, r1, npp, 0, j, c, cnt);
+        for (int base = 64 * CH; base < npp; base += 64 * CH) {  // rows longer than one chunk
+            int j2[CH];
+            load_idx(b, base, j2);
+            float4 c2[CH];
 #pragma unroll
-            for (int u = 0; u < CH; ++u) {
-                const int k = base + 64 * u + lane;
-                j[u] = k < npp ? (int)row[k] : -1;
-            }
-            float4 c[CH];
-#pragma unroll
-            for (int u = 0; u < CH; ++u) c[u] = j[u] >= 0 ? cur[j[u]] : make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-            for (int u = 0; u < CH; ++u) {
-                if (base + 64 * u >= npp) break;
-                bool ok = false;
-                if (j[u] >= 0) {
-                    const float x = c[u].x - r1.x, y = c[u].y - r1.y, z = c[u].z - r1.z;
-                    ok = x * x + y * y + z * z <= d2max;
-                }
-                const unsigned long long m = __ballot(ok);
-                if (ok) {
-                    const int pos = cnt + __popcll(m & lanes_below(lane));
-                    if (pos < E) rows[b * RS + pos] = (IdxT)j[u];
-                    else if (pos < cap) pj[(size_t)pos * n + i0 + b] = (IdxT)j[u];
-                }
-                cnt += __popcll(m);
-            }
+            for (int u = 0; u < CH; ++u) c2[u] = j2[u] >= 0 ? cur[j2[u]] : make_float4(0.f, 0.f, 0.f, 0.f);
+            filter(b, r1, npp, base, j2, c2, cnt);
         }
         if (lane == 0) {
             cnt_s[b] = cnt;
             if (cnt > cap) atomicOr(&S.overflow, 2);
         }
+#pragma unroll
+        for (int u = 0; u < CH; ++u) j[u] = jn[u];
+        r1 = r1n;
     }
 }
 
