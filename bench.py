@@ -403,11 +403,13 @@ def main():
     bpb = bytes_per_bead_step(avg_lists)
     achieved = bpb * nbeads / (avg_ms * 1e-3)
     traffic = None
+    kernel = "sop::k_step_lds"  # every job of the mix is at most 5120 beads: the LDS step kernel
     tfile = os.path.join(ROOT, "profiles", "step_kernel_traffic.json")
     if os.path.exists(tfile):
         try:
-            traffic = json.load(open(tfile)).get("bytes_per_launch_per_bead", None)
-            traffic = None if traffic is None else traffic * nbeads
+            tj = json.load(open(tfile))
+            if tj.get("kernel") == kernel and tj.get("bytes_per_launch_per_bead") is not None:
+                traffic = tj["bytes_per_launch_per_bead"] * nbeads
         except (ValueError, OSError):
             traffic = None
     eng.close()
@@ -479,7 +481,7 @@ def main():
         "plain": {"value": plain_value, "jobs_sampled": int(len(pick)), "steps_per_job": args.plain_steps,
                   "batched_over_plain": value / plain_value},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": traffic, "kernel": "sop::k_step",
+                     "frac": achieved / HBM_PEAK, "traffic": traffic, "kernel": kernel,
                      "avg_launch_ms": avg_ms, "launches": launches, "bytes_per_bead_step": bpb,
                      "copy_kernel_gbps": copy_gbps, "frac_of_copy": achieved / (copy_gbps * 1e9),
                      "avg_lists": {"covalent": avg_lists[0], "native": avg_lists[1], "verlet": avg_lists[2],
