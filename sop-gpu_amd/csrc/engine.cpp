@@ -916,8 +916,11 @@ int Engine::launch_block(int slot_only) {
     if (int rc = check(launch_possible(jobs, st, tiles, nt, nj, cw, 0, stream_), "possible pairs")) return rc;
     const int i32 = slot_only >= 0 ? !jobs_[slot_only].dj.idx16 : any_idx32_;
     const int nu1 = slot_only >= 0 && jobs_[slot_only].dj.sort_rows ? (jobs_[slot_only].n + kLdsUnit - 1) / kLdsUnit : 0;
-    if (int rc = check(slot_only >= 0 ? launch_verlet(jobs, st, tiles, nt, i32, d_units1_, nu1, 0, stream_)
-                                      : launch_verlet(jobs, st, tiles, nt, i32, d_units_, nunits_, 0, stream_),
+    if (int rc = check(slot_only >= 0
+                           ? launch_verlet(jobs, st, tiles, nt, i32, d_units1_, nu1, d_sunits1_, d_segs1_,
+                                           jobs_[slot_only].dj.lds_step ? nsunits1_ : 0, jobs_[slot_only].n, 0, stream_)
+                           : launch_verlet(jobs, st, tiles, nt, i32, d_units_, nunits_, d_sunits_, d_segs_, nsunits_,
+                                           lds_beads_, 0, stream_),
                        "verlet"))
         return rc;
     const bool tea = slot_only >= 0 ? ((jobs_[slot_only].dj.flags & JF_HI) && !(jobs_[slot_only].dj.flags & JF_HI_EXACT))
@@ -1034,7 +1037,8 @@ int Engine::stage_possible() {
 }
 int Engine::stage_verlet() {
     if (int rc = sync_tables()) return rc;
-    return check(launch_verlet(d_jobs_, d_state_, d_tiles_, ntiles_, any_idx32_, d_units_, nunits_, 0, stream_),
+    return check(launch_verlet(d_jobs_, d_state_, d_tiles_, ntiles_, any_idx32_, d_units_, nunits_, d_sunits_, d_segs_,
+                               nsunits_, lds_beads_, 0, stream_),
                  "verlet");
 }
 int Engine::stage_tea_update() {
@@ -1341,7 +1345,8 @@ int Engine::rebuild_lists(int id, int mask) {
         if (int rc = check(launch_verlet(d_jobs_ + slot, d_state_ + slot, d_tiles1_, nt, !jobs_[slot].dj.idx16,
                                          d_units1_,
                                          jobs_[slot].dj.sort_rows ? (jobs_[slot].n + kLdsUnit - 1) / kLdsUnit : 0,
-                                         1, stream_),
+                                         d_sunits1_, d_segs1_, jobs_[slot].dj.lds_step ? nsunits1_ : 0,
+                                         jobs_[slot].n, 1, stream_),
                            "verlet"))
             return rc;
     return synchronize();

@@ -23,9 +23,13 @@ hipError_t launch_possible(const DevJob* jobs, DevJobState* st, const int2* tile
 // Largest cell_words launch_possible accepts (the LDS bitmap of one wavefront).
 constexpr int kMaxCellWords = 8192;
 // any_idx32: some job of the launch stores its lists as int32 (n >= 65536).
-// units (job, first row) of the jobs whose rows are re-sorted first.
+// units (job, first row) of the jobs whose rows are re-sorted first;
+// sunits/segs/nsunits/lds_beads: the LDS step plan (see launch_step), whose
+// jobs get their lists built from LDS-staged coordinates (nsunits = 0: all
+// jobs by the tile kernel).
 hipError_t launch_verlet(const DevJob* jobs, DevJobState* st, const int2* tiles, int ntiles, int any_idx32,
-                         const int2* units, int nunits, int force, hipStream_t stream);
+                         const int2* units, int nunits, const int2* sunits, const int4* segs, int nsunits,
+                         int lds_beads, int force, hipStream_t stream);
 // Per-job updaters at a block start, and the step counter advance.
 hipError_t launch_job_update(const DevJob* jobs, DevJobState* st, int njobs, hipStream_t stream, int mask = 7);
 hipError_t launch_job_advance(const DevJob* jobs, DevJobState* st, int njobs, int nsteps, hipStream_t stream);

@@ -696,15 +696,17 @@ struct VerletLds<int> {
     static constexpr int entries = 255, stride = 256;
 };
 
-template <typename IdxT>
+// w: this wavefront's index among the WAVES wavefronts that build the tile.
+template <typename IdxT, int WAVES>
 __device__ __forceinline__ void verlet_tile(const DevJob& J, DevJobState& S, const float4* __restrict__ cur, int i0,
-                                            IdxT* __restrict__ rows, int* __restrict__ cnt_s) {
+                                            IdxT* __restrict__ rows, int* __restrict__ cnt_s, int w) {
     constexpr int E = VerletLds<IdxT>::entries, RS = VerletLds<IdxT>::stride;
     constexpr int CH = 8;  // chunks of 64 entries whose loads are in flight together
-    const int lane = threadIdx.x & 63, w = wave_id();
+    constexpr int RPW = 64 / WAVES;  // rows per wave
+    const int lane = threadIdx.x & 63;
     const int n = J.n;
     const int nrow = min(64, n - i0);
-    const int b_lo = 16 * w, b_hi = min(b_lo + 16, nrow);
+    const int b_lo = RPW * w, b_hi = min(b_lo + RPW, nrow);
     const float d2max = J.pl_d2;  // sqrtf(d2) < pl_cut, exactly
     const int cap = J.pair_cap, ppcap = J.pp_cap;
     const IdxT* __restrict__ pp = static_cast<const IdxT*>(J.pp_j);
@@ -741,13 +743,20 @@ __device__ __forceinline__ void verlet_tile(const DevJob& J, DevJobState& S, con
             j[u] = k < npp ? (int)row[k] : -1;
         }
     };
-    // rows in a software pipeline: the next row's first chunk of indices
-    // (and its centre) load while this row's gathers and compares run
-    int j[CH];
-    float4 r1 = make_float4(0.f, 0.f, 0.f, 0.f);
+    // rows in a software pipeline two rows deep: the first chunks of
+    // indices of rows b+1 and b+2 (and their centres) are in flight while
+    // row b's gathers and compares run
+    int j[CH], ja[CH];
+    float4 r1 = make_float4(0.f, 0.f, 0.f, 0.f), r1a = r1;
+#pragma unroll
+    for (int u = 0; u < CH; ++u) j[u] = ja[u] = -1;
     if (b_lo < b_hi) {
         load_idx(b_lo, 0, j);
         r1 = cur[__shfl(bl, b_lo)];
+    }
+    if (b_lo + 1 < b_hi) {
+        load_idx(b_lo + 1, 0, ja);
+        r1a = cur[__shfl(bl, b_lo + 1)];
     }
     for (int b = b_lo; b < b_hi; ++b) {
         const int npp = __shfl(cl, b);
@@ -756,9 +765,9 @@ __device__ __forceinline__ void verlet_tile(const DevJob& J, DevJobState& S, con
         for (int u = 0; u < CH; ++u) c[u] = j[u] >= 0 ? cur[j[u]] : make_float4(0.f, 0.f, 0.f, 0.f);
         int jn[CH];
         float4 r1n = r1;
-        if (b + 1 < b_hi) {
-            load_idx(b + 1, 0, jn);
-            r1n = cur[__shfl(bl, b + 1)];
+        if (b + 2 < b_hi) {
+            load_idx(b + 2, 0, jn);
+            r1n = cur[__shfl(bl, b + 2)];
         } else {
 #pragma unroll
             for (int u = 0; u < CH; ++u) jn[u] = -1;
@@ -778,18 +787,23 @@ __device__ __forceinline__ void verlet_tile(const DevJob& J, DevJobState& S, con
             if (cnt > cap) atomicOr(&S.overflow, 2);
         }
 #pragma unroll
-        for (int u = 0; u < CH; ++u) j[u] = jn[u];
-        r1 = r1n;
+        for (int u = 0; u < CH; ++u) {
+            j[u] = ja[u];
+            ja[u] = jn[u];
+        }
+        r1 = r1a;
+        r1a = r1n;
     }
 }
 
-template <typename IdxT>
+// t: this thread's index among the 64 WAVES threads that store the tile.
+template <typename IdxT, int WAVES>
 __device__ __forceinline__ void verlet_store(const DevJob& J, int i0, const IdxT* __restrict__ rows,
-                                             const int* __restrict__ cnt_s) {
+                                             const int* __restrict__ cnt_s, int t) {
     constexpr int E = VerletLds<IdxT>::entries, RS = VerletLds<IdxT>::stride;
     const int n = J.n, cap = J.pair_cap;
     const int nrow = min(64, n - i0);
-    const int b = threadIdx.x & 63, kq = threadIdx.x >> 6;
+    const int b = t & 63, kq = t >> 6;
     const int c = b < nrow ? cnt_s[b] : 0;
     if (b < nrow) {
         if (kq == 0) J.pair_cnt[i0 + b] = c < cap ? c : cap;
@@ -801,7 +815,7 @@ __device__ __forceinline__ void verlet_store(const DevJob& J, int i0, const IdxT
     // see all 64 beads here (b = lane), so maxc is the tile's
     IdxT* __restrict__ pj = static_cast<IdxT*>(J.pair_j);
     const int lim = min(c, E);
-    for (int k = kq; k < maxc; k += 4)
+    for (int k = kq; k < maxc; k += WAVES)
         if (k < lim) pj[(size_t)k * n + i0 + b] = rows[b * RS + k];
 }
 
@@ -843,7 +857,7 @@ __global__ __launch_bounds__(1024) void k_sort_rows(const DevJob* __restrict__ j
 }
 
 __global__ __launch_bounds__(256) void k_verlet(const DevJob* __restrict__ jobs, DevJobState* __restrict__ st,
-                                                const int2* __restrict__ tiles, int ntiles, int force) {
+                                                const int2* __restrict__ tiles, int ntiles, int force, int skip_lds) {
     extern __shared__ __attribute__((aligned(16))) int lds_rows[];  // 64 rows of uint16 or int32 entries
     __shared__ int cnt_s[64];
     const int nb = gridDim.x, nxcd = nb >= 8 ? 8 : 1;
@@ -856,17 +870,64 @@ __global__ __launch_bounds__(256) void k_verlet(const DevJob* __restrict__ jobs,
         DevJobState& S = st[td.x];
         const long long s = S.step;
         if (!force && !due(J, s, J.pairs_freq)) continue;
+        if (skip_lds && J.lds_step) continue;  // built by k_verlet_lds
         const float4* __restrict__ cur = J.coord[s & 1];
         if (J.idx16)
-            verlet_tile(J, S, cur, td.y, reinterpret_cast<unsigned short*>(lds_rows), cnt_s);
+            verlet_tile<unsigned short, 4>(J, S, cur, td.y, reinterpret_cast<unsigned short*>(lds_rows), cnt_s,
+                                           wave_id());
         else
-            verlet_tile(J, S, cur, td.y, lds_rows, cnt_s);
+            verlet_tile<int, 4>(J, S, cur, td.y, lds_rows, cnt_s, wave_id());
         __syncthreads();
         if (J.idx16)
-            verlet_store(J, td.y, reinterpret_cast<const unsigned short*>(lds_rows), cnt_s);
+            verlet_store<unsigned short, 4>(J, td.y, reinterpret_cast<const unsigned short*>(lds_rows), cnt_s,
+                                            threadIdx.x);
         else
-            verlet_store(J, td.y, static_cast<const int*>(lds_rows), cnt_s);
+            verlet_store<int, 4>(J, td.y, static_cast<const int*>(lds_rows), cnt_s, threadIdx.x);
         __syncthreads();
+    }
+}
+
+// The Verlet build of the LDS-step jobs, over the step kernel's units: the
+// workgroup stages the coordinates of each job it serves in LDS, so the
+// candidate gathers read LDS instead of L2, then builds the unit's rows two
+// 64-row tiles at a time (wavefronts 0-7 one tile, 8-15 the other, eight
+// rows each) and writes each tile slot-major as k_verlet does.  Same
+// lists, same order.
+constexpr int kVerletRowsBytes = 64 * VerletLds<unsigned short>::stride * 2;
+__global__ __launch_bounds__(1024) void k_verlet_lds(const DevJob* __restrict__ jobs, DevJobState* __restrict__ st,
+                                                     const int2* __restrict__ units, const int4* __restrict__ segs,
+                                                     int nunits, int force) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+    float4* coords = reinterpret_cast<float4*>(lds_raw + 2 * kVerletRowsBytes);
+    __shared__ int cnt_s[2][64];
+    const int grp = threadIdx.x >> 9, gt = threadIdx.x & 511;  // tile group, thread in group
+    unsigned short* rows = reinterpret_cast<unsigned short*>(lds_raw + grp * kVerletRowsBytes);
+    for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
+        const int2 ud = units[u];
+        __syncthreads();  // the previous unit's reads are done
+        for (int q = 0; q < ud.y; ++q) {
+            const int4 sg = segs[ud.x + q];
+            const DevJob& J = jobs[sg.x];
+            const long long s = st[sg.x].step;
+            if (!force && !due(J, s, J.pairs_freq)) continue;
+            const float4* __restrict__ g = J.coord[s & 1];
+            for (int k = threadIdx.x; k < J.n; k += blockDim.x) coords[sg.w + k] = g[k];
+        }
+        __syncthreads();
+        for (int q = 0; q < ud.y; ++q) {
+            const int4 sg = segs[ud.x + q];
+            const DevJob& J = jobs[sg.x];
+            DevJobState& S = st[sg.x];
+            if (!force && !due(J, S.step, J.pairs_freq)) continue;
+            for (int t0 = 0; t0 < sg.z; t0 += 128) {
+                const int i0 = sg.y + t0 + 64 * grp;
+                const bool mine = t0 + 64 * grp < sg.z;
+                if (mine) verlet_tile<unsigned short, 8>(J, S, coords + sg.w, i0, rows, cnt_s[grp], gt >> 6);
+                __syncthreads();
+                if (mine) verlet_store<unsigned short, 8>(J, i0, rows, cnt_s[grp], gt);
+                __syncthreads();
+            }
+        }
     }
 }
 
@@ -1167,14 +1228,18 @@ hipError_t launch_possible(const DevJob* jobs, DevJobState* st, const int2* tile
 }
 
 hipError_t launch_verlet(const DevJob* jobs, DevJobState* st, const int2* tiles, int ntiles, int any_idx32,
-                         const int2* units, int nunits, int force, hipStream_t stream) {
+                         const int2* units, int nunits, const int2* sunits, const int4* segs, int nsunits,
+                         int lds_beads, int force, hipStream_t stream) {
     if (ntiles <= 0) return hipSuccess;
     if (nunits > 0) hipLaunchKernelGGL(k_sort_rows, dim3(nunits), dim3(kLdsUnit), 0, stream, jobs, st, units, nunits, force);
     int blocks = ntiles;
     if (blocks >= 8) blocks = (blocks + 7) / 8 * 8;
     const size_t lds = any_idx32 ? 64 * VerletLds<int>::stride * sizeof(int)
                                  : 64 * VerletLds<unsigned short>::stride * sizeof(unsigned short);
-    hipLaunchKernelGGL(k_verlet, dim3(blocks), dim3(256), lds, stream, jobs, st, tiles, ntiles, force);
+    hipLaunchKernelGGL(k_verlet, dim3(blocks), dim3(256), lds, stream, jobs, st, tiles, ntiles, force, nsunits > 0);
+    if (nsunits > 0)
+        hipLaunchKernelGGL(k_verlet_lds, dim3(nsunits), dim3(1024), 2 * (size_t)kVerletRowsBytes + (size_t)lds_beads * 16,
+                           stream, jobs, st, sunits, segs, nsunits, force);
     return hipGetLastError();
 }
 
