@@ -255,55 +255,98 @@ __global__ __launch_bounds__(256) void k_hi_tea(const DevJob* __restrict__ jobs,
 // ---------------------------------------------------------------------------
 // exact path
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ bool refactor_due(const DevJob& J, long long s) {
-    return s < J.end_step && (s % J.hi_chol_freq) == 0;
+
+// F_i + sum_j D_ij F_j into fdf, summed in bead order.  One 1024-thread
+// workgroup per 64-bead tile of the (single) exact-HI job of the launch:
+// for each chunk of 64 beads j the sixteen wavefronts form the 64x64 terms
+// D_ij F_j (the three row sums, written exactly as the sequential loop
+// writes them) into LDS, then wavefront 0 adds them to bead i's sums in j
+// order, so every sum is the same bits as the bead-by-bead loop.
+__global__ __launch_bounds__(1024) void k_chol_D(const DevJob* __restrict__ jobs, const DevJobState* __restrict__ st,
+                                                 const int2* __restrict__ tiles, int ntiles, int kstep) {
+    __shared__ float term[3][64][65];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int2 td = tiles[blockIdx.x];
+    const DevJob& J = jobs[td.x];
+    if (!(J.flags & JF_HI_EXACT)) return;  // one job per launch: uniform
+    const long long s = st[td.x].step + kstep;
+    if (s >= J.end_step) return;
+    const int n = J.n;
+    const int d = td.y + lane;  // this lane's bead i (the sums' owner in wavefront 0)
+    const float4* __restrict__ cur = J.coord[s & 1];
+    const float4 c = d < n ? cur[d] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float a = J.hi_a;
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    if (w == 0 && d < n) {
+        const float4 f0 = J.mforce[d];
+        fx = f0.x, fy = f0.y, fz = f0.z;
+    }
+    for (int base = 0; base < n; base += 64) {
+        for (int jl = w; jl < 64; jl += 16) {
+            const int j = base + jl;
+            float tx = 0.f, ty = 0.f, tz = 0.f;
+            if (j < n && j != d && d < n) {
+                const float4 dr = unit_sep(c, cur[j]);
+                const F6 D = rpy(dr, a);
+                const float4 g = J.mforce[j];
+                tx = D.xx * g.x + D.xy * g.y + D.xz * g.z;
+                ty = D.xy * g.x + D.yy * g.y + D.yz * g.z;
+                tz = D.xz * g.x + D.yz * g.y + D.zz * g.z;
+            }
+            term[0][lane][jl] = tx;
+            term[1][lane][jl] = ty;
+            term[2][lane][jl] = tz;
+        }
+        __syncthreads();
+        if (w == 0 && d < n) {
+            const int m = min(64, n - base);
+            for (int jl = 0; jl < m; ++jl) {
+                if (base + jl == d) continue;
+                fx += term[0][lane][jl];
+                fy += term[1][lane][jl];
+                fz += term[2][lane][jl];
+            }
+        }
+        __syncthreads();
+    }
+    if (w == 0 && d < n) J.fdf[d] = make_float4(fx, fy, fz, 0.0f);
 }
 
-// D/D_ii lower block triangle into M (refactorisation steps) and
-// F_i + sum_j D_ij F_j into fdf.
-__global__ __launch_bounds__(256) void k_chol_D(const DevJob* __restrict__ jobs, const DevJobState* __restrict__ st,
-                                                const int2* __restrict__ tiles, int ntiles, int kstep) {
-    FOR_TILES(ntiles) {
-        const int2 td = tiles[t];
-        const DevJob& J = jobs[td.x];
-        if (!(J.flags & JF_HI_EXACT)) continue;
-        const long long s = st[td.x].step + kstep;
-        if (s >= J.end_step) continue;
-        const int d = td.y + lane;
-        const int n = J.n;
-        if (d >= n) continue;
-        const bool build = refactor_due(J, s);
-        const float4* __restrict__ cur = J.coord[s & 1];
-        const size_t ld = J.chol_ld;
-        float* __restrict__ M = J.chol_M;
-        const float4 c = cur[d];
-        const float4 f0 = J.mforce[d];
-        float fx = f0.x, fy = f0.y, fz = f0.z;
-        const float a = J.hi_a;
-        for (int i = 0; i < n; ++i) {
-            const size_t r = 3 * (size_t)i, col = 3 * (size_t)d;
-            if (i == d) {
-                if (build)
-                    for (int p = 0; p < 3; ++p)
-                        for (int q = 0; q < 3; ++q) M[(r + p) * ld + col + q] = p == q ? 1.f : 0.f;
-                continue;
-            }
-            const float4 dr = unit_sep(c, cur[i]);
-            const F6 D = rpy(dr, a);
-            if (build && i > d) {
-                float* m0 = M + r * ld + col;
-                m0[0] = D.xx; m0[1] = D.xy; m0[2] = D.xz;
-                m0 += ld;
-                m0[0] = D.xy; m0[1] = D.yy; m0[2] = D.yz;
-                m0 += ld;
-                m0[0] = D.xz; m0[1] = D.yz; m0[2] = D.zz;
-            }
-            const float4 g = J.mforce[i];
-            fx += D.xx * g.x + D.xy * g.y + D.xz * g.z;
-            fy += D.xy * g.x + D.yy * g.y + D.yz * g.z;
-            fz += D.xz * g.x + D.yz * g.y + D.zz * g.z;
+// D/D_ii lower block triangle into M on refactorisation steps: block
+// (blockIdx.x, blockIdx.y) of 64x64 bead pairs (i in tile x, d in tile y,
+// d <= i), identity 3x3 blocks on the diagonal; the same RPY blocks k_chol_D
+// sums, written row by row (consecutive threads: consecutive d).
+__global__ __launch_bounds__(256) void k_chol_build(const DevJob* __restrict__ jobs, const DevJobState* __restrict__ st,
+                                                    const int2* __restrict__ tiles, int kstep) {
+    if (blockIdx.y > blockIdx.x) return;
+    const int2 ti = tiles[blockIdx.x], tdd = tiles[blockIdx.y];
+    const DevJob& J = jobs[ti.x];
+    const long long s = st[ti.x].step + kstep;
+    const int n = J.n;
+    const float4* __restrict__ cur = J.coord[s & 1];
+    const size_t ld = J.chol_ld;
+    float* __restrict__ M = J.chol_M;
+    const float a = J.hi_a;
+    const int dl = threadIdx.x & 63;
+    const int d = tdd.y + dl;
+    if (d >= n) return;
+    const float4 c = cur[d];
+    for (int il = threadIdx.x >> 6; il < 64; il += 4) {
+        const int i = ti.y + il;
+        if (i >= n || i < d) continue;
+        const size_t r = 3 * (size_t)i, col = 3 * (size_t)d;
+        float* m0 = M + r * ld + col;
+        if (i == d) {
+            for (int p = 0; p < 3; ++p)
+                for (int q = 0; q < 3; ++q) m0[p * ld + q] = p == q ? 1.f : 0.f;
+            continue;
         }
-        J.fdf[d] = make_float4(fx, fy, fz, 0.0f);
+        const F6 D = rpy(unit_sep(c, cur[i]), a);
+        m0[0] = D.xx; m0[1] = D.xy; m0[2] = D.xz;
+        m0 += ld;
+        m0[0] = D.xy; m0[1] = D.yy; m0[2] = D.yz;
+        m0 += ld;
+        m0[0] = D.xz; m0[1] = D.yz; m0[2] = D.zz;
     }
 }
 
@@ -314,54 +357,53 @@ __global__ __launch_bounds__(256) void k_chol_D(const DevJob* __restrict__ jobs,
 __global__ __launch_bounds__(256) void k_chol_L(const DevJob* __restrict__ jobs, const DevJobState* __restrict__ st,
                                                 const int2* __restrict__ tiles, int ntiles, int kstep) {
     const int lane = threadIdx.x & 63;
-    // one wavefront per bead: bead index = tile*64 + r for r in 0..63
-    for (int t = blockIdx.x * 4 + wave_of(); t < ntiles; t += gridDim.x * 4) {
-        const int2 td = tiles[t];
+    // wavefront gw: bead (gw % 64) of tile gw / 64
+    for (int gw = blockIdx.x * 4 + wave_of(); gw < ntiles * 64; gw += gridDim.x * 4) {
+        const int2 td = tiles[gw >> 6];
         const DevJob& J = jobs[td.x];
         if (!(J.flags & JF_HI_EXACT)) continue;
         const long long s = st[td.x].step + kstep;
         if (s >= J.end_step) continue;
         const int n = J.n;
+        const int d = td.y + (gw & 63);
+        if (d >= n) continue;
         const size_t ld = J.chol_ld;
         const float* __restrict__ M = J.chol_M;
         const float* __restrict__ g = reinterpret_cast<const float*>(J.rforce);
         const bool faithful = (J.flags & JF_HI_FAITHFUL) != 0;
-        const int dend = min(td.y + 64, n);
-        for (int d = td.y; d < dend; ++d) {
-            const int r = 3 * d;
-            float a0 = 0.f, a1 = 0.f, a2 = 0.f;
-            for (int c = lane; c < r + 3; c += 64) {
-                const int blk = c / 3, b = c - 3 * blk;
-                const float gc = g[4 * blk + b];
-                float b0, b1, b2;
-                if (faithful) {
-                    const int c0 = 3 * blk;
-                    // B[a][c0+b] = L[r+max(a,b)][c0+min(a,b)]
+        const int r = 3 * d;
+        float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+        for (int c = lane; c < r + 3; c += 64) {
+            const int blk = c / 3, b = c - 3 * blk;
+            const float gc = g[4 * blk + b];
+            float b0, b1, b2;
+            if (faithful) {
+                const int c0 = 3 * blk;
+                // B[a][c0+b] = L[r+max(a,b)][c0+min(a,b)]
+                b0 = M[(size_t)(r + b) * ld + c0];
+                b1 = M[(size_t)(r + (b > 1 ? b : 1)) * ld + c0 + (b < 1 ? b : 1)];
+                b2 = M[(size_t)(r + 2) * ld + c0 + b];
+                if (blk == d) {  // inside the diagonal block only the lower part exists
                     b0 = M[(size_t)(r + b) * ld + c0];
-                    b1 = M[(size_t)(r + (b > 1 ? b : 1)) * ld + c0 + (b < 1 ? b : 1)];
-                    b2 = M[(size_t)(r + 2) * ld + c0 + b];
-                    if (blk == d) {  // inside the diagonal block only the lower part exists
-                        b0 = M[(size_t)(r + b) * ld + c0];
-                    }
-                } else {
-                    b0 = c <= r ? M[(size_t)r * ld + c] : 0.f;
-                    b1 = c <= r + 1 ? M[(size_t)(r + 1) * ld + c] : 0.f;
-                    b2 = M[(size_t)(r + 2) * ld + c];
                 }
-                a0 += b0 * gc;
-                a1 += b1 * gc;
-                a2 += b2 * gc;
+            } else {
+                b0 = c <= r ? M[(size_t)r * ld + c] : 0.f;
+                b1 = c <= r + 1 ? M[(size_t)(r + 1) * ld + c] : 0.f;
+                b2 = M[(size_t)(r + 2) * ld + c];
             }
-            // fixed-order tree reduction across the wavefront
-            for (int off = 32; off > 0; off >>= 1) {
-                a0 += __shfl_xor(a0, off);
-                a1 += __shfl_xor(a1, off);
-                a2 += __shfl_xor(a2, off);
-            }
-            if (lane == 0) {
-                const float4 f = J.fdf[d];
-                hi_finish(J, d, s, f.x + a0, f.y + a1, f.z + a2, J.coord[s & 1][d]);
-            }
+            a0 += b0 * gc;
+            a1 += b1 * gc;
+            a2 += b2 * gc;
+        }
+        // fixed-order tree reduction across the wavefront
+        for (int off = 32; off > 0; off >>= 1) {
+            a0 += __shfl_xor(a0, off);
+            a1 += __shfl_xor(a1, off);
+            a2 += __shfl_xor(a2, off);
+        }
+        if (lane == 0) {
+            const float4 f = J.fdf[d];
+            hi_finish(J, d, s, f.x + a0, f.y + a1, f.z + a2, J.coord[s & 1][d]);
         }
     }
 }
@@ -1138,11 +1180,13 @@ hipError_t launch_chol_factor(float* M, size_t ld, int mp, int mode, double* lin
 
 hipError_t launch_chol_step(const DevJob* jobs, const DevJobState* st, const int2* tiles, int ntiles, int kstep,
                             const HiExactWork* w, hipStream_t stream) {
-    hipLaunchKernelGGL(k_chol_D, dim3(grid_tiles(ntiles)), dim3(256), 0, stream, jobs, st, tiles, ntiles, kstep);
-    if (w)
+    hipLaunchKernelGGL(k_chol_D, dim3(ntiles), dim3(1024), 0, stream, jobs, st, tiles, ntiles, kstep);
+    if (w) {
+        hipLaunchKernelGGL(k_chol_build, dim3(ntiles, ntiles), dim3(256), 0, stream, jobs, st, tiles, kstep);
         (void)launch_chol_factor(w->M, w->ld, w->mp, w->mode, w->linv, w->phi, w->plo, nullptr, stream, w->aux,
                                  w->ev_f, w->ev_t2);
-    hipLaunchKernelGGL(k_chol_L, dim3(grid_tiles(ntiles)), dim3(256), 0, stream, jobs, st, tiles, ntiles, kstep);
+    }
+    hipLaunchKernelGGL(k_chol_L, dim3(ntiles * 16), dim3(256), 0, stream, jobs, st, tiles, ntiles, kstep);
     return hipGetLastError();
 }
 
