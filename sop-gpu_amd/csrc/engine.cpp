@@ -153,6 +153,7 @@ Engine::~Engine() {
     (void)hipFree(d_units_);
     (void)hipFree(d_units1_);
     (void)hipFree(d_sunits_);
+    (void)hipFree(d_tea_tiles_);
     (void)hipFree(d_segs_);
     (void)hipFree(d_sunits1_);
     (void)hipFree(d_segs1_);
@@ -709,12 +710,18 @@ int Engine::sync_tables() {
     any_tea_ = any_exact_ = false;
     cell_words_ = 0;
     any_idx32_ = 0;
+    std::vector<int2> tea_tiles;  // the tiles of the TEA jobs
     for (auto& j : jobs_) {
         if (!j.dj.idx16) any_idx32_ = 1;
         cell_words_ = std::max(cell_words_, cell_words_of(j.dj));
-        if ((j.dj.flags & JF_HI) && !(j.dj.flags & JF_HI_EXACT)) any_tea_ = true;
+        if ((j.dj.flags & JF_HI) && !(j.dj.flags & JF_HI_EXACT)) {
+            any_tea_ = true;
+            tea_tiles.insert(tea_tiles.end(), tiles.begin() + j.tile0, tiles.begin() + j.tile0 + j.ntile);
+        }
         if (j.dj.flags & JF_HI_EXACT) any_exact_ = true;
     }
+    ntea_tiles_ = (int)tea_tiles.size();
+    if (int rc = check(upload_grow(d_tea_tiles_, cap_tea_tiles_, tea_tiles, stream_), "TEA tiles")) return rc;
     ntiles_ = (int)tiles.size();
     if (ntiles_ > cap_tiles_) {
         int cap = std::max(256, ntiles_ * 2);
@@ -927,7 +934,10 @@ int Engine::launch_block(int slot_only) {
                                     : any_tea_;
     const bool exact = slot_only >= 0 ? (jobs_[slot_only].dj.flags & JF_HI_EXACT) != 0 : any_exact_;
     if (tea)
-        if (int rc = check(launch_hi_tea_update(jobs, st, tiles, nt, nj, stream_), "TEA update")) return rc;
+        if (int rc = check(slot_only >= 0 ? launch_hi_tea_update(jobs, st, tiles, nt, nj, stream_)
+                                          : launch_hi_tea_update(jobs, st, d_tea_tiles_, ntea_tiles_, nj, stream_),
+                           "TEA update"))
+            return rc;
     // profiling: HIP events around the step launches; when nothing else is
     // launched between them, one pair spans the block's L launches
     const bool span = !tea && !exact;
@@ -961,7 +971,10 @@ int Engine::launch_block(int slot_only) {
             ev_used_++;
         }
         if (tea)
-            if (int rc = check(launch_hi_tea_step(jobs, st, tiles, nt, k, stream_), "TEA step")) return rc;
+            if (int rc = check(slot_only >= 0 ? launch_hi_tea_step(jobs, st, tiles, nt, k, stream_)
+                                              : launch_hi_tea_step(jobs, st, d_tea_tiles_, ntea_tiles_, k, stream_),
+                               "TEA step"))
+                return rc;
         if (exact) {
             for (int q = 0; q < (int)jobs_.size(); ++q) {
                 if (slot_only >= 0 && q != slot_only) continue;
@@ -1044,7 +1057,8 @@ int Engine::stage_verlet() {
 int Engine::stage_tea_update() {
     if (int rc = sync_tables()) return rc;
     if (!any_tea_) return 0;
-    return check(launch_hi_tea_update(d_jobs_, d_state_, d_tiles_, ntiles_, (int)jobs_.size(), stream_), "TEA update");
+    return check(launch_hi_tea_update(d_jobs_, d_state_, d_tea_tiles_, ntea_tiles_, (int)jobs_.size(), stream_),
+                 "TEA update");
 }
 int Engine::stage_step(int k) {
     if (int rc = sync_tables()) return rc;
@@ -1053,7 +1067,8 @@ int Engine::stage_step(int k) {
                        "step"))
         return rc;
     if (any_tea_)
-        if (int rc = check(launch_hi_tea_step(d_jobs_, d_state_, d_tiles_, ntiles_, k, stream_), "TEA step")) return rc;
+        if (int rc = check(launch_hi_tea_step(d_jobs_, d_state_, d_tea_tiles_, ntea_tiles_, k, stream_), "TEA step"))
+            return rc;
     if (any_exact_)
         for (auto& hj : jobs_) {
             if (!(hj.dj.flags & JF_HI_EXACT)) continue;

@@ -160,6 +160,8 @@ class Engine {
     bool use_lds_step_ = true;
     int2* d_units_ = nullptr;  // row-sort units (job slot, first row) per kLdsUnit rows of LDS-step jobs
     int cap_units_ = 0, nunits_ = 0, lds_beads_ = 0, any_global_ = 1;
+    int2* d_tea_tiles_ = nullptr;  // tiles of the TEA jobs
+    int cap_tea_tiles_ = 0, ntea_tiles_ = 0;
     int2* d_units1_ = nullptr;  // row-sort units of the single-job path
     int cap_units1_ = 0;
     // LDS step plan: units (first segment, segments) over segments (job

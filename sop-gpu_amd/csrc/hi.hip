@@ -120,29 +120,63 @@ __device__ __forceinline__ void eps_add(const float4 c1, const float4 c2, float 
     sum.w += p.w;
 }
 
-__global__ __launch_bounds__(256) void k_hi_eps(const DevJob* __restrict__ jobs, const DevJobState* __restrict__ st,
-                                                const int2* __restrict__ tiles, int ntiles) {
-    FOR_TILES(ntiles) {
-        const int2 td = tiles[t];
-        const DevJob& J = jobs[td.x];
-        if (!(J.flags & JF_HI) || (J.flags & JF_HI_EXACT)) continue;
-        const long long s = st[td.x].step;
-        if (!(s < J.end_step && s % J.stride == 0 && s % J.hi_eps_freq == 0)) continue;
-        const int d = td.y + lane;
-        const int n = J.n;
-        if (d >= n) continue;
-        const float4* __restrict__ cur = J.coord[s & 1];
-        const float4 c = cur[d];
-        const float a = J.hi_a;
-        float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (!(J.flags & JF_HI_LISTED)) {
-            for (int i = 0; i < n; ++i)
-                if (i != d) eps_add(c, cur[i], a, sum);
-        } else {
-            for (int k = 0; k < J.cov_cnt[d]; ++k) eps_add(c, cur[J.cov_j[(size_t)k * n + d]], a, sum);
-            for (int k = 0; k < J.nat_cnt[d]; ++k) eps_add(c, cur[J.nat_j[(size_t)k * n + d]], a, sum);
-            for (int k = 0; k < J.pair_cnt[d]; ++k) eps_add(c, cur[list_at(J.pair_j, J.idx16, (size_t)k * n + d)], a, sum);
+// Per-bead sums of the TEA epsilon (bdhitea_kernel.cu:212-249), one
+// 1024-thread workgroup per 64-bead tile of a TEA job: without lists, the
+// sixteen wavefronts form the terms of 64 partners j at a time into LDS
+// and wavefront 0 adds them to bead i's sums in j order (the same bits as
+// the bead-by-bead loop); with lists, wavefront 0 walks bead i's lists.
+__global__ __launch_bounds__(1024) void k_hi_eps(const DevJob* __restrict__ jobs, const DevJobState* __restrict__ st,
+                                                 const int2* __restrict__ tiles, int ntiles) {
+    __shared__ float term[4][64][65];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int2 td = tiles[blockIdx.x];
+    const DevJob& J = jobs[td.x];
+    if (!(J.flags & JF_HI) || (J.flags & JF_HI_EXACT)) return;
+    const long long s = st[td.x].step;
+    if (!(s < J.end_step && s % J.stride == 0 && s % J.hi_eps_freq == 0)) return;
+    const int d = td.y + lane;
+    const int n = J.n;
+    const float4* __restrict__ cur = J.coord[s & 1];
+    const float4 c = d < n ? cur[d] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float a = J.hi_a;
+    float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (J.flags & JF_HI_LISTED) {
+        if (w != 0 || d >= n) return;
+        for (int k = 0; k < J.cov_cnt[d]; ++k) eps_add(c, cur[J.cov_j[(size_t)k * n + d]], a, sum);
+        for (int k = 0; k < J.nat_cnt[d]; ++k) eps_add(c, cur[J.nat_j[(size_t)k * n + d]], a, sum);
+        for (int k = 0; k < J.pair_cnt[d]; ++k) eps_add(c, cur[list_at(J.pair_j, J.idx16, (size_t)k * n + d)], a, sum);
+        J.ci[d] = sum;
+        J.eps[d] = sum.w;
+        return;
+    }
+    for (int base = 0; base < n; base += 64) {
+        for (int jl = w; jl < 64; jl += 16) {
+            const int i = base + jl;
+            float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (i < n && i != d && d < n) {
+                float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+                eps_add(c, cur[i], a, z);
+                p = z;
+            }
+            term[0][lane][jl] = p.x;
+            term[1][lane][jl] = p.y;
+            term[2][lane][jl] = p.z;
+            term[3][lane][jl] = p.w;
         }
+        __syncthreads();
+        if (w == 0 && d < n) {
+            const int m = min(64, n - base);
+            for (int jl = 0; jl < m; ++jl) {
+                if (base + jl == d) continue;
+                sum.x += term[0][lane][jl];
+                sum.y += term[1][lane][jl];
+                sum.z += term[2][lane][jl];
+                sum.w += term[3][lane][jl];
+            }
+        }
+        __syncthreads();
+    }
+    if (w == 0 && d < n) {
         J.ci[d] = sum;
         J.eps[d] = sum.w;
     }
@@ -190,66 +224,92 @@ __device__ __forceinline__ void tea_force(const float4 c1, const float4 c2, cons
     fz += D.xz * gx + D.yz * gy + D.zz * gz;
 }
 
-__global__ __launch_bounds__(256) void k_hi_tea(const DevJob* __restrict__ jobs, const DevJobState* __restrict__ st,
-                                                const int2* __restrict__ tiles, int ntiles, int kstep) {
-    FOR_TILES(ntiles) {
-        const int2 td = tiles[t];
-        const DevJob& J = jobs[td.x];
-        if (!(J.flags & JF_HI) || (J.flags & JF_HI_EXACT)) continue;
-        const DevJobState& S = st[td.x];
-        const long long s = S.step + kstep;
-        if (s >= J.end_step) continue;
-        const int d = td.y + lane;
-        const int n = J.n;
-        if (d >= n) continue;
-        const float4* __restrict__ cur = J.coord[s & 1];
+// The TEA step (bdhitea_kernel.cu:364-515), one 1024-thread workgroup per
+// 64-bead tile of a TEA job; the partner sum over all beads is formed as in
+// k_hi_eps (terms by all wavefronts, sums in j order by wavefront 0), the
+// listed variant walks the lists in wavefront 0.
+__global__ __launch_bounds__(1024) void k_hi_tea(const DevJob* __restrict__ jobs, const DevJobState* __restrict__ st,
+                                                 const int2* __restrict__ tiles, int ntiles, int kstep) {
+    __shared__ float term[3][64][65];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int2 td = tiles[blockIdx.x];
+    const DevJob& J = jobs[td.x];
+    if (!(J.flags & JF_HI) || (J.flags & JF_HI_EXACT)) return;
+    const DevJobState& S = st[td.x];
+    const long long s = S.step + kstep;
+    if (s >= J.end_step) return;
+    const int d = td.y + lane;
+    const int n = J.n;
+    const bool own = d < n;
+    const float4* __restrict__ cur = J.coord[s & 1];
+    float ci0 = 0.f, ci1 = 0.f, ci2 = 0.f, fx = 0.f, fy = 0.f, fz = 0.f;
+    float4 coord = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (own) {
         float4 cs = J.ci[d];
         const float4 f0 = J.mforce[d];
         const float4 df = J.rforce[d];
         const float beta = S.beta;
         cs.w = beta * beta;
-        float ci0 = 1.f / sqrtf(1.f + cs.w * cs.x);
-        float ci1 = 1.f / sqrtf(1.f + cs.w * cs.y);
-        float ci2 = 1.f / sqrtf(1.f + cs.w * cs.z);
-        const float4 coord = cur[d];
-        float fx = f0.x, fy = f0.y, fz = f0.z;
+        ci0 = 1.f / sqrtf(1.f + cs.w * cs.x);
+        ci1 = 1.f / sqrtf(1.f + cs.w * cs.y);
+        ci2 = 1.f / sqrtf(1.f + cs.w * cs.z);
+        coord = cur[d];
+        fx = f0.x, fy = f0.y, fz = f0.z;
         fx += df.x * ci0;
         fy += df.y * ci1;
         fz += df.z * ci2;
         ci0 *= beta;
         ci1 *= beta;
         ci2 *= beta;
-        const float a = J.hi_a;
-        float tx, ty, tz;
-        if (!(J.flags & JF_HI_LISTED)) {
-            for (int i = 0; i < n; ++i) {
-                if (i == d) continue;
-                tx = 0.f; ty = 0.f; tz = 0.f;
-                tea_force(coord, cur[i], J.mforce[i], J.rforce[i], ci0, ci1, ci2, a, tx, ty, tz);
-                fx += tx; fy += ty; fz += tz;
-            }
-        } else {
-            for (int k = 0; k < J.cov_cnt[d]; ++k) {
-                const int i = J.cov_j[(size_t)k * n + d];
-                tx = 0.f; ty = 0.f; tz = 0.f;
-                tea_force(coord, cur[i], J.mforce[i], J.rforce[i], ci0, ci1, ci2, a, tx, ty, tz);
-                fx += tx; fy += ty; fz += tz;
-            }
-            for (int k = 0; k < J.nat_cnt[d]; ++k) {
-                const int i = J.nat_j[(size_t)k * n + d];
-                tx = 0.f; ty = 0.f; tz = 0.f;
-                tea_force(coord, cur[i], J.mforce[i], J.rforce[i], ci0, ci1, ci2, a, tx, ty, tz);
-                fx += tx; fy += ty; fz += tz;
-            }
-            for (int k = 0; k < J.pair_cnt[d]; ++k) {
-                const int i = list_at(J.pair_j, J.idx16, (size_t)k * n + d);
-                tx = 0.f; ty = 0.f; tz = 0.f;
-                tea_force(coord, cur[i], J.mforce[i], J.rforce[i], ci0, ci1, ci2, a, tx, ty, tz);
-                fx += tx; fy += ty; fz += tz;
-            }
+    }
+    const float a = J.hi_a;
+    float tx, ty, tz;
+    if (J.flags & JF_HI_LISTED) {
+        if (w != 0 || !own) return;
+        for (int k = 0; k < J.cov_cnt[d]; ++k) {
+            const int i = J.cov_j[(size_t)k * n + d];
+            tx = 0.f; ty = 0.f; tz = 0.f;
+            tea_force(coord, cur[i], J.mforce[i], J.rforce[i], ci0, ci1, ci2, a, tx, ty, tz);
+            fx += tx; fy += ty; fz += tz;
+        }
+        for (int k = 0; k < J.nat_cnt[d]; ++k) {
+            const int i = J.nat_j[(size_t)k * n + d];
+            tx = 0.f; ty = 0.f; tz = 0.f;
+            tea_force(coord, cur[i], J.mforce[i], J.rforce[i], ci0, ci1, ci2, a, tx, ty, tz);
+            fx += tx; fy += ty; fz += tz;
+        }
+        for (int k = 0; k < J.pair_cnt[d]; ++k) {
+            const int i = list_at(J.pair_j, J.idx16, (size_t)k * n + d);
+            tx = 0.f; ty = 0.f; tz = 0.f;
+            tea_force(coord, cur[i], J.mforce[i], J.rforce[i], ci0, ci1, ci2, a, tx, ty, tz);
+            fx += tx; fy += ty; fz += tz;
         }
         hi_finish(J, d, s, fx, fy, fz, coord);
+        return;
     }
+    for (int base = 0; base < n; base += 64) {
+        for (int jl = w; jl < 64; jl += 16) {
+            const int i = base + jl;
+            tx = 0.f; ty = 0.f; tz = 0.f;
+            if (i < n && i != d && own)
+                tea_force(coord, cur[i], J.mforce[i], J.rforce[i], ci0, ci1, ci2, a, tx, ty, tz);
+            term[0][lane][jl] = tx;
+            term[1][lane][jl] = ty;
+            term[2][lane][jl] = tz;
+        }
+        __syncthreads();
+        if (w == 0 && own) {
+            const int m = min(64, n - base);
+            for (int jl = 0; jl < m; ++jl) {
+                if (base + jl == d) continue;
+                fx += term[0][lane][jl];
+                fy += term[1][lane][jl];
+                fz += term[2][lane][jl];
+            }
+        }
+        __syncthreads();
+    }
+    if (w == 0 && own) hi_finish(J, d, s, fx, fy, fz, coord);
 }
 
 // ---------------------------------------------------------------------------
@@ -1089,14 +1149,14 @@ static int grid_tiles(int ntiles) { return ntiles > 0 ? (ntiles + 3) / 4 : 1; }
 
 hipError_t launch_hi_tea_update(const DevJob* jobs, DevJobState* st, const int2* tiles, int ntiles, int njobs,
                                 hipStream_t stream) {
-    hipLaunchKernelGGL(k_hi_eps, dim3(grid_tiles(ntiles)), dim3(256), 0, stream, jobs, st, tiles, ntiles);
+    if (ntiles > 0) hipLaunchKernelGGL(k_hi_eps, dim3(ntiles), dim3(1024), 0, stream, jobs, st, tiles, ntiles);
     hipLaunchKernelGGL(k_hi_beta, dim3((njobs + 63) / 64), dim3(64), 0, stream, jobs, st, njobs);
     return hipGetLastError();
 }
 
 hipError_t launch_hi_tea_step(const DevJob* jobs, const DevJobState* st, const int2* tiles, int ntiles, int kstep,
                               hipStream_t stream) {
-    hipLaunchKernelGGL(k_hi_tea, dim3(grid_tiles(ntiles)), dim3(256), 0, stream, jobs, st, tiles, ntiles, kstep);
+    if (ntiles > 0) hipLaunchKernelGGL(k_hi_tea, dim3(ntiles), dim3(1024), 0, stream, jobs, st, tiles, ntiles, kstep);
     return hipGetLastError();
 }
 
