@@ -57,6 +57,19 @@ __device__ __forceinline__ TileRange xcd_tiles(int ntiles) {
     return t;
 }
 
+// The descriptor's arrays as global-memory pointers.  Through a generic
+// pointer the compiler issues flat loads, which are counted with the LDS
+// operations as well, so a wait for an LDS read also waits for every
+// prefetch in flight; global loads are counted apart from the LDS reads.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* gmem(const T* p) {
+    return (const __attribute__((address_space(1))) T*)p;
+}
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gmem(T* p) {
+    return (__attribute__((address_space(1))) T*)p;
+}
+
 // ---------------------------------------------------------------------------
 // fused step
 // ---------------------------------------------------------------------------
@@ -66,7 +79,7 @@ __device__ __forceinline__ TileRange xcd_tiles(int ntiles) {
 // step) and fetched one batch of eight ahead: while batch b is computed the
 // indices of batch b+1 and the coordinates of batch b are in flight.
 template <typename IdxT>
-__device__ __forceinline__ void pair_loop(const IdxT* __restrict__ pj, int np, int n, const float4* __restrict__ cin,
+__device__ __forceinline__ void pair_loop(const __attribute__((address_space(1))) IdxT* __restrict__ pj, int np, int n, const float4* __restrict__ cin,
                                           const float4 c, float rc2, float a2, float m6, float& fx, float& fy,
                                           float& fz) {
 #define SOP_PAIR(o)                                                  \
@@ -121,8 +134,8 @@ __device__ __forceinline__ void step_bead(const DevJob& J, const DevJobState& S,
     // FENE (covalent_kernel.cu:36-48)
     {
         const int nc = J.cov_cnt[i];
-        const int* __restrict__ cj = J.cov_j + i;
-        const float* __restrict__ cr = J.cov_r0 + i;
+        const auto cj = gmem(J.cov_j) + i;
+        const auto cr = gmem(J.cov_r0) + i;
         for (int k = 0; k < nc; ++k) {
             const int j = cj[(size_t)k * n];
             const float r0 = cr[(size_t)k * n];
@@ -139,7 +152,7 @@ __device__ __forceinline__ void step_bead(const DevJob& J, const DevJobState& S,
     // native LJ 12-6 (native_kernel.cu:36-49), gathers four ahead
     {
         const int nn = J.nat_cnt[i];
-        const int* __restrict__ nj = J.nat_j + i;
+        const auto nj = gmem(J.nat_j) + i;
         const float2* __restrict__ npar = J.nat_par + i;
 #define SOP_NAT(o, par)                                                 \
 {                                                                   \
@@ -174,11 +187,11 @@ __device__ __forceinline__ void step_bead(const DevJob& J, const DevJobState& S,
 #undef SOP_NAT
     }
     // repulsive LJ over the Verlet list (pairs_kernel.cu:41-56)
-    const int np = J.pair_cnt[r];
+    const int np = gmem(J.pair_cnt)[r];
     if (J.idx16)
-        pair_loop(static_cast<const unsigned short*>(J.pair_j) + r, np, n, cin, c, J.rc2, J.a2, J.m6, fx, fy, fz);
+        pair_loop(gmem(static_cast<const unsigned short*>(J.pair_j)) + r, np, n, cin, c, J.rc2, J.a2, J.m6, fx, fy, fz);
     else
-        pair_loop(static_cast<const int*>(J.pair_j) + r, np, n, cin, c, J.rc2, J.a2, J.m6, fx, fy, fz);
+        pair_loop(gmem(static_cast<const int*>(J.pair_j)) + r, np, n, cin, c, J.rc2, J.a2, J.m6, fx, fy, fz);
     // indentation tip and substrate plane (indentation_kernel.cu:71-128)
     if (J.flags & JF_INDENTATION) {
         float4 fin = J.tipF[i];
@@ -709,8 +722,8 @@ __device__ __forceinline__ void verlet_tile(const DevJob& J, DevJobState& S, con
     const int b_lo = RPW * w, b_hi = min(b_lo + RPW, nrow);
     const float d2max = J.pl_d2;  // sqrtf(d2) < pl_cut, exactly
     const int cap = J.pair_cap, ppcap = J.pp_cap;
-    const IdxT* __restrict__ pp = static_cast<const IdxT*>(J.pp_j);
-    IdxT* __restrict__ pj = static_cast<IdxT*>(J.pair_j);
+    const auto pp = gmem(static_cast<const IdxT*>(J.pp_j));
+    const auto pj = gmem(static_cast<IdxT*>(J.pair_j));
     const int bl = lane < nrow ? J.perm[i0 + lane] : 0;    // beads of the tile's rows
     const int cl = lane < nrow ? J.pp_cnt[bl] : 0;         // their possible counts
     // One row's chunk of candidates (64 CH entries from `base`): compare and
@@ -736,7 +749,7 @@ __device__ __forceinline__ void verlet_tile(const DevJob& J, DevJobState& S, con
     };
     auto load_idx = [&](int b, int base, int (&j)[CH]) {
         const int npp = __shfl(cl, b);
-        const IdxT* __restrict__ row = pp + (size_t)__shfl(bl, b) * ppcap;
+        const auto row = pp + (size_t)__shfl(bl, b) * ppcap;
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
             const int k = base + 64 * u + lane;
@@ -813,7 +826,7 @@ __device__ __forceinline__ void verlet_store(const DevJob& J, int i0, const IdxT
     for (int o = 32; o > 0; o >>= 1) maxc = max(maxc, __shfl_xor(maxc, o));
     // the four waves' maxima differ only in which beads they saw: all waves
     // see all 64 beads here (b = lane), so maxc is the tile's
-    IdxT* __restrict__ pj = static_cast<IdxT*>(J.pair_j);
+    const auto pj = gmem(static_cast<IdxT*>(J.pair_j));
     const int lim = min(c, E);
     for (int k = kq; k < maxc; k += WAVES)
         if (k < lim) pj[(size_t)k * n + i0 + b] = rows[b * RS + k];
