@@ -719,93 +719,105 @@ __device__ __forceinline__ void verlet_tile(const DevJob& J, DevJobState& S, con
     const int lane = threadIdx.x & 63;
     const int n = J.n;
     const int nrow = min(64, n - i0);
+    w = __builtin_amdgcn_readfirstlane(w);
     const int b_lo = RPW * w, b_hi = min(b_lo + RPW, nrow);
     const float d2max = J.pl_d2;  // sqrtf(d2) < pl_cut, exactly
     const int cap = J.pair_cap, ppcap = J.pp_cap;
     const auto pp = gmem(static_cast<const IdxT*>(J.pp_j));
     const auto pj = gmem(static_cast<IdxT*>(J.pair_j));
-    const int bl = lane < nrow ? J.perm[i0 + lane] : 0;    // beads of the tile's rows
-    const int cl = lane < nrow ? J.pp_cnt[bl] : 0;         // their possible counts
-    // One row's chunk of candidates (64 CH entries from `base`): compare and
-    // compact the accepted ones into the row's LDS entries (or their slots).
-    auto filter = [&](int b, const float4 r1, int npp, int base, const int (&j)[CH], const float4 (&c)[CH],
-                      int& cnt) {
-#pragma unroll
-        for (int u = 0; u < CH; ++u) {
-            if (base + 64 * u >= npp) break;
-            bool ok = false;
-            if (j[u] >= 0) {
-                const float x = c[u].x - r1.x, y = c[u].y - r1.y, z = c[u].z - r1.z;
-                ok = x * x + y * y + z * z <= d2max;
-            }
-            const unsigned long long m = __ballot(ok);
-            if (ok) {
-                const int pos = cnt + __popcll(m & lanes_below(lane));
-                if (pos < E) rows[b * RS + pos] = (IdxT)j[u];
-                else if (pos < cap) pj[(size_t)pos * n + i0 + b] = (IdxT)j[u];
-            }
-            cnt += __popcll(m);
-        }
-    };
+    // beads of the tile's rows and their possible counts (loads issued
+    // unconditionally, from clamped addresses, so that no branch leaves a
+    // load pending on one path only)
+    const int bl0 = gmem(J.perm)[i0 + min(lane, nrow - 1)];
+    const int cl0 = gmem(J.pp_cnt)[bl0];
+    const int bl = lane < nrow ? bl0 : 0;
+    const int cl = lane < nrow ? cl0 : 0;
+    // row b's bead and count (b is wave-uniform: a lane read, not a shuffle)
+    auto bead_of = [&](int b) { return __builtin_amdgcn_readlane(bl, b); };
+    auto count_of = [&](int b) { return __builtin_amdgcn_readlane(cl, b); };
+    // Raw words of 64 CH entries of row b from `base`, loaded without a
+    // branch from addresses clamped into the row; the filter replaces the
+    // words past the row's count by 0 (a valid bead) and rejects them.
     auto load_idx = [&](int b, int base, int (&j)[CH]) {
-        const int npp = __shfl(cl, b);
-        const auto row = pp + (size_t)__shfl(bl, b) * ppcap;
+        const auto row = pp + (size_t)bead_of(b) * ppcap;
 #pragma unroll
-        for (int u = 0; u < CH; ++u) {
-            const int k = base + 64 * u + lane;
-            j[u] = k < npp ? (int)row[k] : -1;
+        for (int u = 0; u < CH; ++u) j[u] = (int)row[(unsigned)min(base + 64 * u + lane, ppcap - 1)];
+    };
+    // One chunk: gather, compare, and compact the accepted entries in order
+    // into the row's LDS entries (or, past them, their slots).  Gathers go
+    // out four at a time.
+    auto filter = [&](int b, const float4 r1, int npp, int base, int (&j)[CH], int& cnt) {
+#pragma unroll
+        for (int u = 0; u < CH; ++u) j[u] = base + 64 * u + lane < npp ? j[u] : 0;
+#pragma unroll
+        for (int h = 0; h < CH; h += 4) {
+            float4 c[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) c[u] = cur[j[h + u]];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int k0 = base + 64 * (h + u);
+                if (k0 < npp) {
+                    const float x = c[u].x - r1.x, y = c[u].y - r1.y, z = c[u].z - r1.z;
+                    const bool ok = k0 + lane < npp && x * x + y * y + z * z <= d2max;
+                    const unsigned long long m = __ballot(ok);
+                    if (ok) {
+                        const int pos = cnt + __popcll(m & lanes_below(lane));
+                        if (pos < E) rows[b * RS + pos] = (IdxT)j[h + u];
+                        else if (pos < cap) pj[(size_t)pos * n + i0 + b] = (IdxT)j[h + u];
+                    }
+                    cnt += __popcll(m);
+                }
+            }
         }
     };
-    // rows in a software pipeline two rows deep: the first chunks of
-    // indices of rows b+1 and b+2 (and their centres) are in flight while
-    // row b's gathers and compares run
-    int j[CH], ja[CH];
-    float4 r1 = make_float4(0.f, 0.f, 0.f, 0.f), r1a = r1;
-#pragma unroll
-    for (int u = 0; u < CH; ++u) j[u] = ja[u] = -1;
-    if (b_lo < b_hi) {
-        load_idx(b_lo, 0, j);
-        r1 = cur[__shfl(bl, b_lo)];
-    }
-    if (b_lo + 1 < b_hi) {
-        load_idx(b_lo + 1, 0, ja);
-        r1a = cur[__shfl(bl, b_lo + 1)];
-    }
-    for (int b = b_lo; b < b_hi; ++b) {
-        const int npp = __shfl(cl, b);
-        float4 c[CH];
-#pragma unroll
-        for (int u = 0; u < CH; ++u) c[u] = j[u] >= 0 ? cur[j[u]] : make_float4(0.f, 0.f, 0.f, 0.f);
-        int jn[CH];
-        float4 r1n = r1;
-        if (b + 2 < b_hi) {
-            load_idx(b + 2, 0, jn);
-            r1n = cur[__shfl(bl, b + 2)];
-        } else {
-#pragma unroll
-            for (int u = 0; u < CH; ++u) jn[u] = -1;
-        }
+    // Row b from its first chunk of indices j (loaded ahead); a second chunk
+    // is loaded before the first is filtered, further ones as they come.
+    auto do_row = [&](int b, int (&j)[CH], const float4 r1) {
+        const int npp = count_of(b);
         int cnt = 0;
-        filter(b, r1, npp, 0, j, c, cnt);
-        for (int base = 64 * CH; base < npp; base += 64 * CH) {  // rows longer than one chunk
-            int j2[CH];
-            load_idx(b, base, j2);
-            float4 c2[CH];
-#pragma unroll
-            for (int u = 0; u < CH; ++u) c2[u] = j2[u] >= 0 ? cur[j2[u]] : make_float4(0.f, 0.f, 0.f, 0.f);
-            filter(b, r1, npp, base, j2, c2, cnt);
+        int j2[CH];
+        const bool two = npp > 64 * CH;
+        if (two) load_idx(b, 64 * CH, j2);
+        filter(b, r1, npp, 0, j, cnt);
+        if (two) {
+            filter(b, r1, npp, 64 * CH, j2, cnt);
+            for (int base = 128 * CH; base < npp; base += 64 * CH) {
+                int j3[CH];
+                load_idx(b, base, j3);
+                filter(b, r1, npp, base, j3, cnt);
+            }
         }
         if (lane == 0) {
             cnt_s[b] = cnt;
             if (cnt > cap) atomicOr(&S.overflow, 2);
         }
+    };
+    // first chunk and centre of row b (rows past the wave's last load that
+    // row's again, unused: no branch)
+    auto ahead = [&](int b, int (&j)[CH], float4& r1) {
+        b = min(b, b_hi - 1);
+        load_idx(b, 0, j);
+        r1 = cur[bead_of(b)];
+    };
+    // The wave's rows as straight-line code (the row loop fully unrolled),
+    // a software pipeline two rows deep: while row b is filtered the first
+    // chunk of row b+1 is in flight, and row b+2's is issued as soon as row
+    // b is done with its registers.  Unrolled, every prefetch has registers
+    // of its own and the loads can be waited for one by one.
+    if (b_lo >= b_hi) return;
+    constexpr int D = 2;
+    int js[D][CH];
+    float4 rs[D];
 #pragma unroll
-        for (int u = 0; u < CH; ++u) {
-            j[u] = ja[u];
-            ja[u] = jn[u];
+    for (int q = 0; q < D; ++q) ahead(b_lo + q, js[q], rs[q]);
+#pragma unroll
+    for (int q = 0; q < RPW; ++q) {
+        const int b = b_lo + q;
+        if (b < b_hi) {
+            do_row(b, js[q % D], rs[q % D]);
+            if (q + D < RPW) ahead(b + D, js[q % D], rs[q % D]);
         }
-        r1 = r1a;
-        r1a = r1n;
     }
 }
 
