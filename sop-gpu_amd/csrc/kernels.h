@@ -18,8 +18,12 @@ hipError_t launch_step(const DevJob* jobs, const DevJobState* st, const int2* ti
 // List rebuilds for jobs whose cadence is due (force != 0: rebuild all).
 // cell_words: bitmap words (ceil(n/64)) of the largest job with a cell grid,
 // 0 to build every job's possible list by the all-pairs scan.
+// sunits/segs/nsunits/lds_beads: the LDS step plan (see launch_step), whose
+// jobs get their possible lists scanned from LDS-staged coordinates
+// (nsunits = 0: all jobs by the tile kernels).
 hipError_t launch_possible(const DevJob* jobs, DevJobState* st, const int2* tiles, int ntiles, int njobs,
-                           int cell_words, int force, hipStream_t stream);
+                           int cell_words, const int2* sunits, const int4* segs, int nsunits, int lds_beads,
+                           int force, hipStream_t stream);
 // Largest cell_words launch_possible accepts (the LDS bitmap of one wavefront).
 constexpr int kMaxCellWords = 8192;
 // any_idx32: some job of the launch stores its lists as int32 (n >= 65536).

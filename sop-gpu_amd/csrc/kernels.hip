@@ -355,13 +355,14 @@ __device__ __forceinline__ unsigned long long lanes_below(int lane) {
 // is evaluated as d^2 <= pp_d2 (the same set of floats).
 __global__ __launch_bounds__(256) void k_possible(const DevJob* __restrict__ jobs, DevJobState* __restrict__ st,
                                                   const int2* __restrict__ tiles, int ntiles, int force,
-                                                  int use_cells) {
+                                                  int use_cells, int skip_lds) {
     const int lane = threadIdx.x & 63;
     const TileRange tr = xcd_tiles(ntiles);
     for (int t = tr.begin; t < tr.end; t += tr.step) {
         const int2 td = tiles[t];
         const DevJob& J = jobs[td.x];
         if (use_cells && J.cell_cnt) continue;  // built by k_possible_cells
+        if (skip_lds && J.lds_step && J.idx16) continue;  // built by k_possible_lds
         const long long s = st[td.x].step;
         if (!force && !due(J, s, J.pp_freq)) continue;
         const int n = J.n;
@@ -409,6 +410,143 @@ __global__ __launch_bounds__(256) void k_possible(const DevJob* __restrict__ job
         if (own) {
             J.pp_cnt[i0 + lane] = cnt < cap ? cnt : cap;
             if (cnt > cap) atomicOr(&st[td.x].overflow, 1);
+        }
+    }
+}
+
+// Possible pairs of the LDS-step jobs (16-bit entries; the LDS-step bound
+// keeps them far below 65,536 beads), over the step kernel's units: the
+// workgroup stages the coordinates of each due job in LDS, and wavefront w
+// builds the lists of the beads of the unit's rows 64w..64w+63, one bead at
+// a time.  The bead is wave-uniform, so its coordinates are one broadcast
+// read, its sorted exclusions (and the bead itself) become a 64-bit mask
+// per chunk of candidates in scalar registers, and the accepted count is a
+// scalar; the candidates j come 64 per chunk from LDS in ascending order.
+// The same list in the same order as k_possible.
+__device__ __forceinline__ bool possible_lds_wanted(const DevJob& J, long long s, int force, int use_cells) {
+    return J.lds_step && J.idx16 && !(use_cells && J.cell_cnt) && (force || due(J, s, J.pp_freq));
+}
+
+__global__ __launch_bounds__(1024) void k_possible_lds(const DevJob* __restrict__ jobs, DevJobState* __restrict__ st,
+                                                       const int2* __restrict__ units, const int4* __restrict__ segs,
+                                                       int nunits, int force, int use_cells) {
+    extern __shared__ float4 lds_coord[];
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
+        const int2 ud = units[u];  // first segment, segments
+        __syncthreads();           // the previous unit's reads are done
+        bool any = false;
+        for (int q = 0; q < ud.y; ++q) {
+            const int4 sg = segs[ud.x + q];
+            const DevJob& J = jobs[sg.x];
+            const long long s = st[sg.x].step;
+            if (!possible_lds_wanted(J, s, force, use_cells)) continue;
+            any = true;
+            const float4* __restrict__ g = J.coord[s & 1];
+            for (int k = threadIdx.x; k < J.n; k += blockDim.x) lds_coord[sg.w + k] = g[k];
+        }
+        if (!any) continue;  // the same for every thread of the block
+        __syncthreads();
+        int t = 64 * w;
+        int q = 0;
+        int4 sg = make_int4(0, 0, 0, 0);
+        for (; q < ud.y; ++q) {  // the segment holding this wavefront's rows
+            sg = segs[ud.x + q];
+            const int span = (sg.z + 63) & ~63;
+            if (t < span) break;
+            t -= span;
+        }
+        if (q == ud.y) continue;
+        const DevJob& J = jobs[sg.x];
+        if (!possible_lds_wanted(J, st[sg.x].step, force, use_cells)) continue;
+        const int n = J.n;
+        const float4* __restrict__ cur = lds_coord + sg.w;
+        const float d2max = J.pp_d2;
+        const int cap = J.pp_cap, ecap = J.excl_cap;
+        const auto ecnt = gmem(J.excl_cnt);
+        const auto ej = gmem(J.excl_j);
+        const auto pp = gmem(static_cast<unsigned short*>(J.pp_j));
+        const auto pcnt = gmem(J.pp_cnt);
+        int* const ovf = &st[sg.x].overflow;
+        const int r0 = sg.y + t, r1 = sg.y + min(t + 64, sg.z);
+        // bead i's exclusion count and its first 64 exclusions (lane k holds
+        // the k-th), loaded one bead ahead from clamped addresses
+        auto load_ex = [&](int i, int& ne, int& ev) {
+            ne = ecnt[i];
+            ev = ej[(size_t)min(lane, ecap - 1) * n + i];
+        };
+        int ne_n, ev_n;
+        load_ex(r0, ne_n, ev_n);
+        for (int i = r0; i < r1; ++i) {
+            const int ne = ne_n;
+            int ev = lane < ne ? ev_n : INT_MAX;
+            load_ex(min(i + 1, r1 - 1), ne_n, ev_n);
+            const float4 ci = cur[i];
+            const auto row = pp + (size_t)i * cap;
+            int eg = 0, pe = 0;  // exclusion group (of 64) and index within it
+            int e_next = ne > 0 ? __builtin_amdgcn_readlane(ev, 0) : INT_MAX;
+            int self = i;  // the bead itself, until its chunk is passed
+            int cnt = 0;
+            // One chunk of 64 candidates from `base` with coordinates c; TAIL:
+            // the chunk may run past the job's end.  The exclusion mask is
+            // formed only for chunks holding an excluded partner or the bead.
+            auto chunk = [&](int base, const float4 c, bool tail) {
+                const int lim = base + 64;
+                const float x = c.x - ci.x, y = c.y - ci.y, z = c.z - ci.z;
+                bool acc = x * x + y * y + z * z <= d2max;
+                if (tail) acc = acc & (base + lane < n);
+                unsigned long long m = __ballot(acc);
+                if (min(e_next, self) < lim) {
+                    unsigned long long drop = 0ull;
+                    while (e_next < lim) {
+                        drop |= 1ull << (e_next - base);
+                        if (++pe == 64) {
+                            eg += 64;
+                            pe = 0;
+                            ev = eg + lane < ne ? ej[(size_t)(eg + lane) * n + i] : INT_MAX;
+                        }
+                        e_next = eg + pe < ne ? __builtin_amdgcn_readlane(ev, pe) : INT_MAX;
+                    }
+                    if (self < lim) {
+                        drop |= 1ull << (self - base);
+                        self = INT_MAX;
+                    }
+                    m &= ~drop;
+                    acc = acc && !((drop >> lane) & 1ull);
+                }
+                if (m) {
+                    if (acc) {
+                        const unsigned pos = __builtin_amdgcn_mbcnt_hi(
+                            (unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, (unsigned)cnt));
+                        if ((int)pos < cap)
+                            *reinterpret_cast<__attribute__((address_space(1))) unsigned short*>(
+                                reinterpret_cast<__attribute__((address_space(1))) char*>(row) + (pos << 1)) =
+                                (unsigned short)(base + lane);
+                    }
+                    cnt += __popcll(m);
+                }
+            };
+            int b4 = 0;
+            for (; b4 + 256 <= n; b4 += 256) {  // four whole chunks, their LDS reads in flight together
+                float4 c[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) c[v] = cur[b4 + 64 * v + lane];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) chunk(b4 + 64 * v, c[v], false);
+            }
+            if (b4 < n) {  // the last one to four chunks
+                float4 c[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) c[v] = cur[min(b4 + 64 * v + lane, n - 1)];
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    if (b4 + 64 * v < n) chunk(b4 + 64 * v, c[v], true);
+            }
+            if (lane == 0) {
+                pcnt[i] = cnt < cap ? cnt : cap;
+                if (cnt > cap) atomicOr(ovf, 1);
+            }
         }
     }
 }
@@ -1232,11 +1370,15 @@ hipError_t launch_step(const DevJob* jobs, const DevJobState* st, const int2* ti
 }
 
 hipError_t launch_possible(const DevJob* jobs, DevJobState* st, const int2* tiles, int ntiles, int njobs,
-                           int cell_words, int force, hipStream_t stream) {
+                           int cell_words, const int2* sunits, const int4* segs, int nsunits, int lds_beads,
+                           int force, hipStream_t stream) {
     if (ntiles <= 0) return hipSuccess;
     const int use_cells = cell_words > 0;
     hipLaunchKernelGGL(k_possible, dim3(grid_for(ntiles, 0)), dim3(256), 0, stream, jobs, st, tiles, ntiles, force,
-                       use_cells);
+                       use_cells, nsunits > 0);
+    if (nsunits > 0)
+        hipLaunchKernelGGL(k_possible_lds, dim3(nsunits), dim3(kLdsUnit), (size_t)lds_beads * sizeof(float4), stream,
+                           jobs, st, sunits, segs, nsunits, force, use_cells);
     if (use_cells) {
         hipLaunchKernelGGL(k_cells_setup, dim3(njobs), dim3(1024), 0, stream, jobs, st, njobs, force);
         hipLaunchKernelGGL(k_cells_count, dim3(grid_for(ntiles, 0)), dim3(256), 0, stream, jobs, st, tiles, ntiles,

@@ -78,6 +78,32 @@ def test_possible_pairs_with_many_exclusions(eng):
         eng.remove_job(jid)
 
 
+def test_possible_pairs_more_than_64_exclusions(eng):
+    """Beads with more than 64 exclusions (the list scan takes them 64 at a
+    time): bead 5 and bead 250 get ~100 extra native contacts each."""
+    sysm = numpy_chain(300, seed=11)
+    have = {tuple(sorted(p)) for p in sysm.natives.tolist()}
+    extra = [p for p in [(5, j) for j in range(9, 260, 2)] + [(j, 250) for j in range(20, 246, 2)]
+             if tuple(sorted(p)) not in have]
+    assert sum(1 for p in extra if 5 in p) > 64 and sum(1 for p in extra if 250 in p) > 64
+    c = sysm.coords[:, :3].astype(np.float64)
+    ex = np.array(extra, np.int32)
+    sysm.natives = np.concatenate([sysm.natives, ex])
+    sysm.nat_r0 = np.concatenate([sysm.nat_r0, np.linalg.norm(c[ex[:, 0]] - c[ex[:, 1]], axis=1).astype(np.float32)])
+    sysm.nat_eh = np.concatenate([sysm.nat_eh, np.ones(len(ex), np.float32)])
+    kw = c0_params(pairs_threshold=200.0, pairlist_cutoff=30.0, max_native=256)
+    jid = eng.add_job(S.make_params(**kw), sysm, 0)
+    try:
+        eng.rebuild(jid, 3)
+        ref = O.Sim(O.make_params(**kw), sysm)
+        ref.build_possible()
+        ref.build_verlet()
+        assert lists_equal(*eng.list(jid, 3), *ref.list(3))
+        assert lists_equal(*eng.list(jid, 2), *ref.list(2))
+    finally:
+        eng.remove_job(jid)
+
+
 def _pull_system(n, seed):
     sysm = numpy_chain(n, seed=seed) if n <= 400 else S.synth_chain(n, seed=seed)
     sysm.fixed = np.array([0], np.int32)
