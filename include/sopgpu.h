@@ -218,6 +218,15 @@ int sop_copy_bandwidth(int device, unsigned long long bytes, int repeats, double
 int sop_generate_seeds(int seed, int n, uint32_t* out4);
 int sop_rng_probe(int device, const uint32_t* seeds4, int n, int count, int gaussian, float* out, uint32_t* state_out4);
 
+/* Self-check with no counterpart in the reference: the repulsive-pair
+ * kernel divides a^2 / d^2 by a short correctly rounded sequence (hardware
+ * reciprocal estimate, one Newton step, Markstein's correction) whenever
+ * d^2 lies in [2^-64, 2^62]; this counts, over every float bit pattern b in
+ * [lo, hi) within that range and each numerator a[0..na) (each 0 or in
+ * [2^-60, 2^60]), the quotients that differ from the IEEE division. */
+int sop_check_division(int device, unsigned int lo, unsigned int hi, const float* a, int na,
+                       unsigned long long* mismatches);
+
 /* The sop-gpu program (src/sop-gpu.cpp:18-88, runGPU in src/gsop.cu:111-163):
  * reads a parameter file in the reference's format, its .top topology and
  * PDB/XYZ coordinates, runs the configured protocol (Langevin or BDHI,

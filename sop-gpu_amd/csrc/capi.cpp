@@ -364,4 +364,32 @@ int sop_rng_probe(int device, const uint32_t* seeds4, int n, int count, int gaus
     return rc;
 }
 
+int sop_check_division(int device, unsigned int lo, unsigned int hi, const float* a, int na,
+                       unsigned long long* mismatches) {
+    if (!a || na <= 0 || !mismatches || hi < lo) return sop::fail(SOP_E_ARG, "bad arguments");
+    // the range the step kernel sends to the short division
+    const unsigned bmin = 0x1f800000u, bmax = 0x5e800000u;  // 2^-64, 2^62
+    if (lo < bmin || hi > bmax + 1) return sop::fail(SOP_E_ARG, "b outside [2^-64, 2^62]");
+    for (int k = 0; k < na; ++k)
+        if (!(a[k] == 0.0f || (a[k] >= 0x1p-60f && a[k] <= 0x1p60f)))
+            return sop::fail(SOP_E_ARG, "a outside {0} U [2^-60, 2^60]");
+    if (hipSetDevice(device) != hipSuccess) return sop::fail(SOP_E_HIP, "hipSetDevice failed");
+    float* d_a = nullptr;
+    unsigned long long* d_bad = nullptr;
+    if (hipMalloc(&d_a, (size_t)na * 4) != hipSuccess || hipMalloc(&d_bad, 8) != hipSuccess) {
+        (void)hipFree(d_a);
+        return sop::fail(SOP_E_HIP, "hipMalloc failed");
+    }
+    int rc = 0;
+    hipError_t err = hipMemcpy(d_a, a, (size_t)na * 4, hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = hipMemset(d_bad, 0, 8);
+    if (err == hipSuccess) err = sop::launch_division_check(lo, hi, d_a, na, d_bad, nullptr);
+    if (err == hipSuccess) err = hipDeviceSynchronize();
+    if (err == hipSuccess) err = hipMemcpy(mismatches, d_bad, 8, hipMemcpyDeviceToHost);
+    if (err != hipSuccess) rc = sop::fail(SOP_E_HIP, std::string("division check: ") + hipGetErrorString(err));
+    (void)hipFree(d_a);
+    (void)hipFree(d_bad);
+    return rc;
+}
+
 }  // extern "C"

@@ -31,6 +31,35 @@ __device__ __forceinline__ uint32_t hybrid_taus(uint4& s) {
     return a ^ b ^ c ^ s.w;
 }
 
+// a / b correctly rounded, in fewer operations than the general IEEE
+// division, for operands kept away from overflow and underflow: b in
+// [2^-64, 2^62] and a = 0 or a in [2^-60, 2^60] (kDivFast* below).  The
+// reciprocal y = RN(1/b) comes from the hardware estimate and one Newton
+// step; q = RN(a y) is then within one ulp of a/b, the residual a - b q is
+// exact in an FMA, and RN(q + (a - b q) y) = RN(a/b) by Markstein's theorem
+// (P. Markstein, IBM J. Res. Develop. 34 (1990) 111-119).  That the Newton
+// step yields RN(1/b) for every normal b is a property of the hardware
+// estimate, checked over all of them by sop_check_division.
+constexpr float kDivFastMinB = 0x1p-64f, kDivFastMaxB = 0x1p62f;
+constexpr float kDivFastMinA = 0x1p-60f, kDivFastMaxA = 0x1p60f;
+__device__ __forceinline__ float div_rn_fast(float a, float b) {
+    const float y0 = __builtin_amdgcn_rcpf(b);
+    const float y = __builtin_fmaf(__builtin_fmaf(-b, y0, 1.0f), y0, y0);
+    const float q = a * y;
+    return __builtin_fmaf(__builtin_fmaf(-b, q, a), y, q);
+}
+// The IEEE division, kept on its own rare branch: the divisor passes
+// through an empty asm, so the compiler cannot compute the quotient ahead of
+// the branch for every lane.
+__device__ __forceinline__ float div_ieee(float a, float b) {
+    asm volatile("" : "+v"(b));
+    return a / b;
+}
+// a can take the fast division (a wave-uniform test, made once per bead).
+__device__ __forceinline__ bool div_fast_numerator(float a) {
+    return a == 0.0f || (a >= kDivFastMinA && a <= kDivFastMaxA);
+}
+
 // Mantissa bits into [1,2), minus one; an exact zero becomes 1e-8.
 __device__ __forceinline__ float uint_to_unit(uint32_t u) {
     float f = __uint_as_float(0x3f800000u | (0x007fffffu & u)) - 1.0f;

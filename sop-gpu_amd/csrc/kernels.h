@@ -41,6 +41,10 @@ hipError_t launch_job_advance(const DevJob* jobs, DevJobState* st, int njobs, in
 hipError_t launch_energies(const DevJob* jobs, const DevJobState* st, int job, int n, hipStream_t stream);
 hipError_t launch_forces(const DevJob* jobs, const DevJobState* st, int job, int n, float4* out, hipStream_t stream);
 hipError_t launch_rng_probe(uint4* state, int n, int count, int gaussian, float* out, hipStream_t stream);
+// Counts (into *bad, device) the b bit patterns in [lo, hi) times numerators
+// a[0..na) (device) whose short division differs from the IEEE quotient.
+hipError_t launch_division_check(unsigned lo, unsigned hi, const float* a, int na, unsigned long long* bad,
+                                 hipStream_t stream);
 const void* step_kernel_symbol();
 hipError_t launch_copy(const float4* src, float4* dst, size_t n, hipStream_t stream);
 

@@ -82,12 +82,15 @@ template <typename IdxT>
 __device__ __forceinline__ void pair_loop(const __attribute__((address_space(1))) IdxT* __restrict__ pj, int np, int n, const float4* __restrict__ cin,
                                           const float4 c, float rc2, float a2, float m6, float& fx, float& fy,
                                           float& fz) {
+    // every d2 below rc2 and above kDivFastMinB takes the short division
+    const bool fast = div_fast_numerator(a2) && rc2 <= kDivFastMaxB;
 #define SOP_PAIR(o)                                                  \
     {                                                                \
         const float x = (o).x - c.x, y = (o).y - c.y, z = (o).z - c.z; \
         const float d2 = x * x + y * y + z * z;                      \
         if (d2 < rc2) {                                              \
-            float q = a2 / d2;                                       \
+            float q = div_rn_fast(a2, d2);                           \
+            if (!(fast && d2 >= kDivFastMinB)) q = div_ieee(a2, d2); \
             q = q * q;                                               \
             q = m6 * q * q;                                          \
             fx += x * q;                                             \
@@ -1313,6 +1316,26 @@ __global__ __launch_bounds__(256) void k_forces(const DevJob* __restrict__ jobs,
         fz += J.micaN[2] * sw + dfz;
     }
     out[i] = make_float4(fx, fy, fz, 0.0f);
+}
+
+// Self-check of div_rn_fast against the IEEE division: every float bit
+// pattern b in [lo, hi) (inside the fast range) with each numerator a[k];
+// counts the results that differ.
+__global__ void k_division_check(unsigned lo, unsigned hi, const float* __restrict__ a, int na,
+                                 unsigned long long* __restrict__ bad) {
+    unsigned long long mine = 0;
+    for (unsigned u = lo + blockIdx.x * blockDim.x + threadIdx.x; u < hi; u += gridDim.x * blockDim.x) {
+        const float b = __uint_as_float(u);
+        for (int k = 0; k < na; ++k)
+            mine += __float_as_uint(div_rn_fast(a[k], b)) != __float_as_uint(a[k] / b) ? 1ull : 0ull;
+    }
+    if (mine) atomicAdd(bad, mine);
+}
+
+hipError_t launch_division_check(unsigned lo, unsigned hi, const float* a, int na, unsigned long long* bad,
+                                 hipStream_t stream) {
+    hipLaunchKernelGGL(k_division_check, dim3(8192), dim3(256), 0, stream, lo, hi, a, na, bad);
+    return hipGetLastError();
 }
 
 // Advances `count` draws of the Hybrid Taus state of beads [0, n) and

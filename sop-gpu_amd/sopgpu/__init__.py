@@ -238,6 +238,7 @@ _sig("sop_server_stats", C.c_int, VP, C.POINTER(C.c_double), C.POINTER(C.c_doubl
 _sig("sop_copy_bandwidth", C.c_int, C.c_int, C.c_ulonglong, C.c_int, C.POINTER(C.c_double))
 _sig("sop_generate_seeds", C.c_int, C.c_int, C.c_int, VP)
 _sig("sop_rng_probe", C.c_int, C.c_int, VP, C.c_int, C.c_int, C.c_int, VP, VP)
+_sig("sop_check_division", C.c_int, C.c_int, C.c_uint, C.c_uint, VP, C.c_int, C.POINTER(C.c_ulonglong))
 _sig("sop_synth_chain", C.c_int, C.c_int, C.c_ulonglong, C.c_float, C.c_float, C.c_float, C.POINTER(SynthStruct))
 _sig("sop_synth_shell", C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.c_float, C.c_float, C.c_float,
      C.POINTER(SynthStruct))
@@ -258,7 +259,7 @@ EXPORTED = [
     "sop_engine_list_stats", "sop_engine_set_chol_mode", "sop_engine_get_hi", "sop_engine_get_hi_factor",
     "sop_chol_factor", "sop_server_create", "sop_server_destroy", "sop_server_set_tuning", "sop_server_submit",
     "sop_server_pump", "sop_server_poll", "sop_server_result", "sop_server_stats", "sop_copy_bandwidth",
-    "sop_generate_seeds", "sop_rng_probe", "sop_synth_chain", "sop_synth_shell",
+    "sop_generate_seeds", "sop_rng_probe", "sop_check_division", "sop_synth_chain", "sop_synth_shell",
     "sop_free_system", "sop_run_config", "sop_read_topology", "sop_write_pdb", "sop_write_dcd",
 ]
 
@@ -347,6 +348,15 @@ def rng_probe(seeds: np.ndarray, count: int, gaussian: bool, device: int = 0):
     st = np.zeros((n, 4), np.uint32)
     check(lib.sop_rng_probe(device, _ptr(seeds), n, count, int(gaussian), _ptr(out), _ptr(st)))
     return out, st
+
+
+def check_division(lo: int, hi: int, a, device: int = 0) -> int:
+    """Mismatches of the step kernel's short division against IEEE a/b over
+    the float bit patterns b in [lo, hi) and the numerators a."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    bad = C.c_ulonglong(0)
+    check(lib.sop_check_division(device, lo, hi, _ptr(a), len(a), C.byref(bad)))
+    return int(bad.value)
 
 
 class Engine:

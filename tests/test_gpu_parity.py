@@ -78,6 +78,20 @@ def test_possible_pairs_with_many_exclusions(eng):
         eng.remove_job(jid)
 
 
+def test_short_division_equals_ieee_over_all_divisors():
+    """The repulsive-pair kernel's short division (hardware reciprocal
+    estimate, one Newton step, Markstein's correction) against IEEE a/b:
+    every float divisor b in [2^-64, 2^62] (all 1.06e9 bit patterns) with
+    64 numerators: 0, 1, sigma^2 = 3.8^2 and 4^2, the ends of the numerator
+    range, and log-uniform random ones."""
+    rng = np.random.default_rng(77)
+    a = [0.0, 1.0, np.float32(3.8) * np.float32(3.8), 16.0, 2.0 ** -60, 2.0 ** 60]
+    a += list(np.exp2(rng.uniform(-60, 60, size=58)))
+    a = np.array(a, np.float32)
+    a = np.clip(a, np.float32(2.0 ** -60), np.float32(2.0 ** 60)) * (a != 0)
+    assert S.check_division(0x1F800000, 0x5E800001, a) == 0
+
+
 def test_possible_pairs_more_than_64_exclusions(eng):
     """Beads with more than 64 exclusions (the list scan takes them 64 at a
     time): bead 5 and bead 250 get ~100 extra native contacts each."""
