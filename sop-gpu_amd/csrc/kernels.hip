@@ -401,7 +401,7 @@ __global__ __launch_bounds__(256) void k_possible(const DevJob* __restrict__ job
                     (unsigned)__builtin_amdgcn_readlane(exlo, b);
                 const float x = cj.x - xi, y = cj.y - yi, z = cj.z - zi;
                 const bool ok = j < n && j != i0 + b && !((exb >> lane) & 1ull) && x * x + y * y + z * z <= d2max;
-                const unsigned long long m = __ballot(ok);
+                const unsigned long long m = __builtin_amdgcn_ballot_w64(ok);
                 const int c0 = __builtin_amdgcn_readlane(cnt, b);
                 if (ok) {
                     const int pos = c0 + __popcll(m & lanes_below(lane));
@@ -499,7 +499,7 @@ __global__ __launch_bounds__(1024) void k_possible_lds(const DevJob* __restrict_
                 const float x = c.x - ci.x, y = c.y - ci.y, z = c.z - ci.z;
                 bool acc = x * x + y * y + z * z <= d2max;
                 if (tail) acc = acc & (base + lane < n);
-                unsigned long long m = __ballot(acc);
+                unsigned long long m = __builtin_amdgcn_ballot_w64(acc);
                 if (min(e_next, self) < lim) {
                     unsigned long long drop = 0ull;
                     while (e_next < lim) {
@@ -900,14 +900,16 @@ __device__ __forceinline__ void verlet_tile(const DevJob& J, DevJobState& S, con
                 const int k0 = base + 64 * (h + u);
                 if (k0 < npp) {
                     const float x = c[u].x - r1.x, y = c[u].y - r1.y, z = c[u].z - r1.z;
-                    const bool ok = k0 + lane < npp && x * x + y * y + z * z <= d2max;
-                    const unsigned long long m = __ballot(ok);
+                    const bool ok = (k0 + lane < npp) & (x * x + y * y + z * z <= d2max);
+                    const unsigned long long m = __builtin_amdgcn_ballot_w64(ok);
+                    const int total = cnt + __popcll(m);
                     if (ok) {
-                        const int pos = cnt + __popcll(m & lanes_below(lane));
-                        if (pos < E) rows[b * RS + pos] = (IdxT)j[h + u];
+                        const int pos = (int)__builtin_amdgcn_mbcnt_hi(
+                            (unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, (unsigned)cnt));
+                        if (total <= E || pos < E) rows[b * RS + pos] = (IdxT)j[h + u];
                         else if (pos < cap) pj[(size_t)pos * n + i0 + b] = (IdxT)j[h + u];
                     }
-                    cnt += __popcll(m);
+                    cnt = total;
                 }
             }
         }
