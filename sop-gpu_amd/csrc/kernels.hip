@@ -303,7 +303,7 @@ __global__ __launch_bounds__(256) void k_step(const DevJob* __restrict__ jobs, c
 // coordinates in LDS (coalesced), so every neighbour gather of the step
 // reads LDS.  Segments start at multiples of 64 threads: no wavefront spans
 // two jobs.
-__global__ __launch_bounds__(1024) void k_step_lds(const DevJob* __restrict__ jobs, const DevJobState* __restrict__ st,
+__global__ __launch_bounds__(1024, 8) void k_step_lds(const DevJob* __restrict__ jobs, const DevJobState* __restrict__ st,
                                                    const int2* __restrict__ units, const int4* __restrict__ segs,
                                                    int nunits, int kstep) {
     extern __shared__ float4 lds_coord[];
