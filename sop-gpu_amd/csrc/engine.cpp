@@ -921,19 +921,23 @@ int Engine::launch_block(int slot_only) {
     if (int rc = check(launch_job_update(jobs, st, nj, stream_), "job update")) return rc;
     const int cw = slot_only >= 0 ? cell_words_of(jobs_[slot_only].dj) : cell_words_;
     if (int rc = check(slot_only >= 0
-                           ? launch_possible(jobs, st, tiles, nt, nj, cw, d_sunits1_, d_segs1_,
+                           ? launch_possible(jobs, st, tiles, nt, nj, cw, !jobs_[slot_only].dj.lds_step, d_sunits1_,
+                                             d_segs1_,
                                              jobs_[slot_only].dj.lds_step ? nsunits1_ : 0, jobs_[slot_only].n, 0,
                                              stream_)
-                           : launch_possible(jobs, st, tiles, nt, nj, cw, d_sunits_, d_segs_, nsunits_, lds_beads_, 0,
+                           : launch_possible(jobs, st, tiles, nt, nj, cw, any_global_, d_sunits_, d_segs_, nsunits_,
+                                             lds_beads_, 0,
                                              stream_),
                        "possible pairs"))
         return rc;
     const int i32 = slot_only >= 0 ? !jobs_[slot_only].dj.idx16 : any_idx32_;
     const int nu1 = slot_only >= 0 && jobs_[slot_only].dj.sort_rows ? (jobs_[slot_only].n + kLdsUnit - 1) / kLdsUnit : 0;
     if (int rc = check(slot_only >= 0
-                           ? launch_verlet(jobs, st, tiles, nt, i32, d_units1_, nu1, d_sunits1_, d_segs1_,
+                           ? launch_verlet(jobs, st, tiles, nt, i32, !jobs_[slot_only].dj.lds_step, d_units1_, nu1,
+                                           d_sunits1_, d_segs1_,
                                            jobs_[slot_only].dj.lds_step ? nsunits1_ : 0, jobs_[slot_only].n, 0, stream_)
-                           : launch_verlet(jobs, st, tiles, nt, i32, d_units_, nunits_, d_sunits_, d_segs_, nsunits_,
+                           : launch_verlet(jobs, st, tiles, nt, i32, any_global_, d_units_, nunits_, d_sunits_,
+                                           d_segs_, nsunits_,
                                            lds_beads_, 0, stream_),
                        "verlet"))
         return rc;
@@ -1052,13 +1056,15 @@ int Engine::stage_job_update(int mask) {
 }
 int Engine::stage_possible() {
     if (int rc = sync_tables()) return rc;
-    return check(launch_possible(d_jobs_, d_state_, d_tiles_, ntiles_, (int)jobs_.size(), cell_words_, d_sunits_,
+    return check(launch_possible(d_jobs_, d_state_, d_tiles_, ntiles_, (int)jobs_.size(), cell_words_, any_global_,
+                                 d_sunits_,
                                  d_segs_, nsunits_, lds_beads_, 0, stream_),
                  "possible pairs");
 }
 int Engine::stage_verlet() {
     if (int rc = sync_tables()) return rc;
-    return check(launch_verlet(d_jobs_, d_state_, d_tiles_, ntiles_, any_idx32_, d_units_, nunits_, d_sunits_, d_segs_,
+    return check(launch_verlet(d_jobs_, d_state_, d_tiles_, ntiles_, any_idx32_, any_global_, d_units_, nunits_,
+                               d_sunits_, d_segs_,
                                nsunits_, lds_beads_, 0, stream_),
                  "verlet");
 }
@@ -1361,12 +1367,14 @@ int Engine::rebuild_lists(int id, int mask) {
     if (int rc = tiles_for_slot(slot, &nt)) return rc;
     if (mask & 1)
         if (int rc = check(launch_possible(d_jobs_ + slot, d_state_ + slot, d_tiles1_, nt, 1,
-                                           cell_words_of(jobs_[slot].dj), d_sunits1_, d_segs1_,
+                                           cell_words_of(jobs_[slot].dj), !jobs_[slot].dj.lds_step, d_sunits1_,
+                                           d_segs1_,
                                            jobs_[slot].dj.lds_step ? nsunits1_ : 0, jobs_[slot].n, 1, stream_),
                            "possible pairs"))
             return rc;
     if (mask & 2)
         if (int rc = check(launch_verlet(d_jobs_ + slot, d_state_ + slot, d_tiles1_, nt, !jobs_[slot].dj.idx16,
+                                         !jobs_[slot].dj.lds_step,
                                          d_units1_,
                                          jobs_[slot].dj.sort_rows ? (jobs_[slot].n + kLdsUnit - 1) / kLdsUnit : 0,
                                          d_sunits1_, d_segs1_, jobs_[slot].dj.lds_step ? nsunits1_ : 0,

@@ -20,20 +20,21 @@ hipError_t launch_step(const DevJob* jobs, const DevJobState* st, const int2* ti
 // 0 to build every job's possible list by the all-pairs scan.
 // sunits/segs/nsunits/lds_beads: the LDS step plan (see launch_step), whose
 // jobs get their possible lists scanned from LDS-staged coordinates
-// (nsunits = 0: all jobs by the tile kernels).
+// (nsunits = 0: all jobs by the tile kernels); any_global: some job is
+// outside that plan (the tile kernel is launched only then).
 hipError_t launch_possible(const DevJob* jobs, DevJobState* st, const int2* tiles, int ntiles, int njobs,
-                           int cell_words, const int2* sunits, const int4* segs, int nsunits, int lds_beads,
-                           int force, hipStream_t stream);
+                           int cell_words, int any_global, const int2* sunits, const int4* segs, int nsunits,
+                           int lds_beads, int force, hipStream_t stream);
 // Largest cell_words launch_possible accepts (the LDS bitmap of one wavefront).
 constexpr int kMaxCellWords = 8192;
 // any_idx32: some job of the launch stores its lists as int32 (n >= 65536).
 // units (job, first row) of the jobs whose rows are re-sorted first;
 // sunits/segs/nsunits/lds_beads: the LDS step plan (see launch_step), whose
 // jobs get their lists built from LDS-staged coordinates (nsunits = 0: all
-// jobs by the tile kernel).
+// jobs by the tile kernel); any_global: some job is outside that plan.
 hipError_t launch_verlet(const DevJob* jobs, DevJobState* st, const int2* tiles, int ntiles, int any_idx32,
-                         const int2* units, int nunits, const int2* sunits, const int4* segs, int nsunits,
-                         int lds_beads, int force, hipStream_t stream);
+                         int any_global, const int2* units, int nunits, const int2* sunits, const int4* segs,
+                         int nsunits, int lds_beads, int force, hipStream_t stream);
 // Per-job updaters at a block start, and the step counter advance.
 hipError_t launch_job_update(const DevJob* jobs, DevJobState* st, int njobs, hipStream_t stream, int mask = 7);
 hipError_t launch_job_advance(const DevJob* jobs, DevJobState* st, int njobs, int nsteps, hipStream_t stream);

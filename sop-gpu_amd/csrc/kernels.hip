@@ -1395,12 +1395,13 @@ hipError_t launch_step(const DevJob* jobs, const DevJobState* st, const int2* ti
 }
 
 hipError_t launch_possible(const DevJob* jobs, DevJobState* st, const int2* tiles, int ntiles, int njobs,
-                           int cell_words, const int2* sunits, const int4* segs, int nsunits, int lds_beads,
-                           int force, hipStream_t stream) {
+                           int cell_words, int any_global, const int2* sunits, const int4* segs, int nsunits,
+                           int lds_beads, int force, hipStream_t stream) {
     if (ntiles <= 0) return hipSuccess;
     const int use_cells = cell_words > 0;
-    hipLaunchKernelGGL(k_possible, dim3(grid_for(ntiles, 0)), dim3(256), 0, stream, jobs, st, tiles, ntiles, force,
-                       use_cells, nsunits > 0);
+    if (any_global || nsunits <= 0)
+        hipLaunchKernelGGL(k_possible, dim3(grid_for(ntiles, 0)), dim3(256), 0, stream, jobs, st, tiles, ntiles,
+                           force, use_cells, nsunits > 0);
     if (nsunits > 0)
         hipLaunchKernelGGL(k_possible_lds, dim3(nsunits), dim3(kLdsUnit), (size_t)lds_beads * sizeof(float4), stream,
                            jobs, st, sunits, segs, nsunits, force, use_cells);
@@ -1420,15 +1421,17 @@ hipError_t launch_possible(const DevJob* jobs, DevJobState* st, const int2* tile
 }
 
 hipError_t launch_verlet(const DevJob* jobs, DevJobState* st, const int2* tiles, int ntiles, int any_idx32,
-                         const int2* units, int nunits, const int2* sunits, const int4* segs, int nsunits,
-                         int lds_beads, int force, hipStream_t stream) {
+                         int any_global, const int2* units, int nunits, const int2* sunits, const int4* segs,
+                         int nsunits, int lds_beads, int force, hipStream_t stream) {
     if (ntiles <= 0) return hipSuccess;
     if (nunits > 0) hipLaunchKernelGGL(k_sort_rows, dim3(nunits), dim3(kLdsUnit), 0, stream, jobs, st, units, nunits, force);
     int blocks = ntiles;
     if (blocks >= 8) blocks = (blocks + 7) / 8 * 8;
     const size_t lds = any_idx32 ? 64 * VerletLds<int>::stride * sizeof(int)
                                  : 64 * VerletLds<unsigned short>::stride * sizeof(unsigned short);
-    hipLaunchKernelGGL(k_verlet, dim3(blocks), dim3(256), lds, stream, jobs, st, tiles, ntiles, force, nsunits > 0);
+    if (any_global || nsunits <= 0)
+        hipLaunchKernelGGL(k_verlet, dim3(blocks), dim3(256), lds, stream, jobs, st, tiles, ntiles, force,
+                           nsunits > 0);
     if (nsunits > 0)
         hipLaunchKernelGGL(k_verlet_lds, dim3(nsunits), dim3(1024), 2 * (size_t)kVerletRowsBytes + (size_t)lds_beads * 16,
                            stream, jobs, st, sunits, segs, nsunits, force);
