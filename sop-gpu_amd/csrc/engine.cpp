@@ -474,7 +474,7 @@ static unsigned long long built_bytes(const BuiltJob& b, bool pulling, bool inde
     t += round_up(n * b.cov_cap * 4) * 2; // cov j, r0
     t += round_up(n * b.nat_cap * 4) + round_up(n * b.nat_cap * 8);
     const unsigned long long es = b.dj.idx16 ? 2 : 4;
-    t += round_up(n * b.pair_cap * es) + round_up(n * b.pp_cap * es) + round_up(n * b.excl_cap * 4);
+    t += round_up((n + 63) / 64 * 64 * b.pair_cap * es) + round_up(n * b.pp_cap * es) + round_up(n * b.excl_cap * 4);
     t += round_up(std::max<size_t>(1, b.pulled.size()) * 4);
     t += round_up(n * 12);                // energies
     if (n >= (unsigned long long)kCellMinBeads && n <= 64ull * kMaxCellWords)
@@ -530,7 +530,8 @@ int Engine::add_job(const sop_params& p, const sop_system& s, long long numsteps
     J.pair_cnt = (int*)A(n * 4);
     J.perm = (int*)A(n * 4);
     const size_t es = J.idx16 ? 2 : 4;
-    J.pair_j = A(n * b.pair_cap * es);
+    J.pair_ld = (int)((n + 63) / 64 * 64);
+    J.pair_j = A((size_t)J.pair_ld * b.pair_cap * es);
     J.pp_cnt = (int*)A(n * 4);
     J.pp_j = A(n * b.pp_cap * es);
     J.excl_cnt = (int*)A(n * 4);
@@ -1265,13 +1266,14 @@ int Engine::get_list(int id, int which, int* counts, int* entries) {
     default: return fail(SOP_E_ARG, "unknown list");
     }
     if (int rc = check(hipMemcpy(counts, c, n * 4, hipMemcpyDeviceToHost), "list counts")) return rc;
-    std::vector<int> words(n * cap);
+    const size_t ld = which == 2 ? (size_t)J.pair_ld : n;  // slot stride on the device
+    std::vector<int> words(ld * cap);
     if (typed && J.idx16) {
-        std::vector<unsigned short> w16(n * cap);
-        if (int rc = check(hipMemcpy(w16.data(), e, n * cap * 2, hipMemcpyDeviceToHost), "list entries")) return rc;
+        std::vector<unsigned short> w16(ld * cap);
+        if (int rc = check(hipMemcpy(w16.data(), e, ld * cap * 2, hipMemcpyDeviceToHost), "list entries")) return rc;
         words.assign(w16.begin(), w16.end());
     } else {
-        if (int rc = check(hipMemcpy(words.data(), e, n * cap * 4, hipMemcpyDeviceToHost), "list entries")) return rc;
+        if (int rc = check(hipMemcpy(words.data(), e, ld * cap * 4, hipMemcpyDeviceToHost), "list entries")) return rc;
     }
     if (which == 2) {  // row r holds bead perm[r]'s list
         std::vector<int> perm(n), rc(counts, counts + n);
@@ -1279,7 +1281,7 @@ int Engine::get_list(int id, int which, int* counts, int* entries) {
         for (size_t r = 0; r < n; ++r) {
             const size_t i = (size_t)perm[r];
             counts[i] = rc[r];
-            for (int k = 0; k < cap; ++k) entries[(size_t)k * n + i] = words[(size_t)k * n + r];
+            for (int k = 0; k < cap; ++k) entries[(size_t)k * n + i] = words[(size_t)k * ld + r];
         }
         return 0;
     }

@@ -144,7 +144,8 @@ __global__ __launch_bounds__(1024) void k_hi_eps(const DevJob* __restrict__ jobs
         if (w != 0 || d >= n) return;
         for (int k = 0; k < J.cov_cnt[d]; ++k) eps_add(c, cur[J.cov_j[(size_t)k * n + d]], a, sum);
         for (int k = 0; k < J.nat_cnt[d]; ++k) eps_add(c, cur[J.nat_j[(size_t)k * n + d]], a, sum);
-        for (int k = 0; k < J.pair_cnt[d]; ++k) eps_add(c, cur[list_at(J.pair_j, J.idx16, (size_t)k * n + d)], a, sum);
+        for (int k = 0; k < J.pair_cnt[d]; ++k)
+            eps_add(c, cur[list_at(J.pair_j, J.idx16, (size_t)k * J.pair_ld + d)], a, sum);
         J.ci[d] = sum;
         J.eps[d] = sum.w;
         return;
@@ -279,7 +280,7 @@ __global__ __launch_bounds__(1024) void k_hi_tea(const DevJob* __restrict__ jobs
             fx += tx; fy += ty; fz += tz;
         }
         for (int k = 0; k < J.pair_cnt[d]; ++k) {
-            const int i = list_at(J.pair_j, J.idx16, (size_t)k * n + d);
+            const int i = list_at(J.pair_j, J.idx16, (size_t)k * J.pair_ld + d);
             tx = 0.f; ty = 0.f; tz = 0.f;
             tea_force(coord, cur[i], J.mforce[i], J.rforce[i], ci0, ci1, ci2, a, tx, ty, tz);
             fx += tx; fy += ty; fz += tz;

@@ -71,6 +71,7 @@ struct DevJob {
     int* pulled;    // bead indices of pulled beads
     float* energies; // 3n per-bead energies (output cadence)
     int cov_cap, nat_cap, pair_cap, pp_cap, excl_cap;
+    int pair_ld;    // Verlet slot stride: n rounded up to 64 rows (entry k of row r at [k*pair_ld + r])
     int idx16;
     int npulled;
     // force-field constants, formed as the reference's constructors do

@@ -78,6 +78,9 @@ __device__ __forceinline__ __attribute__((address_space(1))) T* gmem(T* p) {
 // list words are streamed with non-temporal loads (each is used once per
 // step) and fetched one batch of eight ahead: while batch b is computed the
 // indices of batch b+1 and the coordinates of batch b are in flight.
+#ifndef SOP_PAIR_BATCH
+#define SOP_PAIR_BATCH 6
+#endif
 template <typename IdxT>
 __device__ __forceinline__ void pair_loop(const __attribute__((address_space(1))) IdxT* __restrict__ pj, int np, int n, const float4* __restrict__ cin,
                                           const float4 c, float rc2, float a2, float m6, float& fx, float& fy,
@@ -98,21 +101,22 @@ __device__ __forceinline__ void pair_loop(const __attribute__((address_space(1))
             fz += z * q;                                             \
         }                                                            \
     }
+    constexpr int PB = SOP_PAIR_BATCH;  // entries per batch
     int k = 0;
-    if (np >= 8) {
-        int jn[8];
+    if (np >= PB) {
+        int jn[PB];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) jn[u] = (int)__builtin_nontemporal_load(pj + (size_t)u * n);
-        for (; k + 8 <= np; k += 8) {
-            float4 o[8];
+        for (int u = 0; u < PB; ++u) jn[u] = (int)__builtin_nontemporal_load(pj + (size_t)u * n);
+        for (; k + PB <= np; k += PB) {
+            float4 o[PB];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) o[u] = cin[jn[u]];
-            if (k + 16 <= np) {
+            for (int u = 0; u < PB; ++u) o[u] = cin[jn[u]];
+            if (k + 2 * PB <= np) {
 #pragma unroll
-                for (int u = 0; u < 8; ++u) jn[u] = (int)__builtin_nontemporal_load(pj + (size_t)(k + 8 + u) * n);
+                for (int u = 0; u < PB; ++u) jn[u] = (int)__builtin_nontemporal_load(pj + (size_t)(k + PB + u) * n);
             }
 #pragma unroll
-            for (int u = 0; u < 8; ++u) SOP_PAIR(o[u]);
+            for (int u = 0; u < PB; ++u) SOP_PAIR(o[u]);
         }
     }
     for (; k < np; ++k) {
@@ -192,9 +196,11 @@ __device__ __forceinline__ void step_bead(const DevJob& J, const DevJobState& S,
     // repulsive LJ over the Verlet list (pairs_kernel.cu:41-56)
     const int np = gmem(J.pair_cnt)[r];
     if (J.idx16)
-        pair_loop(gmem(static_cast<const unsigned short*>(J.pair_j)) + r, np, n, cin, c, J.rc2, J.a2, J.m6, fx, fy, fz);
+        pair_loop(gmem(static_cast<const unsigned short*>(J.pair_j)) + r, np, J.pair_ld, cin, c, J.rc2, J.a2, J.m6,
+                  fx, fy, fz);
     else
-        pair_loop(gmem(static_cast<const int*>(J.pair_j)) + r, np, n, cin, c, J.rc2, J.a2, J.m6, fx, fy, fz);
+        pair_loop(gmem(static_cast<const int*>(J.pair_j)) + r, np, J.pair_ld, cin, c, J.rc2, J.a2, J.m6, fx, fy,
+                  fz);
     // indentation tip and substrate plane (indentation_kernel.cu:71-128)
     if (J.flags & JF_INDENTATION) {
         float4 fin = J.tipF[i];
@@ -907,7 +913,7 @@ __device__ __forceinline__ void verlet_tile(const DevJob& J, DevJobState& S, con
                         const int pos = (int)__builtin_amdgcn_mbcnt_hi(
                             (unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, (unsigned)cnt));
                         if (total <= E || pos < E) rows[b * RS + pos] = (IdxT)j[h + u];
-                        else if (pos < cap) pj[(size_t)pos * n + i0 + b] = (IdxT)j[h + u];
+                        else if (pos < cap) pj[(size_t)pos * J.pair_ld + i0 + b] = (IdxT)j[h + u];
                     }
                     cnt = total;
                 }
@@ -984,7 +990,7 @@ __device__ __forceinline__ void verlet_store(const DevJob& J, int i0, const IdxT
     const auto pj = gmem(static_cast<IdxT*>(J.pair_j));
     const int lim = min(c, E);
     for (int k = kq; k < maxc; k += WAVES)
-        if (k < lim) pj[(size_t)k * n + i0 + b] = rows[b * RS + k];
+        if (k < lim) pj[(size_t)k * J.pair_ld + i0 + b] = rows[b * RS + k];
 }
 
 // Re-sorts the rows of each kLdsUnit-row unit of a job by Verlet list
@@ -1230,7 +1236,7 @@ __global__ __launch_bounds__(256) void k_energies(const DevJob* __restrict__ job
     J.energies[n + i] = pot;
     pot = 0.0f;
     for (int k = 0; k < J.pair_cnt[r]; ++k) {
-        const int j = list_at(J.pair_j, J.idx16, (size_t)k * n + r);
+        const int j = list_at(J.pair_j, J.idx16, (size_t)k * J.pair_ld + r);
         const float4 o = cur[j];
         const float x = o.x - c.x, y = o.y - c.y, z = o.z - c.z;
         const float d2 = x * x + y * y + z * z;
@@ -1280,7 +1286,7 @@ __global__ __launch_bounds__(256) void k_forces(const DevJob* __restrict__ jobs,
         fz += z * q;
     }
     for (int k = 0; k < J.pair_cnt[r]; ++k) {
-        const int j = list_at(J.pair_j, J.idx16, (size_t)k * n + r);
+        const int j = list_at(J.pair_j, J.idx16, (size_t)k * J.pair_ld + r);
         const float4 o = cin[j];
         const float x = o.x - c.x, y = o.y - c.y, z = o.z - c.z;
         const float d2 = x * x + y * y + z * z;

@@ -14,7 +14,9 @@ from typing import Optional
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "libsopgpu.so")
+# SOPGPU_LIB names another build of the library (e.g. one compiled with
+# different tile parameters); by default the in-tree build is loaded.
+LIB_PATH = os.environ.get("SOPGPU_LIB") or os.path.join(os.path.dirname(_HERE), "libsopgpu.so")
 
 
 class SopError(RuntimeError):
