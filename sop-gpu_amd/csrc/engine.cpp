@@ -189,7 +189,7 @@ int Engine::slot_of(int id) const {
 // pairlist.cu:41-57, pulling.cu:36-146, indentation.cu:84-200).
 struct BuiltJob {
     int n = 0;
-    int cov_cap = 1, nat_cap = 1, pair_cap = 1, pp_cap = 1, excl_cap = 1;
+    int cov_cap = 1, nat_cap = 1, pair_cap = 1, pp_cap = 1, excl_cap = 1, pp_ld = 64;
     std::vector<int> cov_cnt, cov_j, nat_cnt, nat_j, pp_cnt, pp_j, excl_cnt, excl_j, pulled;
     std::vector<float> cov_r0;
     std::vector<float2> nat_par;
@@ -289,10 +289,11 @@ static int build_job(const sop_params& p, const sop_system& s, long long numstep
                                          std::to_string(p.max_possiblePairs) + ".");
     }
     for (int i = 0; i < n; ++i) b.pp_cap = std::max(b.pp_cap, (int)pp[i].size());
-    b.pp_j.assign((size_t)b.pp_cap * n, 0);
+    b.pp_ld = (b.pp_cap + 63) / 64 * 64;  // rows start on 128-byte lines
+    b.pp_j.assign((size_t)b.pp_ld * n, 0);
     for (int i = 0; i < n; ++i) {
         b.pp_cnt[i] = (int)pp[i].size();
-        for (size_t k = 0; k < pp[i].size(); ++k) b.pp_j[(size_t)i * b.pp_cap + k] = pp[i][k];
+        for (size_t k = 0; k < pp[i].size(); ++k) b.pp_j[(size_t)i * b.pp_ld + k] = pp[i][k];
     }
     // coordinates, w = 0 (gsop.cu:196)
     b.coord.resize(n);
@@ -313,6 +314,7 @@ static int build_job(const sop_params& p, const sop_system& s, long long numstep
     J.nat_cap = b.nat_cap;
     J.pair_cap = b.pair_cap;
     J.pp_cap = b.pp_cap;
+    J.pp_ld = b.pp_ld;
     J.excl_cap = b.excl_cap;
     J.ks = p.kspring_cov;
     J.R2 = p.R_limit * p.R_limit;
@@ -474,7 +476,7 @@ static unsigned long long built_bytes(const BuiltJob& b, bool pulling, bool inde
     t += round_up(n * b.cov_cap * 4) * 2; // cov j, r0
     t += round_up(n * b.nat_cap * 4) + round_up(n * b.nat_cap * 8);
     const unsigned long long es = b.dj.idx16 ? 2 : 4;
-    t += round_up((n + 63) / 64 * 64 * b.pair_cap * es) + round_up(n * b.pp_cap * es) + round_up(n * b.excl_cap * 4);
+    t += round_up((n + 63) / 64 * 64 * b.pair_cap * es) + round_up(n * b.pp_ld * es) + round_up(n * b.excl_cap * 4);
     t += round_up(std::max<size_t>(1, b.pulled.size()) * 4);
     t += round_up(n * 12);                // energies
     if (n >= (unsigned long long)kCellMinBeads && n <= 64ull * kMaxCellWords)
@@ -533,7 +535,7 @@ int Engine::add_job(const sop_params& p, const sop_system& s, long long numsteps
     J.pair_ld = (int)((n + 63) / 64 * 64);
     J.pair_j = A((size_t)J.pair_ld * b.pair_cap * es);
     J.pp_cnt = (int*)A(n * 4);
-    J.pp_j = A(n * b.pp_cap * es);
+    J.pp_j = A(n * b.pp_ld * es);
     J.excl_cnt = (int*)A(n * 4);
     J.excl_j = (int*)A(n * b.excl_cap * 4);
     J.pulled = (int*)A(std::max<size_t>(1, b.pulled.size()) * 4);
@@ -1266,14 +1268,17 @@ int Engine::get_list(int id, int which, int* counts, int* entries) {
     default: return fail(SOP_E_ARG, "unknown list");
     }
     if (int rc = check(hipMemcpy(counts, c, n * 4, hipMemcpyDeviceToHost), "list counts")) return rc;
-    const size_t ld = which == 2 ? (size_t)J.pair_ld : n;  // slot stride on the device
-    std::vector<int> words(ld * cap);
+    // device layout: slot-major [k*ld + r] (Verlet: ld = pair_ld; covalent,
+    // native: ld = n) or bead-major [i*ld + k] (possible: ld = pp_ld)
+    const size_t ld = which == 2 ? (size_t)J.pair_ld : which == 3 ? (size_t)J.pp_ld : n;
+    const size_t nw = which == 3 ? n * ld : ld * cap;
+    std::vector<int> words(nw);
     if (typed && J.idx16) {
-        std::vector<unsigned short> w16(ld * cap);
-        if (int rc = check(hipMemcpy(w16.data(), e, ld * cap * 2, hipMemcpyDeviceToHost), "list entries")) return rc;
+        std::vector<unsigned short> w16(nw);
+        if (int rc = check(hipMemcpy(w16.data(), e, nw * 2, hipMemcpyDeviceToHost), "list entries")) return rc;
         words.assign(w16.begin(), w16.end());
     } else {
-        if (int rc = check(hipMemcpy(words.data(), e, ld * cap * 4, hipMemcpyDeviceToHost), "list entries")) return rc;
+        if (int rc = check(hipMemcpy(words.data(), e, nw * 4, hipMemcpyDeviceToHost), "list entries")) return rc;
     }
     if (which == 2) {  // row r holds bead perm[r]'s list
         std::vector<int> perm(n), rc(counts, counts + n);
@@ -1291,7 +1296,7 @@ int Engine::get_list(int id, int which, int* counts, int* entries) {
     }
     // the possible list is kept bead-major on the device
     for (size_t i = 0; i < n; ++i)
-        for (int k = 0; k < cap; ++k) entries[(size_t)k * n + i] = words[i * cap + k];
+        for (int k = 0; k < cap; ++k) entries[(size_t)k * n + i] = words[i * ld + k];
     return 0;
 }
 

@@ -55,7 +55,7 @@ struct DevJob {
     int* pair_cnt;  // per row
     void* pair_j;
     int* pp_cnt;
-    void* pp_j;     // bead-major: entry k of bead i at [i*pp_cap + k]
+    void* pp_j;     // bead-major: entry k of bead i at [i*pp_ld + k]
     int* excl_cnt;  // sorted covalent+native partners (possible-pair exclusions)
     int* excl_j;
     // cell grid of the possible-pairs build, for jobs of kCellMinBeads
@@ -72,6 +72,7 @@ struct DevJob {
     float* energies; // 3n per-bead energies (output cadence)
     int cov_cap, nat_cap, pair_cap, pp_cap, excl_cap;
     int pair_ld;    // Verlet slot stride: n rounded up to 64 rows (entry k of row r at [k*pair_ld + r])
+    int pp_ld;      // possible-list row stride: pp_cap rounded up to 64 entries (entry k of bead i at [i*pp_ld + k])
     int idx16;
     int npulled;
     // force-field constants, formed as the reference's constructors do

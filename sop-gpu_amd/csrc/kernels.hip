@@ -411,7 +411,7 @@ __global__ __launch_bounds__(256) void k_possible(const DevJob* __restrict__ job
                 const int c0 = __builtin_amdgcn_readlane(cnt, b);
                 if (ok) {
                     const int pos = c0 + __popcll(m & lanes_below(lane));
-                    if (pos < cap) list_put(J.pp_j, J.idx16, (size_t)(i0 + b) * cap + pos, j);
+                    if (pos < cap) list_put(J.pp_j, J.idx16, (size_t)(i0 + b) * J.pp_ld + pos, j);
                 }
                 if (lane == b) cnt = c0 + __popcll(m);
             }
@@ -492,7 +492,7 @@ __global__ __launch_bounds__(1024) void k_possible_lds(const DevJob* __restrict_
             int ev = lane < ne ? ev_n : INT_MAX;
             load_ex(min(i + 1, r1 - 1), ne_n, ev_n);
             const float4 ci = cur[i];
-            const auto row = pp + (size_t)i * cap;
+            const auto row = pp + (size_t)i * J.pp_ld;
             int eg = 0, pe = 0;  // exclusion group (of 64) and index within it
             int e_next = ne > 0 ? __builtin_amdgcn_readlane(ev, 0) : INT_MAX;
             int self = i;  // the bead itself, until its chunk is passed
@@ -805,7 +805,7 @@ __global__ __launch_bounds__(64) void k_possible_cells(const DevJob* __restrict_
             jlo = wave_min(jlo);
             jhi = wave_max(jhi);
             int cnt = 0;
-            const size_t rowp = (size_t)i * cap;
+            const size_t rowp = (size_t)i * J.pp_ld;
             if (jhi >= 0) {
                 const int wlo = jlo >> 6, whi = jhi >> 6;
                 for (int wb = wlo; wb <= whi; wb += 64) {
@@ -869,7 +869,7 @@ __device__ __forceinline__ void verlet_tile(const DevJob& J, DevJobState& S, con
     w = __builtin_amdgcn_readfirstlane(w);
     const int b_lo = RPW * w, b_hi = min(b_lo + RPW, nrow);
     const float d2max = J.pl_d2;  // sqrtf(d2) < pl_cut, exactly
-    const int cap = J.pair_cap, ppcap = J.pp_cap;
+    const int cap = J.pair_cap, ppcap = J.pp_cap, ppld = J.pp_ld;
     const auto pp = gmem(static_cast<const IdxT*>(J.pp_j));
     const auto pj = gmem(static_cast<IdxT*>(J.pair_j));
     // beads of the tile's rows and their possible counts (loads issued
@@ -886,7 +886,7 @@ __device__ __forceinline__ void verlet_tile(const DevJob& J, DevJobState& S, con
     // branch from addresses clamped into the row; the filter replaces the
     // words past the row's count by 0 (a valid bead) and rejects them.
     auto load_idx = [&](int b, int base, int (&j)[CH]) {
-        const auto row = pp + (size_t)bead_of(b) * ppcap;
+        const auto row = pp + (size_t)bead_of(b) * ppld;
 #pragma unroll
         for (int u = 0; u < CH; ++u) j[u] = (int)row[(unsigned)min(base + 64 * u + lane, ppcap - 1)];
     };
