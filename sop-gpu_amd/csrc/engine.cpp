@@ -470,7 +470,7 @@ static unsigned long long built_bytes(const BuiltJob& b, bool pulling, bool inde
     t += 2 * round_up(n * 16);            // coord[2]
     t += round_up(n * 16);                // rng
     t += round_up(n * 4);                 // T
-    if (pulling) t += round_up(n * 16);   // ext
+    if (pulling) t += round_up(n * 16) + round_up(n);  // ext, flags
     if (indentation) t += round_up(n * 16);
     t += round_up(n * 4) * 6;             // five count arrays, Verlet row order
     t += round_up(n * b.cov_cap * 4) * 2; // cov j, r0
@@ -521,6 +521,7 @@ int Engine::add_job(const sop_params& p, const sop_system& s, long long numsteps
     J.rng = (uint4*)A(n * 16);
     J.T = (float*)A(n * 4);
     J.ext = p.pulling ? (float4*)A(n * 16) : nullptr;
+    J.bflag = p.pulling ? (unsigned char*)A(n) : nullptr;
     J.tipF = p.indentation ? (float4*)A(n * 16) : nullptr;
     J.fextra = nullptr;
     J.cov_cnt = (int*)A(n * 4);
@@ -596,6 +597,12 @@ int Engine::add_job(const sop_params& p, const sop_system& s, long long numsteps
     std::vector<float> zeros(n * 4, 0.0f);
     H2D(J.T, zeros.data(), n * 4);
     if (J.ext) H2D(J.ext, b.ext.data(), nbytes(b.ext));
+    std::vector<unsigned char> flags;
+    if (J.bflag) {
+        flags.resize(n);
+        for (size_t i = 0; i < (size_t)n; ++i) flags[i] = (unsigned char)b.ext[i].w;
+        H2D(J.bflag, flags.data(), flags.size());
+    }
     if (J.tipF) H2D(J.tipF, zeros.data(), n * 16);
     H2D(J.cov_cnt, b.cov_cnt.data(), nbytes(b.cov_cnt));
     H2D(J.cov_j, b.cov_j.data(), nbytes(b.cov_j));

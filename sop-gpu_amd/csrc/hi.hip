@@ -86,11 +86,8 @@ __device__ __forceinline__ int wave_of() { return __builtin_amdgcn_readfirstlane
 
 __device__ __forceinline__ void hi_finish(const DevJob& J, int d, long long s, float fx, float fy, float fz,
                                           float4 coord) {
-    if (J.flags & JF_PULLING) {
-        const float4 e = J.ext[d];
-        if (e.w == 1.0f) {
-            fx = 0.0f; fy = 0.0f; fz = 0.0f;
-        }
+    if ((J.flags & JF_PULLING) && J.bflag[d] == 1) {
+        fx = 0.0f; fy = 0.0f; fz = 0.0f;
     }
     const float m = J.hOverZeta;
     const float dx = m * fx, dy = m * fy, dz = m * fz;

@@ -34,6 +34,7 @@ struct DevJob {
     uint4* rng;
     float* T;
     float4* ext;    // x,y,z force, w: 0 free, 1 fixed, 2 pulled
+    unsigned char* bflag;  // ext[i].w as one byte, read by every bead's step (ext only for pulled beads)
     float4* tipF;   // per-bead reaction on the indentation tip
     float4* fextra; // optional extra forces (custom potentials)
     int* cov_cnt;

@@ -125,6 +125,20 @@ __device__ __forceinline__ void pair_loop(const __attribute__((address_space(1))
     }
 #undef SOP_PAIR
 }
+// The fixed/pulled rules of the reference's extForces (pulling.cu:108-125,
+// langevin_kernel.cu:43-50): a fixed bead's force is zeroed, a pulled bead's
+// gets the cantilever force.  The one-byte flag is read for every bead, the
+// 16-byte force record only for pulled beads.
+__device__ __forceinline__ void ext_rules(const DevJob& J, int i, float& fx, float& fy, float& fz) {
+    const int f = J.bflag[i];
+    if (f == 1) {
+        fx = 0.0f; fy = 0.0f; fz = 0.0f;
+    } else if (f == 2) {
+        const float4 e = J.ext[i];
+        fx += e.x; fy += e.y; fz += e.z;
+    }
+}
+
 // One bead's step: FENE, native and repulsive terms gathered from `cin`
 // (global memory, or the job's coordinates staged in LDS), indentation
 // walls, then noise, the fixed/pulled rules and the Euler update, written to
@@ -239,15 +253,7 @@ __device__ __forceinline__ void step_bead(const DevJob& J, const DevJobState& S,
         const float3 g = gauss3(rs);
         J.rng[i] = rs;
         J.rforce[i] = make_float4(g.x * var, g.y * var, g.z * var, 0.0f);
-        if (J.flags & JF_PULLING) {
-            const float4 e = J.ext[i];
-            if (e.w == 1.0f) {
-                fx = 0.0f; fy = 0.0f; fz = 0.0f;
-            }
-            if (e.w == 2.0f) {
-                fx += e.x; fy += e.y; fz += e.z;
-            }
-        }
+        if (J.flags & JF_PULLING) ext_rules(J, i, fx, fy, fz);
         J.mforce[i] = make_float4(fx, fy, fz, 0.0f);
         return;
     }
@@ -261,15 +267,7 @@ __device__ __forceinline__ void step_bead(const DevJob& J, const DevJobState& S,
         fy += g.y * var;
         fz += g.z * var;
     }
-    if (J.flags & JF_PULLING) {
-        const float4 e = J.ext[i];
-        if (e.w == 1.0f) {
-            fx = 0.0f; fy = 0.0f; fz = 0.0f;
-        }
-        if (e.w == 2.0f) {
-            fx += e.x; fy += e.y; fz += e.z;
-        }
-    }
+    if (J.flags & JF_PULLING) ext_rules(J, i, fx, fy, fz);
     const float m = J.hOverZeta;
     const float dx = m * fx, dy = m * fy, dz = m * fz;
     float4 cn = c;
@@ -1167,7 +1165,10 @@ __device__ void pull_update(const DevJob& J, DevJobState& S, long long s) {
     S.extF[1] = J.k_trans * (S.cant[1] - cn.y);
     S.extF[2] = J.k_trans * (S.cant[2] - cn.z);
     for (int d = 0; d < 3; ++d) S.cant[d] = J.cant0[d] + xt * J.pullVec[d];
-    for (int k = 0; k < J.npulled; ++k) J.ext[J.pulled[k]] = make_float4(S.extF[0], S.extF[1], S.extF[2], 2.0f);
+    for (int k = 0; k < J.npulled; ++k) {
+        J.ext[J.pulled[k]] = make_float4(S.extF[0], S.extF[1], S.extF[2], 2.0f);
+        J.bflag[J.pulled[k]] = 2;
+    }
 }
 
 // mask: 1 indentation tip, 2 pulling cantilever, 4 heating schedule.
