@@ -475,6 +475,7 @@ static unsigned long long built_bytes(const BuiltJob& b, bool pulling, bool inde
     t += round_up(n * 4) * 6;             // five count arrays, Verlet row order
     t += round_up(n * b.cov_cap * 4) * 2; // cov j, r0
     t += round_up(n * b.nat_cap * 4) + round_up(n * b.nat_cap * 8);
+    if (b.dj.idx16) t += round_up(n * b.cov_cap * 2) + round_up(n * b.nat_cap * 2);  // 16-bit copies
     const unsigned long long es = b.dj.idx16 ? 2 : 4;
     t += round_up((n + 63) / 64 * 64 * b.pair_cap * es) + round_up(n * b.pp_ld * es) + round_up(n * b.excl_cap * 4);
     t += round_up(std::max<size_t>(1, b.pulled.size()) * 4);
@@ -526,9 +527,11 @@ int Engine::add_job(const sop_params& p, const sop_system& s, long long numsteps
     J.fextra = nullptr;
     J.cov_cnt = (int*)A(n * 4);
     J.cov_j = (int*)A(n * b.cov_cap * 4);
+    J.cov_j16 = J.idx16 ? (unsigned short*)A(n * b.cov_cap * 2) : nullptr;
     J.cov_r0 = (float*)A(n * b.cov_cap * 4);
     J.nat_cnt = (int*)A(n * 4);
     J.nat_j = (int*)A(n * b.nat_cap * 4);
+    J.nat_j16 = J.idx16 ? (unsigned short*)A(n * b.nat_cap * 2) : nullptr;
     J.nat_par = (float2*)A(n * b.nat_cap * 8);
     J.pair_cnt = (int*)A(n * 4);
     J.perm = (int*)A(n * 4);
@@ -606,9 +609,18 @@ int Engine::add_job(const sop_params& p, const sop_system& s, long long numsteps
     if (J.tipF) H2D(J.tipF, zeros.data(), n * 16);
     H2D(J.cov_cnt, b.cov_cnt.data(), nbytes(b.cov_cnt));
     H2D(J.cov_j, b.cov_j.data(), nbytes(b.cov_j));
+    std::vector<unsigned short> cov16, nat16;
+    if (J.cov_j16) {
+        cov16.assign(b.cov_j.begin(), b.cov_j.end());
+        H2D(J.cov_j16, cov16.data(), nbytes(cov16));
+    }
     H2D(J.cov_r0, b.cov_r0.data(), nbytes(b.cov_r0));
     H2D(J.nat_cnt, b.nat_cnt.data(), nbytes(b.nat_cnt));
     H2D(J.nat_j, b.nat_j.data(), nbytes(b.nat_j));
+    if (J.nat_j16) {
+        nat16.assign(b.nat_j.begin(), b.nat_j.end());
+        H2D(J.nat_j16, nat16.data(), nbytes(nat16));
+    }
     H2D(J.nat_par, b.nat_par.data(), nbytes(b.nat_par));
     H2D(J.pair_cnt, zeros.data(), n * 4);
     std::vector<int> ident(n);

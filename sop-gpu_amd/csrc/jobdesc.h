@@ -39,9 +39,11 @@ struct DevJob {
     float4* fextra; // optional extra forces (custom potentials)
     int* cov_cnt;
     int* cov_j;
+    unsigned short* cov_j16;  // the same entries as 16-bit indices (n < 65,536; else null), read by the step
     float* cov_r0;
     int* nat_cnt;
     int* nat_j;
+    unsigned short* nat_j16;  // the same entries as 16-bit indices (n < 65,536; else null), read by the step
     float2* nat_par; // (r0^2, -12 eh / r0^2)
     // Verlet and possible lists hold job-local bead indices as uint16 when the
     // job has fewer than 65536 beads (idx16), as int32 otherwise

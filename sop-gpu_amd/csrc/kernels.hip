@@ -139,23 +139,16 @@ __device__ __forceinline__ void ext_rules(const DevJob& J, int i, float& fx, flo
     }
 }
 
-// One bead's step: FENE, native and repulsive terms gathered from `cin`
-// (global memory, or the job's coordinates staged in LDS), indentation
-// walls, then noise, the fixed/pulled rules and the Euler update, written to
-// `cout` in global memory.  Hydrodynamic jobs stop after the forces.
-__device__ __forceinline__ void step_bead(const DevJob& J, const DevJobState& S, long long s, int i, int r, int n,
-                                          const float4* __restrict__ cin, float4* __restrict__ cout) {
-    const float4 c = cin[i];
-    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    if (J.flags & JF_EXTRA_FORCE) {
-        const float4 e = J.fextra[i];
-        fx = e.x; fy = e.y; fz = e.z;
-        J.fextra[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    }
+// FENE (covalent_kernel.cu:36-48) and native LJ 12-6 (native_kernel.cu:36-49)
+// of bead i, partners from slot-major lists whose bead-i columns start at
+// cj and nj (32- or 16-bit indices).
+template <typename CJ, typename NJ>
+__device__ __forceinline__ void bonded_terms(const DevJob& J, int i, int n, CJ cj, NJ nj,
+                                             const float4* __restrict__ cin, const float4 c, float& fx, float& fy,
+                                             float& fz) {
     // FENE (covalent_kernel.cu:36-48)
     {
         const int nc = J.cov_cnt[i];
-        const auto cj = gmem(J.cov_j) + i;
         const auto cr = gmem(J.cov_r0) + i;
         for (int k = 0; k < nc; ++k) {
             const int j = cj[(size_t)k * n];
@@ -173,7 +166,6 @@ __device__ __forceinline__ void step_bead(const DevJob& J, const DevJobState& S,
     // native LJ 12-6 (native_kernel.cu:36-49), gathers four ahead
     {
         const int nn = J.nat_cnt[i];
-        const auto nj = gmem(J.nat_j) + i;
         const float2* __restrict__ npar = J.nat_par + i;
 #define SOP_NAT(o, par)                                                 \
 {                                                                   \
@@ -207,6 +199,27 @@ __device__ __forceinline__ void step_bead(const DevJob& J, const DevJobState& S,
         }
 #undef SOP_NAT
     }
+}
+
+// One bead's step: FENE, native and repulsive terms gathered from `cin`
+// (global memory, or the job's coordinates staged in LDS), indentation
+// walls, then noise, the fixed/pulled rules and the Euler update, written to
+// `cout` in global memory.  Hydrodynamic jobs stop after the forces.
+__device__ __forceinline__ void step_bead(const DevJob& J, const DevJobState& S, long long s, int i, int r, int n,
+                                          const float4* __restrict__ cin, float4* __restrict__ cout) {
+    const float4 c = cin[i];
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    if (J.flags & JF_EXTRA_FORCE) {
+        const float4 e = J.fextra[i];
+        fx = e.x; fy = e.y; fz = e.z;
+        J.fextra[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    // FENE and native LJ 12-6, their partner indices 16-bit where the job
+    // has a 16-bit copy of them
+    if (J.nat_j16)
+        bonded_terms(J, i, n, gmem(J.cov_j16) + i, gmem(J.nat_j16) + i, cin, c, fx, fy, fz);
+    else
+        bonded_terms(J, i, n, gmem(J.cov_j) + i, gmem(J.nat_j) + i, cin, c, fx, fy, fz);
     // repulsive LJ over the Verlet list (pairs_kernel.cu:41-56)
     const int np = gmem(J.pair_cnt)[r];
     if (J.idx16)
