@@ -476,6 +476,10 @@ static unsigned long long built_bytes(const BuiltJob& b, bool pulling, bool inde
     t += round_up(n * b.cov_cap * 4) * 2; // cov j, r0
     t += round_up(n * b.nat_cap * 4) + round_up(n * b.nat_cap * 8);
     if (b.dj.idx16) t += round_up(n * b.cov_cap * 2) + round_up(n * b.nat_cap * 2);  // 16-bit copies
+    if (n <= (unsigned long long)kLdsStepBeads && !(b.dj.flags & JF_HI)) {  // bonded lists in row order
+        const unsigned long long ld = (n + 63) / 64 * 64;
+        t += round_up(n * 4) + round_up(ld * b.cov_cap * 6) + round_up(ld * b.nat_cap * 10);
+    }
     const unsigned long long es = b.dj.idx16 ? 2 : 4;
     t += round_up((n + 63) / 64 * 64 * b.pair_cap * es) + round_up(n * b.pp_ld * es) + round_up(n * b.excl_cap * 4);
     t += round_up(std::max<size_t>(1, b.pulled.size()) * 4);
@@ -532,6 +536,13 @@ int Engine::add_job(const sop_params& p, const sop_system& s, long long numsteps
     J.nat_cnt = (int*)A(n * 4);
     J.nat_j = (int*)A(n * b.nat_cap * 4);
     J.nat_j16 = J.idx16 ? (unsigned short*)A(n * b.nat_cap * 2) : nullptr;
+    const bool rows_eligible = n <= (size_t)kLdsStepBeads && !(J.flags & JF_HI);  // may have its rows sorted
+    const size_t rld = (n + 63) / 64 * 64;
+    J.rb_cnt = rows_eligible ? (int*)A(n * 4) : nullptr;
+    J.rb_cov_j = rows_eligible ? (unsigned short*)A(rld * b.cov_cap * 2) : nullptr;
+    J.rb_cov_r0 = rows_eligible ? (float*)A(rld * b.cov_cap * 4) : nullptr;
+    J.rb_nat_j = rows_eligible ? (unsigned short*)A(rld * b.nat_cap * 2) : nullptr;
+    J.rb_nat_par = rows_eligible ? (float2*)A(rld * b.nat_cap * 8) : nullptr;
     J.nat_par = (float2*)A(n * b.nat_cap * 8);
     J.pair_cnt = (int*)A(n * 4);
     J.perm = (int*)A(n * 4);
@@ -546,7 +557,7 @@ int Engine::add_job(const sop_params& p, const sop_system& s, long long numsteps
     J.npulled = (int)b.pulled.size();
     J.energies = (float*)A(n * 12);
     J.lds_step = use_lds_step_ && n <= (size_t)kLdsStepBeads ? 1 : 0;
-    J.sort_rows = J.lds_step && !(J.flags & JF_HI);
+    J.sort_rows = J.lds_step && J.rb_cnt ? 1 : 0;
     if (uses_cells(n)) {
         J.cell_cap = (int)n + 64;
         J.cell_cnt = (int*)A((size_t)J.cell_cap * 4);
@@ -622,6 +633,34 @@ int Engine::add_job(const sop_params& p, const sop_system& s, long long numsteps
         H2D(J.nat_j16, nat16.data(), nbytes(nat16));
     }
     H2D(J.nat_par, b.nat_par.data(), nbytes(b.nat_par));
+    // the row-ordered bonded lists for the identity row order
+    std::vector<int> rbc;
+    std::vector<unsigned short> rbcj, rbnj;
+    std::vector<float> rbcr;
+    std::vector<float2> rbnp;
+    if (J.rb_cnt) {
+        rbc.resize(n);
+        rbcj.assign(rld * b.cov_cap, 0);
+        rbcr.assign(rld * b.cov_cap, 0.0f);
+        rbnj.assign(rld * b.nat_cap, 0);
+        rbnp.assign(rld * b.nat_cap, float2{0.0f, 0.0f});
+        for (size_t i = 0; i < n; ++i) {
+            rbc[i] = b.cov_cnt[i] | (b.nat_cnt[i] << 16);
+            for (int k = 0; k < b.cov_cap; ++k) {
+                rbcj[k * rld + i] = (unsigned short)b.cov_j[k * n + i];
+                rbcr[k * rld + i] = b.cov_r0[k * n + i];
+            }
+            for (int k = 0; k < b.nat_cap; ++k) {
+                rbnj[k * rld + i] = (unsigned short)b.nat_j[k * n + i];
+                rbnp[k * rld + i] = b.nat_par[k * n + i];
+            }
+        }
+        H2D(J.rb_cnt, rbc.data(), nbytes(rbc));
+        H2D(J.rb_cov_j, rbcj.data(), nbytes(rbcj));
+        H2D(J.rb_cov_r0, rbcr.data(), nbytes(rbcr));
+        H2D(J.rb_nat_j, rbnj.data(), nbytes(rbnj));
+        H2D(J.rb_nat_par, rbnp.data(), nbytes(rbnp));
+    }
     H2D(J.pair_cnt, zeros.data(), n * 4);
     std::vector<int> ident(n);
     for (size_t k = 0; k < n; ++k) ident[k] = (int)k;
@@ -886,7 +925,7 @@ void Engine::set_lds_step(int enabled) {
     use_lds_step_ = enabled != 0;
     for (auto& j : jobs_) {
         j.dj.lds_step = use_lds_step_ && j.n <= kLdsStepBeads ? 1 : 0;
-        j.dj.sort_rows = j.dj.lds_step && !(j.dj.flags & JF_HI);
+        j.dj.sort_rows = j.dj.lds_step && j.dj.rb_cnt ? 1 : 0;
     }
     tables_dirty_ = true;
     drop_graph();

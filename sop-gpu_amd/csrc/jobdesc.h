@@ -55,6 +55,14 @@ struct DevJob {
     // about equal length; elsewhere perm is the identity.
     int* perm;
     int sort_rows;
+    // The bonded lists in Verlet-row order, for jobs whose rows are sorted
+    // (refreshed with every sort): counts per row (covalent | native << 16)
+    // and slot-major entries [k*pair_ld + r] of bead perm[r] (else null).
+    int* rb_cnt;
+    unsigned short* rb_cov_j;
+    float* rb_cov_r0;
+    unsigned short* rb_nat_j;
+    float2* rb_nat_par;
     int* pair_cnt;  // per row
     void* pair_j;
     int* pp_cnt;

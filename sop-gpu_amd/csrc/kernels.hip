@@ -140,16 +140,15 @@ __device__ __forceinline__ void ext_rules(const DevJob& J, int i, float& fx, flo
 }
 
 // FENE (covalent_kernel.cu:36-48) and native LJ 12-6 (native_kernel.cu:36-49)
-// of bead i, partners from slot-major lists whose bead-i columns start at
-// cj and nj (32- or 16-bit indices).
-template <typename CJ, typename NJ>
-__device__ __forceinline__ void bonded_terms(const DevJob& J, int i, int n, CJ cj, NJ nj,
-                                             const float4* __restrict__ cin, const float4 c, float& fx, float& fy,
-                                             float& fz) {
+// of one bead: nc covalent and nn native partners from slot-major lists of
+// slot stride n whose columns for this bead start at cj, cr, nj, npar
+// (32- or 16-bit indices; bead order, or row order for sorted rows).
+template <typename CJ, typename CR, typename NJ>
+__device__ __forceinline__ void bonded_terms(const DevJob& J, int nc, int nn, int n, CJ cj, CR cr, NJ nj,
+                                             const float2* __restrict__ npar, const float4* __restrict__ cin,
+                                             const float4 c, float& fx, float& fy, float& fz) {
     // FENE (covalent_kernel.cu:36-48)
     {
-        const int nc = J.cov_cnt[i];
-        const auto cr = gmem(J.cov_r0) + i;
         for (int k = 0; k < nc; ++k) {
             const int j = cj[(size_t)k * n];
             const float r0 = cr[(size_t)k * n];
@@ -165,8 +164,6 @@ __device__ __forceinline__ void bonded_terms(const DevJob& J, int i, int n, CJ c
     }
     // native LJ 12-6 (native_kernel.cu:36-49), gathers four ahead
     {
-        const int nn = J.nat_cnt[i];
-        const float2* __restrict__ npar = J.nat_par + i;
 #define SOP_NAT(o, par)                                                 \
 {                                                                   \
     const float x = (o).x - c.x, y = (o).y - c.y, z = (o).z - c.z;     \
@@ -214,12 +211,20 @@ __device__ __forceinline__ void step_bead(const DevJob& J, const DevJobState& S,
         fx = e.x; fy = e.y; fz = e.z;
         J.fextra[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
-    // FENE and native LJ 12-6, their partner indices 16-bit where the job
-    // has a 16-bit copy of them
-    if (J.nat_j16)
-        bonded_terms(J, i, n, gmem(J.cov_j16) + i, gmem(J.nat_j16) + i, cin, c, fx, fy, fz);
-    else
-        bonded_terms(J, i, n, gmem(J.cov_j) + i, gmem(J.nat_j) + i, cin, c, fx, fy, fz);
+    // FENE and native LJ 12-6: for sorted rows from the row-ordered copies
+    // (a wavefront's 64 rows read 64 consecutive words per slot), else in
+    // bead order, partner indices 16-bit where the job has a 16-bit copy
+    if (J.sort_rows) {
+        const int cnt = gmem(J.rb_cnt)[r];
+        bonded_terms(J, cnt & 0xffff, cnt >> 16, J.pair_ld, gmem(J.rb_cov_j) + r, gmem(J.rb_cov_r0) + r,
+                     gmem(J.rb_nat_j) + r, J.rb_nat_par + r, cin, c, fx, fy, fz);
+    } else if (J.nat_j16) {
+        bonded_terms(J, J.cov_cnt[i], J.nat_cnt[i], n, gmem(J.cov_j16) + i, gmem(J.cov_r0) + i, gmem(J.nat_j16) + i,
+                     J.nat_par + i, cin, c, fx, fy, fz);
+    } else {
+        bonded_terms(J, J.cov_cnt[i], J.nat_cnt[i], n, gmem(J.cov_j) + i, gmem(J.cov_r0) + i, gmem(J.nat_j) + i,
+                     J.nat_par + i, cin, c, fx, fy, fz);
+    }
     // repulsive LJ over the Verlet list (pairs_kernel.cu:41-56)
     const int np = gmem(J.pair_cnt)[r];
     if (J.idx16)
@@ -1011,6 +1016,10 @@ __device__ __forceinline__ void verlet_store(const DevJob& J, int i0, const IdxT
 // over the longest list idles few lanes and reads few padding slots.  Only
 // which row and lane serve a bead changes, never the arithmetic, so the
 // trajectories are the same bits in any row order.  Bitonic sort in LDS.
+// The sort also moves each bead's bonded lists to its new row (the step
+// reads them in row order), so it runs at every kSortEvery-th Verlet
+// rebuild only: list lengths drift little in between.
+constexpr int kSortEvery = 10;
 __global__ __launch_bounds__(1024) void k_sort_rows(const DevJob* __restrict__ jobs, const DevJobState* __restrict__ st,
                                                     const int2* __restrict__ units, int nunits, int force) {
     __shared__ unsigned long long key[kLdsUnit];
@@ -1019,7 +1028,7 @@ __global__ __launch_bounds__(1024) void k_sort_rows(const DevJob* __restrict__ j
         const int2 ud = units[u];
         const DevJob& J = jobs[ud.x];
         if (!J.sort_rows) continue;
-        if (!force && !due(J, st[ud.x].step, J.pairs_freq)) continue;
+        if (!force && !due(J, st[ud.x].step, J.pairs_freq * kSortEvery)) continue;  // a Verlet rebuild step
         const int m = min(kLdsUnit, J.n - ud.y), r = ud.y + t;
         __syncthreads();  // the previous unit's keys are read
         key[t] = t < m ? ((unsigned long long)(unsigned)(J.pair_cap - J.pair_cnt[r]) << 32) | (unsigned)J.perm[r]
@@ -1037,7 +1046,22 @@ __global__ __launch_bounds__(1024) void k_sort_rows(const DevJob* __restrict__ j
                 }
                 __syncthreads();
             }
-        if (t < m) J.perm[r] = (int)(unsigned)(key[t] & 0xffffffffull);
+        if (t < m) {
+            // row r now steps bead i: its bonded lists move to row r
+            const int i = (int)(unsigned)(key[t] & 0xffffffffull);
+            J.perm[r] = i;
+            const int n = J.n, ld = J.pair_ld;
+            const int nc = J.cov_cnt[i], nn = J.nat_cnt[i];
+            J.rb_cnt[r] = nc | (nn << 16);
+            for (int k = 0; k < nc; ++k) {
+                J.rb_cov_j[(size_t)k * ld + r] = J.cov_j16[(size_t)k * n + i];
+                J.rb_cov_r0[(size_t)k * ld + r] = J.cov_r0[(size_t)k * n + i];
+            }
+            for (int k = 0; k < nn; ++k) {
+                J.rb_nat_j[(size_t)k * ld + r] = J.nat_j16[(size_t)k * n + i];
+                J.rb_nat_par[(size_t)k * ld + r] = J.nat_par[(size_t)k * n + i];
+            }
+        }
     }
 }
 
