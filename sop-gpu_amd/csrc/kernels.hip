@@ -76,12 +76,11 @@ __device__ __forceinline__ __attribute__((address_space(1))) T* gmem(T* p) {
 
 // Repulsive LJ over a bead's slot-major Verlet entries, in list order.  The
 // list words are streamed with non-temporal loads (each is used once per
-// step) and fetched one batch of eight ahead: while batch b is computed the
-// indices of batch b+1 and the coordinates of batch b are in flight.
-#ifndef SOP_PAIR_BATCH
-#define SOP_PAIR_BATCH 6
-#endif
-template <typename IdxT>
+// step) and fetched one batch of PB ahead: while batch b is computed the
+// indices of batch b+1 and the coordinates of batch b are in flight.  PB is
+// 8 for gathers from L2 and 6 for gathers from LDS (which keeps the LDS
+// kernel within the 64 VGPRs of two workgroups per CU).
+template <int PB, typename IdxT>
 __device__ __forceinline__ void pair_loop(const __attribute__((address_space(1))) IdxT* __restrict__ pj, int np, int n, const float4* __restrict__ cin,
                                           const float4 c, float rc2, float a2, float m6, float& fx, float& fy,
                                           float& fz) {
@@ -101,7 +100,6 @@ __device__ __forceinline__ void pair_loop(const __attribute__((address_space(1))
             fz += z * q;                                             \
         }                                                            \
     }
-    constexpr int PB = SOP_PAIR_BATCH;  // entries per batch
     int k = 0;
     if (np >= PB) {
         int jn[PB];
@@ -202,6 +200,7 @@ __device__ __forceinline__ void bonded_terms(const DevJob& J, int nc, int nn, in
 // (global memory, or the job's coordinates staged in LDS), indentation
 // walls, then noise, the fixed/pulled rules and the Euler update, written to
 // `cout` in global memory.  Hydrodynamic jobs stop after the forces.
+template <int PB>
 __device__ __forceinline__ void step_bead(const DevJob& J, const DevJobState& S, long long s, int i, int r, int n,
                                           const float4* __restrict__ cin, float4* __restrict__ cout) {
     const float4 c = cin[i];
@@ -228,10 +227,10 @@ __device__ __forceinline__ void step_bead(const DevJob& J, const DevJobState& S,
     // repulsive LJ over the Verlet list (pairs_kernel.cu:41-56)
     const int np = gmem(J.pair_cnt)[r];
     if (J.idx16)
-        pair_loop(gmem(static_cast<const unsigned short*>(J.pair_j)) + r, np, J.pair_ld, cin, c, J.rc2, J.a2, J.m6,
+        pair_loop<PB>(gmem(static_cast<const unsigned short*>(J.pair_j)) + r, np, J.pair_ld, cin, c, J.rc2, J.a2, J.m6,
                   fx, fy, fz);
     else
-        pair_loop(gmem(static_cast<const int*>(J.pair_j)) + r, np, J.pair_ld, cin, c, J.rc2, J.a2, J.m6, fx, fy,
+        pair_loop<PB>(gmem(static_cast<const int*>(J.pair_j)) + r, np, J.pair_ld, cin, c, J.rc2, J.a2, J.m6, fx, fy,
                   fz);
     // indentation tip and substrate plane (indentation_kernel.cu:71-128)
     if (J.flags & JF_INDENTATION) {
@@ -313,7 +312,7 @@ __global__ __launch_bounds__(256) void k_step(const DevJob* __restrict__ jobs, c
         const int r = td.y + lane;  // Verlet row; the lane steps bead perm[r]
         const int n = J.n;
         if (r >= n) continue;
-        step_bead(J, st[td.x], s, J.perm[r], r, n, J.coord[s & 1], J.coord[(s + 1) & 1]);
+        step_bead<8>(J, st[td.x], s, J.perm[r], r, n, J.coord[s & 1], J.coord[(s + 1) & 1]);
     }
 }
 
@@ -350,7 +349,7 @@ __global__ __launch_bounds__(1024, 8) void k_step_lds(const DevJob* __restrict__
                 const long long s = st[sg.x].step + kstep;
                 const int r = sg.y + t;  // Verlet row; the thread steps bead perm[r]
                 if (t < sg.z && s < J.end_step)
-                    step_bead(J, st[sg.x], s, J.perm[r], r, J.n, lds_coord + sg.w, J.coord[(s + 1) & 1]);
+                    step_bead<6>(J, st[sg.x], s, J.perm[r], r, J.n, lds_coord + sg.w, J.coord[(s + 1) & 1]);
                 break;
             }
             t -= span;
