@@ -116,6 +116,8 @@ def serving_phase(S, args, capacity):
     rng = np.random.default_rng(args.seed + 12)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, len(mix)))
     srv = S.Server(0, int(args.serve_budget_gb * (1 << 30)))
+    if args.no_graph:
+        srv.set_tuning(0, False)
     tickets = [None] * len(mix)
     nxt = 0
     t0 = time.perf_counter()

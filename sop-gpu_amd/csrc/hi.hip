@@ -222,21 +222,30 @@ __device__ __forceinline__ void tea_force(const float4 c1, const float4 c2, cons
     fz += D.xz * gx + D.yz * gy + D.zz * gz;
 }
 
-// The TEA step (bdhitea_kernel.cu:364-515), one 1024-thread workgroup per
-// 64-bead tile of a TEA job; the partner sum over all beads is formed as in
-// k_hi_eps (terms by all wavefronts, sums in j order by wavefront 0), the
-// listed variant walks the lists in wavefront 0.
+// The TEA step (bdhitea_kernel.cu:364-515).  Unlisted: kTeaSplit 1024-thread
+// workgroups per 64-bead tile of a TEA job, each serving 16 beads, so a
+// single job of a few thousand beads still fills the chip; for each chunk of
+// 64 partners every thread forms one (bead, partner) term into LDS, then 16
+// lanes of wavefront 0 add them to their beads' sums in partner order, the
+// order of the reference's sequential loop.  Listed: wavefront 0 of the
+// tile's first workgroup walks the lists.
+constexpr int kTeaSplit = 4;
 __global__ __launch_bounds__(1024) void k_hi_tea(const DevJob* __restrict__ jobs, const DevJobState* __restrict__ st,
                                                  const int2* __restrict__ tiles, int ntiles, int kstep) {
-    __shared__ float term[3][64][65];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int2 td = tiles[blockIdx.x];
+    __shared__ float term[3][16][65];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int2 td = tiles[blockIdx.x / kTeaSplit];
+    const int q = blockIdx.x % kTeaSplit;
     const DevJob& J = jobs[td.x];
     if (!(J.flags & JF_HI) || (J.flags & JF_HI_EXACT)) return;
     const DevJobState& S = st[td.x];
     const long long s = S.step + kstep;
     if (s >= J.end_step) return;
-    const int d = td.y + lane;
+    const bool listed = (J.flags & JF_HI_LISTED) != 0;
+    if (listed && (q != 0 || w != 0)) return;
+    // listed: lane = bead of the 64-bead tile; unlisted: t & 15 = bead of
+    // this workgroup's 16, t >> 4 = partner of the chunk
+    const int d = listed ? td.y + lane : td.y + 16 * q + (t & 15);
     const int n = J.n;
     const bool own = d < n;
     const float4* __restrict__ cur = J.coord[s & 1];
@@ -262,8 +271,8 @@ __global__ __launch_bounds__(1024) void k_hi_tea(const DevJob* __restrict__ jobs
     }
     const float a = J.hi_a;
     float tx, ty, tz;
-    if (J.flags & JF_HI_LISTED) {
-        if (w != 0 || !own) return;
+    if (listed) {
+        if (!own) return;
         for (int k = 0; k < J.cov_cnt[d]; ++k) {
             const int i = J.cov_j[(size_t)k * n + d];
             tx = 0.f; ty = 0.f; tz = 0.f;
@@ -285,29 +294,27 @@ __global__ __launch_bounds__(1024) void k_hi_tea(const DevJob* __restrict__ jobs
         hi_finish(J, d, s, fx, fy, fz, coord);
         return;
     }
+    const int b16 = t & 15, jl = t >> 4;
     for (int base = 0; base < n; base += 64) {
-        for (int jl = w; jl < 64; jl += 16) {
-            const int i = base + jl;
-            tx = 0.f; ty = 0.f; tz = 0.f;
-            if (i < n && i != d && own)
-                tea_force(coord, cur[i], J.mforce[i], J.rforce[i], ci0, ci1, ci2, a, tx, ty, tz);
-            term[0][lane][jl] = tx;
-            term[1][lane][jl] = ty;
-            term[2][lane][jl] = tz;
-        }
+        const int i = base + jl;
+        tx = 0.f; ty = 0.f; tz = 0.f;
+        if (i < n && i != d && own) tea_force(coord, cur[i], J.mforce[i], J.rforce[i], ci0, ci1, ci2, a, tx, ty, tz);
+        term[0][b16][jl] = tx;
+        term[1][b16][jl] = ty;
+        term[2][b16][jl] = tz;
         __syncthreads();
-        if (w == 0 && own) {
+        if (t < 16 && own) {
             const int m = min(64, n - base);
-            for (int jl = 0; jl < m; ++jl) {
-                if (base + jl == d) continue;
-                fx += term[0][lane][jl];
-                fy += term[1][lane][jl];
-                fz += term[2][lane][jl];
+            for (int k = 0; k < m; ++k) {
+                if (base + k == d) continue;
+                fx += term[0][b16][k];
+                fy += term[1][b16][k];
+                fz += term[2][b16][k];
             }
         }
         __syncthreads();
     }
-    if (w == 0 && own) hi_finish(J, d, s, fx, fy, fz, coord);
+    if (t < 16 && own) hi_finish(J, d, s, fx, fy, fz, coord);
 }
 
 // ---------------------------------------------------------------------------
@@ -1154,7 +1161,8 @@ hipError_t launch_hi_tea_update(const DevJob* jobs, DevJobState* st, const int2*
 
 hipError_t launch_hi_tea_step(const DevJob* jobs, const DevJobState* st, const int2* tiles, int ntiles, int kstep,
                               hipStream_t stream) {
-    if (ntiles > 0) hipLaunchKernelGGL(k_hi_tea, dim3(ntiles), dim3(1024), 0, stream, jobs, st, tiles, ntiles, kstep);
+    if (ntiles > 0)
+        hipLaunchKernelGGL(k_hi_tea, dim3(ntiles * kTeaSplit), dim3(1024), 0, stream, jobs, st, tiles, ntiles, kstep);
     return hipGetLastError();
 }
 
