@@ -48,11 +48,12 @@ struct DevJob {
     // Verlet and possible lists hold job-local bead indices as uint16 when the
     // job has fewer than 65536 beads (idx16), as int32 otherwise
     // Verlet list rows: row p holds the list of bead perm[p] (slot-major:
-    // entry k of row p at [k*n + p]) and the wavefront that steps rows
-    // 64g..64g+63 steps those beads.  For jobs stepped by k_step_lds (not
-    // hydrodynamic) the beads of each kLdsUnit-row unit are re-sorted by
-    // list length at every Verlet rebuild, so a wavefront's lists are of
-    // about equal length; elsewhere perm is the identity.
+    // entry k of row p at [k*pair_ld + p]) and the wavefront that steps rows
+    // 64g..64g+63 steps those beads.  For jobs with sort_rows set (stepped
+    // by k_step_lds, not hydrodynamic) the beads of each kLdsUnit-row unit
+    // are re-sorted by list length at every tenth Verlet rebuild, so a
+    // wavefront's lists are of about equal length; elsewhere perm is the
+    // identity.
     int* perm;
     int sort_rows;
     // The bonded lists in Verlet-row order, for jobs whose rows are sorted
