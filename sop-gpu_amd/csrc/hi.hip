@@ -1150,7 +1150,6 @@ __global__ __launch_bounds__(256) void k_gemm_ks(float* __restrict__ M, size_t l
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-static int grid_tiles(int ntiles) { return ntiles > 0 ? (ntiles + 3) / 4 : 1; }
 
 hipError_t launch_hi_tea_update(const DevJob* jobs, DevJobState* st, const int2* tiles, int ntiles, int njobs,
                                 hipStream_t stream) {

@@ -1,16 +1,22 @@
 // kernels.hip -- CDNA4 kernels of the SOP Langevin hot path.
 //
-// k_step       one launch per step for every admitted job: FENE + native
+// k_step, k_step_lds
+//              one launch per step for every admitted job: FENE + native
 //              LJ 12-6 + repulsive LJ (+ indentation walls) gathered per bead
 //              over slot-major lists, then the Hybrid-Taus noise, the
-//              fixed/pulled-bead rules and the overdamped Euler update.
-//              Reference bodies: src/Potentials/covalent_kernel.cu:28-51,
+//              fixed/pulled-bead rules and the overdamped Euler update; the
+//              LDS variant stages the coordinates of jobs of at most 5,120
+//              beads in LDS.  Reference bodies: src/Potentials/covalent_kernel.cu:28-51,
 //              native_kernel.cu:27-52, pairs_kernel.cu:33-59,
 //              indentation_kernel.cu:71-128, src/Integrators/langevin_kernel.cu:17-58.
-// k_possible   possible-pairs list: same-job beads j != i, ascending j, with
+// k_possible, k_possible_lds, k_possible_cells
+//              possible-pairs list: same-job beads j != i, ascending j, with
 //              sqrtf(d^2) < threshold, excluding covalent and native partners
 //              (src/Updaters/possiblepairlist_kernel.cu:18-71).
-// k_verlet     Verlet list: the possible list filtered by sqrtf(d^2) <
+// k_sort_rows  row order of the Verlet lists (by list length) and the
+//              bonded lists moved into it.
+// k_verlet, k_verlet_lds
+//              Verlet list: the possible list filtered by sqrtf(d^2) <
 //              pairlist_cutoff in stored order (src/Updaters/pairlist_kernel.cu:33-76).
 // k_job_update per-job updaters at stride boundaries: indentation tip,
 //              force-ramp cantilever, heating schedule
