@@ -321,60 +321,60 @@ __global__ __launch_bounds__(1024) void k_hi_tea(const DevJob* __restrict__ jobs
 // exact path
 // ---------------------------------------------------------------------------
 
-// F_i + sum_j D_ij F_j into fdf, summed in bead order.  One 1024-thread
-// workgroup per 64-bead tile of the (single) exact-HI job of the launch:
-// for each chunk of 64 beads j the sixteen wavefronts form the 64x64 terms
+// F_i + sum_j D_ij F_j into fdf, summed in bead order.  kTeaSplit 1024-thread
+// workgroups per 64-bead tile of the (single) exact-HI job of the launch,
+// 16 beads each: for each chunk of 64 beads j every thread forms one term
 // D_ij F_j (the three row sums, written exactly as the sequential loop
-// writes them) into LDS, then wavefront 0 adds them to bead i's sums in j
+// writes them) into LDS, then lanes 0-15 add them to their bead's sums in j
 // order, so every sum is the same bits as the bead-by-bead loop.
 __global__ __launch_bounds__(1024) void k_chol_D(const DevJob* __restrict__ jobs, const DevJobState* __restrict__ st,
                                                  const int2* __restrict__ tiles, int ntiles, int kstep) {
-    __shared__ float term[3][64][65];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int2 td = tiles[blockIdx.x];
+    __shared__ float term[3][16][65];
+    const int t = threadIdx.x, b16 = t & 15, jl = t >> 4;
+    const int2 td = tiles[blockIdx.x / kTeaSplit];
     const DevJob& J = jobs[td.x];
     if (!(J.flags & JF_HI_EXACT)) return;  // one job per launch: uniform
     const long long s = st[td.x].step + kstep;
     if (s >= J.end_step) return;
     const int n = J.n;
-    const int d = td.y + lane;  // this lane's bead i (the sums' owner in wavefront 0)
+    // this thread's bead i: one of the workgroup's 16 (the sums' owners are
+    // lanes 0-15 of wavefront 0); jl: its partner in each chunk of 64
+    const int d = td.y + 16 * (blockIdx.x % kTeaSplit) + b16;
     const float4* __restrict__ cur = J.coord[s & 1];
     const float4 c = d < n ? cur[d] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float a = J.hi_a;
     float fx = 0.f, fy = 0.f, fz = 0.f;
-    if (w == 0 && d < n) {
+    if (t < 16 && d < n) {
         const float4 f0 = J.mforce[d];
         fx = f0.x, fy = f0.y, fz = f0.z;
     }
     for (int base = 0; base < n; base += 64) {
-        for (int jl = w; jl < 64; jl += 16) {
-            const int j = base + jl;
-            float tx = 0.f, ty = 0.f, tz = 0.f;
-            if (j < n && j != d && d < n) {
-                const float4 dr = unit_sep(c, cur[j]);
-                const F6 D = rpy(dr, a);
-                const float4 g = J.mforce[j];
-                tx = D.xx * g.x + D.xy * g.y + D.xz * g.z;
-                ty = D.xy * g.x + D.yy * g.y + D.yz * g.z;
-                tz = D.xz * g.x + D.yz * g.y + D.zz * g.z;
-            }
-            term[0][lane][jl] = tx;
-            term[1][lane][jl] = ty;
-            term[2][lane][jl] = tz;
+        const int j = base + jl;
+        float tx = 0.f, ty = 0.f, tz = 0.f;
+        if (j < n && j != d && d < n) {
+            const float4 dr = unit_sep(c, cur[j]);
+            const F6 D = rpy(dr, a);
+            const float4 g = J.mforce[j];
+            tx = D.xx * g.x + D.xy * g.y + D.xz * g.z;
+            ty = D.xy * g.x + D.yy * g.y + D.yz * g.z;
+            tz = D.xz * g.x + D.yz * g.y + D.zz * g.z;
         }
+        term[0][b16][jl] = tx;
+        term[1][b16][jl] = ty;
+        term[2][b16][jl] = tz;
         __syncthreads();
-        if (w == 0 && d < n) {
+        if (t < 16 && d < n) {
             const int m = min(64, n - base);
-            for (int jl = 0; jl < m; ++jl) {
-                if (base + jl == d) continue;
-                fx += term[0][lane][jl];
-                fy += term[1][lane][jl];
-                fz += term[2][lane][jl];
+            for (int k = 0; k < m; ++k) {
+                if (base + k == d) continue;
+                fx += term[0][b16][k];
+                fy += term[1][b16][k];
+                fz += term[2][b16][k];
             }
         }
         __syncthreads();
     }
-    if (w == 0 && d < n) J.fdf[d] = make_float4(fx, fy, fz, 0.0f);
+    if (t < 16 && d < n) J.fdf[d] = make_float4(fx, fy, fz, 0.0f);
 }
 
 // D/D_ii lower block triangle into M on refactorisation steps: block
@@ -1245,7 +1245,7 @@ hipError_t launch_chol_factor(float* M, size_t ld, int mp, int mode, double* lin
 
 hipError_t launch_chol_step(const DevJob* jobs, const DevJobState* st, const int2* tiles, int ntiles, int kstep,
                             const HiExactWork* w, hipStream_t stream) {
-    hipLaunchKernelGGL(k_chol_D, dim3(ntiles), dim3(1024), 0, stream, jobs, st, tiles, ntiles, kstep);
+    hipLaunchKernelGGL(k_chol_D, dim3(ntiles * kTeaSplit), dim3(1024), 0, stream, jobs, st, tiles, ntiles, kstep);
     if (w) {
         hipLaunchKernelGGL(k_chol_build, dim3(ntiles, ntiles), dim3(256), 0, stream, jobs, st, tiles, kstep);
         (void)launch_chol_factor(w->M, w->ld, w->mp, w->mode, w->linv, w->phi, w->plo, nullptr, stream, w->aux,
