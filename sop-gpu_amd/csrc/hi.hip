@@ -117,28 +117,38 @@ __device__ __forceinline__ void eps_add(const float4 c1, const float4 c2, float 
     sum.w += p.w;
 }
 
-// Per-bead sums of the TEA epsilon (bdhitea_kernel.cu:212-249), one
-// 1024-thread workgroup per 64-bead tile of a TEA job: without lists, the
-// sixteen wavefronts form the terms of 64 partners j at a time into LDS
-// and wavefront 0 adds them to bead i's sums in j order (the same bits as
-// the bead-by-bead loop); with lists, wavefront 0 walks bead i's lists.
+// The all-pairs HI sums (epsilon, TEA step, exact D.F) run kHiSplit
+// 1024-thread workgroups per 64-bead tile, 16 beads each, so that one job
+// of a few thousand beads fills the chip.
+constexpr int kHiSplit = 4;
+
+// Per-bead sums of the TEA epsilon (bdhitea_kernel.cu:212-249): without
+// lists, for each chunk of 64 partners j every thread of the workgroup
+// forms one (bead, partner) term into LDS and lanes 0-15 add them to their
+// bead's sums in j order (the same bits as the bead-by-bead loop); with
+// lists, wavefront 0 of the tile's first workgroup walks bead i's lists.
 __global__ __launch_bounds__(1024) void k_hi_eps(const DevJob* __restrict__ jobs, const DevJobState* __restrict__ st,
                                                  const int2* __restrict__ tiles, int ntiles) {
-    __shared__ float term[4][64][65];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int2 td = tiles[blockIdx.x];
+    __shared__ float term[4][16][65];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int2 td = tiles[blockIdx.x / kHiSplit];
+    const int q = blockIdx.x % kHiSplit;
     const DevJob& J = jobs[td.x];
     if (!(J.flags & JF_HI) || (J.flags & JF_HI_EXACT)) return;
     const long long s = st[td.x].step;
     if (!(s < J.end_step && s % J.stride == 0 && s % J.hi_eps_freq == 0)) return;
-    const int d = td.y + lane;
+    const bool listed = (J.flags & JF_HI_LISTED) != 0;
+    if (listed && (q != 0 || w != 0)) return;
+    // listed: lane = bead of the 64-bead tile; unlisted: t & 15 = bead of
+    // this workgroup's 16, t >> 4 = partner of the chunk
+    const int d = listed ? td.y + lane : td.y + 16 * q + (t & 15);
     const int n = J.n;
     const float4* __restrict__ cur = J.coord[s & 1];
     const float4 c = d < n ? cur[d] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float a = J.hi_a;
     float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (J.flags & JF_HI_LISTED) {
-        if (w != 0 || d >= n) return;
+    if (listed) {
+        if (d >= n) return;
         for (int k = 0; k < J.cov_cnt[d]; ++k) eps_add(c, cur[J.cov_j[(size_t)k * n + d]], a, sum);
         for (int k = 0; k < J.nat_cnt[d]; ++k) eps_add(c, cur[J.nat_j[(size_t)k * n + d]], a, sum);
         for (int k = 0; k < J.pair_cnt[d]; ++k)
@@ -147,34 +157,33 @@ __global__ __launch_bounds__(1024) void k_hi_eps(const DevJob* __restrict__ jobs
         J.eps[d] = sum.w;
         return;
     }
+    const int b16 = t & 15, jl = t >> 4;
     for (int base = 0; base < n; base += 64) {
-        for (int jl = w; jl < 64; jl += 16) {
-            const int i = base + jl;
-            float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (i < n && i != d && d < n) {
-                float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-                eps_add(c, cur[i], a, z);
-                p = z;
-            }
-            term[0][lane][jl] = p.x;
-            term[1][lane][jl] = p.y;
-            term[2][lane][jl] = p.z;
-            term[3][lane][jl] = p.w;
+        const int i = base + jl;
+        float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (i < n && i != d && d < n) {
+            float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+            eps_add(c, cur[i], a, z);
+            p = z;
         }
+        term[0][b16][jl] = p.x;
+        term[1][b16][jl] = p.y;
+        term[2][b16][jl] = p.z;
+        term[3][b16][jl] = p.w;
         __syncthreads();
-        if (w == 0 && d < n) {
+        if (t < 16 && d < n) {
             const int m = min(64, n - base);
-            for (int jl = 0; jl < m; ++jl) {
-                if (base + jl == d) continue;
-                sum.x += term[0][lane][jl];
-                sum.y += term[1][lane][jl];
-                sum.z += term[2][lane][jl];
-                sum.w += term[3][lane][jl];
+            for (int k = 0; k < m; ++k) {
+                if (base + k == d) continue;
+                sum.x += term[0][b16][k];
+                sum.y += term[1][b16][k];
+                sum.z += term[2][b16][k];
+                sum.w += term[3][b16][k];
             }
         }
         __syncthreads();
     }
-    if (w == 0 && d < n) {
+    if (t < 16 && d < n) {
         J.ci[d] = sum;
         J.eps[d] = sum.w;
     }
@@ -222,20 +231,19 @@ __device__ __forceinline__ void tea_force(const float4 c1, const float4 c2, cons
     fz += D.xz * gx + D.yz * gy + D.zz * gz;
 }
 
-// The TEA step (bdhitea_kernel.cu:364-515).  Unlisted: kTeaSplit 1024-thread
+// The TEA step (bdhitea_kernel.cu:364-515).  Unlisted: kHiSplit 1024-thread
 // workgroups per 64-bead tile of a TEA job, each serving 16 beads, so a
 // single job of a few thousand beads still fills the chip; for each chunk of
 // 64 partners every thread forms one (bead, partner) term into LDS, then 16
 // lanes of wavefront 0 add them to their beads' sums in partner order, the
 // order of the reference's sequential loop.  Listed: wavefront 0 of the
 // tile's first workgroup walks the lists.
-constexpr int kTeaSplit = 4;
 __global__ __launch_bounds__(1024) void k_hi_tea(const DevJob* __restrict__ jobs, const DevJobState* __restrict__ st,
                                                  const int2* __restrict__ tiles, int ntiles, int kstep) {
     __shared__ float term[3][16][65];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int2 td = tiles[blockIdx.x / kTeaSplit];
-    const int q = blockIdx.x % kTeaSplit;
+    const int2 td = tiles[blockIdx.x / kHiSplit];
+    const int q = blockIdx.x % kHiSplit;
     const DevJob& J = jobs[td.x];
     if (!(J.flags & JF_HI) || (J.flags & JF_HI_EXACT)) return;
     const DevJobState& S = st[td.x];
@@ -321,7 +329,7 @@ __global__ __launch_bounds__(1024) void k_hi_tea(const DevJob* __restrict__ jobs
 // exact path
 // ---------------------------------------------------------------------------
 
-// F_i + sum_j D_ij F_j into fdf, summed in bead order.  kTeaSplit 1024-thread
+// F_i + sum_j D_ij F_j into fdf, summed in bead order.  kHiSplit 1024-thread
 // workgroups per 64-bead tile of the (single) exact-HI job of the launch,
 // 16 beads each: for each chunk of 64 beads j every thread forms one term
 // D_ij F_j (the three row sums, written exactly as the sequential loop
@@ -331,7 +339,7 @@ __global__ __launch_bounds__(1024) void k_chol_D(const DevJob* __restrict__ jobs
                                                  const int2* __restrict__ tiles, int ntiles, int kstep) {
     __shared__ float term[3][16][65];
     const int t = threadIdx.x, b16 = t & 15, jl = t >> 4;
-    const int2 td = tiles[blockIdx.x / kTeaSplit];
+    const int2 td = tiles[blockIdx.x / kHiSplit];
     const DevJob& J = jobs[td.x];
     if (!(J.flags & JF_HI_EXACT)) return;  // one job per launch: uniform
     const long long s = st[td.x].step + kstep;
@@ -339,7 +347,7 @@ __global__ __launch_bounds__(1024) void k_chol_D(const DevJob* __restrict__ jobs
     const int n = J.n;
     // this thread's bead i: one of the workgroup's 16 (the sums' owners are
     // lanes 0-15 of wavefront 0); jl: its partner in each chunk of 64
-    const int d = td.y + 16 * (blockIdx.x % kTeaSplit) + b16;
+    const int d = td.y + 16 * (blockIdx.x % kHiSplit) + b16;
     const float4* __restrict__ cur = J.coord[s & 1];
     const float4 c = d < n ? cur[d] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float a = J.hi_a;
@@ -1153,7 +1161,8 @@ __global__ __launch_bounds__(256) void k_gemm_ks(float* __restrict__ M, size_t l
 
 hipError_t launch_hi_tea_update(const DevJob* jobs, DevJobState* st, const int2* tiles, int ntiles, int njobs,
                                 hipStream_t stream) {
-    if (ntiles > 0) hipLaunchKernelGGL(k_hi_eps, dim3(ntiles), dim3(1024), 0, stream, jobs, st, tiles, ntiles);
+    if (ntiles > 0)
+        hipLaunchKernelGGL(k_hi_eps, dim3(ntiles * kHiSplit), dim3(1024), 0, stream, jobs, st, tiles, ntiles);
     hipLaunchKernelGGL(k_hi_beta, dim3((njobs + 63) / 64), dim3(64), 0, stream, jobs, st, njobs);
     return hipGetLastError();
 }
@@ -1161,7 +1170,7 @@ hipError_t launch_hi_tea_update(const DevJob* jobs, DevJobState* st, const int2*
 hipError_t launch_hi_tea_step(const DevJob* jobs, const DevJobState* st, const int2* tiles, int ntiles, int kstep,
                               hipStream_t stream) {
     if (ntiles > 0)
-        hipLaunchKernelGGL(k_hi_tea, dim3(ntiles * kTeaSplit), dim3(1024), 0, stream, jobs, st, tiles, ntiles, kstep);
+        hipLaunchKernelGGL(k_hi_tea, dim3(ntiles * kHiSplit), dim3(1024), 0, stream, jobs, st, tiles, ntiles, kstep);
     return hipGetLastError();
 }
 
@@ -1245,7 +1254,7 @@ hipError_t launch_chol_factor(float* M, size_t ld, int mp, int mode, double* lin
 
 hipError_t launch_chol_step(const DevJob* jobs, const DevJobState* st, const int2* tiles, int ntiles, int kstep,
                             const HiExactWork* w, hipStream_t stream) {
-    hipLaunchKernelGGL(k_chol_D, dim3(ntiles * kTeaSplit), dim3(1024), 0, stream, jobs, st, tiles, ntiles, kstep);
+    hipLaunchKernelGGL(k_chol_D, dim3(ntiles * kHiSplit), dim3(1024), 0, stream, jobs, st, tiles, ntiles, kstep);
     if (w) {
         hipLaunchKernelGGL(k_chol_build, dim3(ntiles, ntiles), dim3(256), 0, stream, jobs, st, tiles, kstep);
         (void)launch_chol_factor(w->M, w->ld, w->mp, w->mode, w->linv, w->phi, w->plo, nullptr, stream, w->aux,
