@@ -902,10 +902,11 @@ int Engine::plan_lds_units(std::vector<int2>& units, std::vector<int4>& segs) co
 }
 
 void Engine::drop_graph() {
-    if (graph_) {
-        (void)hipGraphExecDestroy(graph_);
-        graph_ = nullptr;
-    }
+    for (auto& g : graphs_)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
 }
 
 bool Engine::uses_cells(size_t n) const {
@@ -964,6 +965,25 @@ void Engine::profile_read(long long* launches, double* ms) {
     prof_ms_ = 0.0;
 }
 
+// The list kernels a block starting at the jobs' current steps has work for
+// (the host mirrors every job's step): bit 0 the possible-pairs build, bit 1
+// the Verlet build, bit 2 the Verlet row sort.  The kernels test the same
+// cadence per job, so leaving out the others only drops launches that would
+// return at once.
+int Engine::lists_due(int slot_only) const {
+    int m = 0;
+    for (int q = 0; q < (int)jobs_.size(); ++q) {
+        if (slot_only >= 0 && q != slot_only) continue;
+        const HostJob& hj = jobs_[q];
+        const long long s = hj.step;
+        if (s >= hj.end_step || s % hj.dj.stride != 0) continue;
+        if (s % hj.dj.pp_freq == 0) m |= 1;
+        if (s % hj.dj.pairs_freq == 0) m |= 2;
+        if (hj.dj.sort_rows && s % ((long long)hj.dj.pairs_freq * kSortEvery) == 0) m |= 4;
+    }
+    return m;
+}
+
 // One block: per-job updaters, list rebuilds, L steps, step advance.
 // slot_only >= 0 runs that job alone with its own stride (plain mode).
 int Engine::launch_block(int slot_only) {
@@ -980,28 +1000,31 @@ int Engine::launch_block(int slot_only) {
         L = jobs_[slot_only].dj.stride;
     }
     if (int rc = check(launch_job_update(jobs, st, nj, stream_), "job update")) return rc;
-    const int cw = slot_only >= 0 ? cell_words_of(jobs_[slot_only].dj) : cell_words_;
-    if (int rc = check(slot_only >= 0
-                           ? launch_possible(jobs, st, tiles, nt, nj, cw, !jobs_[slot_only].dj.lds_step, d_sunits1_,
-                                             d_segs1_,
-                                             jobs_[slot_only].dj.lds_step ? nsunits1_ : 0, jobs_[slot_only].n, 0,
-                                             stream_)
-                           : launch_possible(jobs, st, tiles, nt, nj, cw, any_global_, d_sunits_, d_segs_, nsunits_,
-                                             lds_beads_, 0,
-                                             stream_),
-                       "possible pairs"))
-        return rc;
-    const int i32 = slot_only >= 0 ? !jobs_[slot_only].dj.idx16 : any_idx32_;
-    const int nu1 = slot_only >= 0 && jobs_[slot_only].dj.sort_rows ? (jobs_[slot_only].n + kLdsUnit - 1) / kLdsUnit : 0;
-    if (int rc = check(slot_only >= 0
-                           ? launch_verlet(jobs, st, tiles, nt, i32, !jobs_[slot_only].dj.lds_step, d_units1_, nu1,
-                                           d_sunits1_, d_segs1_,
-                                           jobs_[slot_only].dj.lds_step ? nsunits1_ : 0, jobs_[slot_only].n, 0, stream_)
-                           : launch_verlet(jobs, st, tiles, nt, i32, any_global_, d_units_, nunits_, d_sunits_,
-                                           d_segs_, nsunits_,
-                                           lds_beads_, 0, stream_),
-                       "verlet"))
-        return rc;
+    const int due = lists_due(slot_only);
+    if (due & 1) {
+        const hipError_t e =
+            slot_only >= 0
+                ? launch_possible(jobs, st, tiles, nt, nj, cell_words_of(jobs_[slot_only].dj),
+                                  !jobs_[slot_only].dj.lds_step, d_sunits1_, d_segs1_,
+                                  jobs_[slot_only].dj.lds_step ? nsunits1_ : 0, jobs_[slot_only].n, 0, stream_)
+                : launch_possible(jobs, st, tiles, nt, nj, cell_words_, any_global_, d_sunits_, d_segs_, nsunits_,
+                                  lds_beads_, 0, stream_);
+        if (int rc = check(e, "possible pairs")) return rc;
+    }
+    if (due & 2) {
+        const bool sort = (due & 4) != 0;
+        const hipError_t e =
+            slot_only >= 0
+                ? launch_verlet(jobs, st, tiles, nt, !jobs_[slot_only].dj.idx16, !jobs_[slot_only].dj.lds_step,
+                                d_units1_,
+                                sort && jobs_[slot_only].dj.sort_rows ? (jobs_[slot_only].n + kLdsUnit - 1) / kLdsUnit
+                                                                      : 0,
+                                d_sunits1_, d_segs1_, jobs_[slot_only].dj.lds_step ? nsunits1_ : 0,
+                                jobs_[slot_only].n, 0, stream_)
+                : launch_verlet(jobs, st, tiles, nt, any_idx32_, any_global_, d_units_, sort ? nunits_ : 0,
+                                d_sunits_, d_segs_, nsunits_, lds_beads_, 0, stream_);
+        if (int rc = check(e, "verlet")) return rc;
+    }
     const bool tea = slot_only >= 0 ? ((jobs_[slot_only].dj.flags & JF_HI) && !(jobs_[slot_only].dj.flags & JF_HI_EXACT))
                                     : any_tea_;
     const bool exact = slot_only >= 0 ? (jobs_[slot_only].dj.flags & JF_HI_EXACT) != 0 : any_exact_;
@@ -1168,8 +1191,7 @@ int Engine::read_state_of(int id, DevJobState* out) {
     return read_state(slot, out);
 }
 
-int Engine::capture_graph() {
-    drop_graph();
+int Engine::capture_graph(int due) {
     hipGraph_t g = nullptr;
     if (int rc = check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture")) return rc;
     int rc = launch_block(-1);
@@ -1179,10 +1201,10 @@ int Engine::capture_graph() {
         return rc;
     }
     if (int r2 = check(e, "end capture")) return r2;
-    e = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
+    e = hipGraphInstantiate(&graphs_[due], g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) {
-        graph_ = nullptr;
+        graphs_[due] = nullptr;
         return check(e, "graph instantiate");
     }
     return 0;
@@ -1195,12 +1217,12 @@ int Engine::run_blocks(int nblocks) {
     // exact-HI factorisations are launched on the steps the host knows are
     // refactorisation steps, so those blocks are not replayed from a graph
     const bool graph = use_graph_ && !profiling_ && !any_exact_;
-    if (graph && !graph_) {
-        if (int rc = capture_graph()) return rc;
-    }
     for (int b = 0; b < nblocks; ++b) {
         if (graph) {
-            if (int rc = check(hipGraphLaunch(graph_, stream_), "graph launch")) return rc;
+            const int due = lists_due(-1);
+            if (!graphs_[due])
+                if (int rc = capture_graph(due)) return rc;
+            if (int rc = check(hipGraphLaunch(graphs_[due], stream_), "graph launch")) return rc;
         } else {
             if (int rc = launch_block(-1)) return rc;
         }

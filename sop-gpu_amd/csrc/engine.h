@@ -124,7 +124,8 @@ class Engine {
    private:
     int sync_tables();
     int launch_block(int slot_only);  // slot_only < 0: all jobs
-    int capture_graph();
+    int lists_due(int slot_only) const;
+    int capture_graph(int due);
     void drop_graph();
     int check(hipError_t e, const char* what);
     int read_state(int slot, DevJobState* out);
@@ -177,7 +178,8 @@ class Engine {
     int exact_mode_ = SOP_CHOL_BF16X3;
     bool any_tea_ = false, any_exact_ = false;
     bool use_graph_ = true;
-    hipGraphExec_t graph_ = nullptr;
+    // one captured block per set of list kernels it launches (lists_due())
+    hipGraphExec_t graphs_[8] = {};
     // profiling
     bool profiling_ = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool_;

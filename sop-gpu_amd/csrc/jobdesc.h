@@ -149,6 +149,8 @@ constexpr int kCellMinBeads = 8192;
 constexpr int kLdsStepBeads = 5120;
 // Beads per LDS-step unit (one 1024-thread workgroup).
 constexpr int kLdsUnit = 1024;
+// Verlet rows are re-sorted by list length at every kSortEvery-th rebuild.
+constexpr int kSortEvery = 10;
 
 // List entry k of the Verlet list (slot-major) / possible list (bead-major).
 __device__ __forceinline__ int list_at(const void* p, int idx16, size_t at) {

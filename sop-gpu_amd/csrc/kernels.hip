@@ -1024,7 +1024,6 @@ __device__ __forceinline__ void verlet_store(const DevJob& J, int i0, const IdxT
 // The sort also moves each bead's bonded lists to its new row (the step
 // reads them in row order), so it runs at every kSortEvery-th Verlet
 // rebuild only: list lengths drift little in between.
-constexpr int kSortEvery = 10;
 __global__ __launch_bounds__(1024) void k_sort_rows(const DevJob* __restrict__ jobs, const DevJobState* __restrict__ st,
                                                     const int2* __restrict__ units, int nunits, int force) {
     __shared__ unsigned long long key[kLdsUnit];
