@@ -244,6 +244,41 @@ def test_job_admitted_mid_run_matches_alone():
     np.testing.assert_array_equal(cb, rb.coords())
 
 
+def test_batched_jobs_with_different_list_cadences_match_oracle():
+    """Jobs whose Verlet / possible-pairs rebuilds fall on different steps
+    (every 10/100, 20/60 and 5/50 steps: blocks of 5 steps in which only
+    some jobs rebuild, and row sorts at different steps) stepped together
+    from captured blocks, and the first one alone: each trajectory is the
+    oracle's, bit for bit."""
+    cad = [(300, 10, 100), (400, 20, 60), (250, 5, 50)]
+    steps = 400
+    jobs = []
+    for k, (n, pf, ppf) in enumerate(cad):
+        sysm = _pull_system(n, 80 + k)
+        kw = c0_params(seed=500 + k, pulling=1, fixedEnd=0, pulledEnd=n - 1, deltax=0.02, pullFreq=100,
+                       nav=100, outputtiming=100, dcdfreq=100, pairs_freq=pf, possiblepairs_freq=ppf)
+        jobs.append((sysm, kw))
+    with S.Engine(0, 2 << 30) as e:
+        ids = [e.add_job(S.make_params(**kw), sysm, steps) for sysm, kw in jobs]
+        L = e.block_steps()
+        assert L == 5  # strides 10, 20, 5
+        ends = [(steps // st + 1) * st for st in (10, 20, 5)]  # the reference's outer loop runs 0..numsteps/stride
+        e.run_blocks(max(ends) // L)
+        got = [e.coords(j) for j in ids]
+        for j, end in zip(ids, ends):
+            assert e.status(j)[0] == end
+        a = e.add_job(S.make_params(**jobs[0][1]), jobs[0][0], steps)
+        e.run_job(a)
+        alone = e.coords(a)
+    for (sysm, kw), c in zip(jobs, got):
+        print("case: oracle, pairs_freq %d possiblepairs_freq %d" % (kw["pairs_freq"], kw["possiblepairs_freq"]),
+              flush=True)
+        r = O.Sim(O.make_params(**kw), sysm)
+        r.run(steps)
+        np.testing.assert_array_equal(c, r.coords())
+    np.testing.assert_array_equal(alone, got[0])
+
+
 def test_indentation_bit_exact(eng):
     n = 300
     sysm = numpy_chain(n, seed=50)
