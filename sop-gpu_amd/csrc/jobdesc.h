@@ -51,7 +51,7 @@ struct DevJob {
     // entry k of row p at [k*pair_ld + p]) and the wavefront that steps rows
     // 64g..64g+63 steps those beads.  For jobs with sort_rows set (stepped
     // by k_step_lds, not hydrodynamic) the beads of each kLdsUnit-row unit
-    // are re-sorted by list length at every tenth Verlet rebuild, so a
+    // are re-sorted by list length at every 20th Verlet rebuild, so a
     // wavefront's lists are of about equal length; elsewhere perm is the
     // identity.
     int* perm;
@@ -150,7 +150,7 @@ constexpr int kLdsStepBeads = 5120;
 // Beads per LDS-step unit (one 1024-thread workgroup).
 constexpr int kLdsUnit = 1024;
 // Verlet rows are re-sorted by list length at every kSortEvery-th rebuild.
-constexpr int kSortEvery = 10;
+constexpr int kSortEvery = 20;
 
 // List entry k of the Verlet list (slot-major) / possible list (bead-major).
 __device__ __forceinline__ int list_at(const void* p, int idx16, size_t at) {
