@@ -51,6 +51,7 @@ class OrParams(C.Structure):
         ("hi_on", C.c_int), ("hi_exact", C.c_int), ("hi_a", C.c_float),
         ("hi_epsilon_freq", C.c_int), ("hi_capricious", C.c_int), ("hi_unlisted", C.c_int),
         ("hi_epsmax", C.c_float), ("hi_chol_freq", C.c_int), ("hi_faithful_blocks", C.c_int),
+        ("updater_freq_min", C.c_int), ("host_forces", C.c_int),
     ]
 
 
@@ -102,6 +103,8 @@ def lib():
         sig("or_stride", C.c_int, VP)
         sig("or_get_indentation", None, VP, VP)
         sig("or_get_dat", C.c_int, VP, VP, C.c_int)
+        sig("or_get_pull_rows", C.c_int, VP, VP, C.c_int)
+        sig("or_get_ind_rows", C.c_int, VP, VP, C.c_int)
         sig("or_generate_seeds", None, C.c_int, C.c_int, VP)
         sig("or_hybrid_taus", C.c_uint32, VP)
         sig("or_uint_to_float", C.c_float, C.c_uint32)
@@ -150,7 +153,7 @@ _DEFAULTS = dict(
     indTipA=0.0, indTipB=1.0, indSurfA=0.0, indSurfB=1.0, indTipZeta=5000.0, indOutputFreq=1000,
     indRetractionStep=-1,
     hi_on=0, hi_exact=0, hi_a=1.8, hi_epsilon_freq=10, hi_capricious=1, hi_unlisted=1, hi_epsmax=999.0,
-    hi_chol_freq=1, hi_faithful_blocks=1,
+    hi_chol_freq=1, hi_faithful_blocks=1, updater_freq_min=0, host_forces=0,
 )
 
 
@@ -274,6 +277,19 @@ class Sim:
     def dat(self, cap=100000):
         o = np.zeros((cap, 6), np.float64)
         m = lib().or_get_dat(self.h, o.ctypes.data, cap)
+        return o[:min(m, cap)]
+
+    def pull_rows(self, cap=100000):
+        """Rows of the pull .dat: traj, step, |end-to-end|, end-to-end . n,
+        F . n, F x, y, z (pulling.cu:166-187)."""
+        o = np.zeros((cap, 8), np.float64)
+        m = lib().or_get_pull_rows(self.h, o.ctypes.data, cap)
+        return o[:min(m, cap)]
+
+    def ind_rows(self, cap=100000):
+        """Rows of the indentation .dat (indentation.cu:509-536)."""
+        o = np.zeros((cap, 15), np.float64)
+        m = lib().or_get_ind_rows(self.h, o.ctypes.data, cap)
         return o[:min(m, cap)]
 
     def hi_tea(self, ntr=1):

@@ -69,6 +69,12 @@ struct or_sim {
     int stride;
     /* dat rows */
     double* dat; int ndat, capdat;
+    /* pull .dat rows (traj, step, e, ex, f, fx, fy, fz) and indentation .dat
+     * rows (step, dx, |f| avg, f mod, Ks dr, Ks dr avg, f xyz, tip xyz,
+     * chip xyz), as the reference prints them */
+    double* prow; int nprow, capprow;
+    double* irow; int nirow, capirow;
+    float chipAv[3], tipAv[3], fav[4], kDeltaXAv;
     int nthreads;
     /* hydrodynamics (bdhitea.cu) */
     f4 *rforce, *mforce, *mforce0, *hcoord, *ci;
@@ -471,6 +477,7 @@ or_sim* or_create(const or_params* p, int N, int ntr, const float* coords4,
         MINF(p->dcdfreq);
         if (p->heating) MINF(p->tempFreq);
         if (p->hi_on && !p->hi_exact) MINF(p->hi_epsilon_freq);
+        if (p->updater_freq_min > 0) MINF(p->updater_freq_min); /* updaters added by the host */
 #undef MINF
         s->stride = st;
     }
@@ -485,6 +492,7 @@ void or_destroy(or_sim* s) {
     free(s->nat_cnt); free(s->nat_j); free(s->nat_r02); free(s->nat_m12);
     free(s->pair_cnt); free(s->pair_j); free(s->pp_cnt); free(s->pp_j);
     free(s->ni); free(s->nj); free(s->nr0); free(s->fixed); free(s->pulled); free(s->dat);
+    free(s->prow); free(s->irow);
     free(s->rforce); free(s->mforce); free(s->mforce0); free(s->hcoord); free(s->ci); free(s->eps); free(s->beta);
     free(s->eps_traj); free(s->L);
     free(s);
@@ -1241,6 +1249,14 @@ void or_rpy_matrix(int n, const float* c4, double a, double* D) {
 /* updaters                                                             */
 /* ------------------------------------------------------------------ */
 
+static double* push_row(double** buf, int* nrow, int* cap, int width) {
+    if (*nrow == *cap) {
+        *cap = *cap ? 2 * *cap : 64;
+        *buf = (double*)realloc(*buf, sizeof(double) * width * *cap);
+    }
+    return &(*buf)[(size_t)width * (*nrow)++];
+}
+
 /* PullingUpdater::update + updateForces + computeForce,
  * pulling.cu:152-165,198-214.  The force uses the cantilever position of
  * the previous update; the new position is set afterwards. */
@@ -1258,6 +1274,20 @@ static void pulling_update(or_sim* s) {
         for (int k = 0; k < s->npulled; k++) {
             f4* e = &s->ext[t * N + s->pulled[k]];
             e->x = s->extForce[0]; e->y = s->extForce[1]; e->z = s->extForce[2]; e->w = 2.0f;
+        }
+        /* savePullingData (pulling.cu:166-187): the row of the pass's
+         * coordinates and force, every pullFreq steps */
+        if (s->step % s->p.pullFreq == 0) {
+            const f4 a = s->coord[N * t + s->p.pulledEnd], b = s->coord[N * t + s->p.fixedEnd];
+            const float vx = a.x - b.x, vy = a.y - b.y, vz = a.z - b.z;
+            const float* pv = s->pullVec;
+            double* r = push_row(&s->prow, &s->nprow, &s->capprow, 8);
+            r[0] = t;
+            r[1] = (double)s->step;
+            r[2] = sqrtf(vx * vx + vy * vy + vz * vz);
+            r[3] = vx * pv[0] + vy * pv[1] + vz * pv[2];
+            r[4] = s->extForce[0] * pv[0] + s->extForce[1] * pv[1] + s->extForce[2] * pv[2];
+            r[5] = s->extForce[0]; r[6] = s->extForce[1]; r[7] = s->extForce[2];
         }
     }
 }
@@ -1281,6 +1311,7 @@ static void tip_update(or_sim* s) {
         for (int d = 0; d < 3; d++) s->chip[d] = s->chip0[d] + s->dir[d] * s->dx;
     }
     float dr[3] = {s->chip[0] - s->tip[0], s->chip[1] - s->tip[1], s->chip[2] - s->tip[2]};
+    const float drw = sqrtf(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
     float df[3];
     for (int d = 0; d < 3; d++) df[d] = -f[d] + dr[d] * p->indKs;
     float mult = ((p->timestep * ((float)p->nav)) / (p->indTipZeta));
@@ -1289,6 +1320,33 @@ static void tip_update(or_sim* s) {
         for (int d = 0; d < 3; d++) s->tip[d] += ct * s->dir[d];
     } else {
         for (int d = 0; d < 3; d++) s->tip[d] += mult * df[d];
+    }
+    /* averages and the output row (indentation.cu:497-536), every
+     * indentationOutputFreq steps */
+    for (int d = 0; d < 3; d++) {
+        s->chipAv[d] += s->chip[d];
+        s->tipAv[d] += s->tip[d];
+        s->fav[d] += f[d];
+    }
+    s->fav[3] += f[3];
+    s->kDeltaXAv += drw * p->indKs;
+    if (p->indOutputFreq > 0 && s->step % p->indOutputFreq == 0) {
+        const int factor = p->indOutputFreq / p->nav;
+        for (int d = 0; d < 3; d++) {
+            s->chipAv[d] /= factor;
+            s->tipAv[d] /= factor;
+            s->fav[d] /= factor;
+        }
+        s->fav[3] /= factor;
+        s->kDeltaXAv /= factor;
+        const float fmod = sqrtf(s->fav[0] * s->fav[0] + s->fav[1] * s->fav[1] + s->fav[2] * s->fav[2]);
+        double* r = push_row(&s->irow, &s->nirow, &s->capirow, 15);
+        r[0] = (double)s->step; r[1] = s->dx; r[2] = s->fav[3]; r[3] = fmod; r[4] = fabsf(drw * p->indKs);
+        r[5] = s->kDeltaXAv; r[6] = s->fav[0]; r[7] = s->fav[1]; r[8] = s->fav[2];
+        r[9] = s->tip[0]; r[10] = s->tip[1]; r[11] = s->tip[2];
+        r[12] = s->chip[0]; r[13] = s->chip[1]; r[14] = s->chip[2];
+        for (int d = 0; d < 3; d++) s->chipAv[d] = s->tipAv[d] = s->fav[d] = 0.0f;
+        s->fav[3] = s->kDeltaXAv = 0.0f;
     }
 }
 
@@ -1326,6 +1384,18 @@ static void output_update(or_sim* s) {
         if (ee != e) free(ee);
     }
     if (s->step % s->p.nav == 0) memset(s->T, 0, sizeof(float) * s->n);
+}
+
+int or_get_pull_rows(const or_sim* s, double* out, int cap) {
+    int m = s->nprow < cap ? s->nprow : cap;
+    if (m) memcpy(out, s->prow, sizeof(double) * 8 * m);
+    return s->nprow;
+}
+
+int or_get_ind_rows(const or_sim* s, double* out, int cap) {
+    int m = s->nirow < cap ? s->nirow : cap;
+    if (m) memcpy(out, s->irow, sizeof(double) * 15 * m);
+    return s->nirow;
 }
 
 int or_get_dat(const or_sim* s, double* out, int cap) {

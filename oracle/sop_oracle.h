@@ -27,6 +27,12 @@
  *   integrator / cadence    -> Euler closed form on a harmonic dimer, the
  *                              reference step count numsteps+stride.
  *   indentation             -> closed-form wall forces for a single bead.
+ *   cantilever, tip update, -> FP64 closed forms of the updater passes
+ *   heating                    (test_oracle_pins.py): k (c_prev - x), the
+ *                              tip's overdamped step, T0 + floor(s/f) dT.
+ *   TEA step, D F, L g      -> FP64 numpy statements of the step from the
+ *                              RPY matrix, LAPACK factor and pinned noise.
+ *   pull / indentation rows -> the same closed forms (values printed).
  */
 #ifndef SOP_ORACLE_H
 #define SOP_ORACLE_H
@@ -81,6 +87,11 @@ typedef struct or_params {
     float hi_epsmax;
     int hi_chol_freq;       /* Cholesky refactorisation interval (reference: every step) */
     int hi_faithful_blocks; /* apply L's 3x3 blocks as integrateCholesky_L reads them */
+    /* smallest frequency of updaters added by the host (0: none): it bounds
+     * the stride as every updater's frequency does; host_forces is not
+     * modelled (the oracle has no host potentials) */
+    int updater_freq_min;
+    int host_forces;
 } or_params;
 
 typedef struct or_sim or_sim;
@@ -142,6 +153,14 @@ void or_get_indentation(const or_sim* s, float* out9);
 /* DAT output rows recorded at outputtiming: step, T, Ecov, Enat, Erep, Q.
  * Returns the number of rows (at most cap). */
 int or_get_dat(const or_sim* s, double* out6, int cap);
+
+/* Rows of the pull .dat (pulling.cu:166-187) and indentation .dat
+ * (indentation.cu:497-536) the reference writes: per row traj, step,
+ * |end-to-end|, end-to-end . n, F . n, F xyz (8 doubles); step, dx,
+ * averaged |f|, |averaged f|, Ks |chip - tip|, its average, averaged f
+ * xyz, tip xyz, chip xyz (15 doubles).  Return the row count. */
+int or_get_pull_rows(const or_sim* s, double* out8, int cap);
+int or_get_ind_rows(const or_sim* s, double* out15, int cap);
 
 /* RNG pieces (src/Integrators/ht.cu, ran2.cpp) */
 typedef struct { int idum2, iy, iv[32]; } or_ran2_state;
