@@ -29,9 +29,20 @@ import numpy as np  # noqa: E402
 
 METRIC = "bead-steps/s on 1 GPU, batched vs plain engine; p50/p99 job latency at fixed load"
 HBM_PEAK = 8.0e12
+# The list cadence of the mix repeats every PERIOD steps: Verlet rebuilds
+# every 10, possible pairs every 100, the Verlet row sort at every 20th
+# Verlet rebuild.  Every timed window is a whole number of periods, so it
+# holds each kind of block in its steady-state share.
+PERIOD = 200
+
+
 TAU_H_PS = 250.0          # tau_H = 6 pi eta a^3 / kT at a = 3.8 A, water
 H_TAU = 0.16              # 40 ps
 PN_PER_KCAL_A = 69.48     # 1 kcal/mol/A in pN
+
+
+def whole_periods(steps):
+    return max(1, -(-int(steps) // PERIOD)) * PERIOD
 
 
 def log_uniform(rng, lo, hi, size):
@@ -109,15 +120,28 @@ def serving_mix(S, njobs, seed, steps_lo, steps_hi):
     return out
 
 
+def serving_tuning(S, mix, count=16):
+    """The autotuner on the first `count` requests of the serving mix
+    admitted together (a typical set of jobs the server holds at once)."""
+    eng = S.Engine(0, 64 << 30)
+    for kind, sysm, p, steps in mix[:count]:
+        eng.add_job(p, sysm, steps)
+    tune = eng.autotune()
+    eng.close()
+    return tune
+
+
 def serving_phase(S, args, capacity):
     mix = serving_mix(S, args.serve_jobs, args.seed + 11, args.serve_steps_lo, args.serve_steps_hi)
+    tune = serving_tuning(S, mix)
+    print(f"# serving autotune: {tune}", file=sys.stderr, flush=True)
     work = np.array([m[1].n * (m[3] // 10 + 1) * 10 for m in mix], dtype=np.float64)
     rate = args.serve_load * capacity / work.mean()  # jobs per second
     rng = np.random.default_rng(args.seed + 12)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, len(mix)))
     srv = S.Server(0, int(args.serve_budget_gb * (1 << 30)))
-    if args.no_graph:
-        srv.set_tuning(0, False)
+    srv.set_shape(tune["shape"])
+    srv.set_tuning(tune["shape"]["max_blocks"], not args.no_graph)
     tickets = [None] * len(mix)
     nxt = 0
     t0 = time.perf_counter()
@@ -153,32 +177,30 @@ def serving_phase(S, args, capacity):
         "makespan_s": makespan, "bead_steps": stats["bead_steps"],
         "bead_steps_per_s": stats["bead_steps"] / makespan,
         "hbm_budget_gb": args.serve_budget_gb, "peak_admitted_gb": stats["peak_bytes"] / 2 ** 30,
+        "tuning": tune,
     }, mix
 
 
-def plain_serial_estimate(S, mix, jobs_c2):
+def plain_serial_measured(S, mix, window):
     """Seconds the same requests take one at a time on the plain engine:
-    per-step time a + b N fitted on three plain runs of chains from the mix."""
-    pe = S.Engine(0, 8 << 30)
-    pts = []
-    for n in (500, 5000, 20000):
-        sysm = S.synth_chain(n, seed=n)
-        sysm.fixed = np.array([0], np.int32)
-        sysm.pulled = np.array([n - 1], np.int32)
-        p = job_params(S, n, 7, 1.0, 35.0)
-        jid = pe.add_job(p, sysm, 190)
+    every request is run alone for `window` steps (whole list periods, from
+    its first step) and its measured time per step is scaled to its own
+    step count."""
+    pe = S.Engine(0, 32 << 30)
+    total, measured = 0.0, 0
+    for kind, sysm, p, steps in mix:
+        run = min(whole_periods(window), (steps // 10 + 1) * 10)
+        jid = pe.add_job(p, sysm, run - 10)
         pe.synchronize()
         t0 = time.perf_counter()
         pe.run_job(jid)
         pe.synchronize()
-        pts.append((n, (time.perf_counter() - t0) / 200))
+        dt = time.perf_counter() - t0
         pe.remove_job(jid)
+        total += dt / run * ((steps // 10 + 1) * 10)
+        measured += run
     pe.close()
-    ns = np.array([q[0] for q in pts], dtype=np.float64)
-    ts = np.array([q[1] for q in pts])
-    b, a = np.polyfit(ns, ts, 1)
-    total = sum(((m[3] // 10 + 1) * 10) * (a + b * m[1].n) for m in mix)
-    return total, {"a_ms": a * 1e3, "b_us_per_bead": b * 1e6}
+    return total, measured
 
 
 def rpy_matrix(coords, a):
@@ -279,7 +301,8 @@ def capsid_phase(S, steps=2000):
                       indTipR=tip_r, indKs=0.07, indDeltaX=dx, indTipZeta=5000.0, indTipA=0.0, indTipB=1.0)
     eng = S.Engine(0, 32 << 30)
     jid = eng.add_job(p, sysm, steps + 100)
-    eng.set_tuning(0, True)
+    tune = eng.autotune()  # the job does not advance
+    eng.set_tuning(tune["shape"]["max_blocks"], True)
     eng.run_blocks(10)  # warm-up: 100 steps incl. one possible-pairs rebuild
     eng.synchronize()
     L = eng.block_steps()
@@ -289,7 +312,7 @@ def capsid_phase(S, steps=2000):
     eng.synchronize()
     dt = time.perf_counter() - t0
     out = {"beads": sysm.n, "natives": int(len(sysm.nat_r0)), "steps": nb * L,
-           "bead_steps_per_s": sysm.n * nb * L / dt, "ms_per_step": dt / (nb * L) * 1e3}
+           "bead_steps_per_s": sysm.n * nb * L / dt, "ms_per_step": dt / (nb * L) * 1e3, "tuning": tune}
     for cells, name in ((True, "possible_rebuild_ms_cells"), (False, "possible_rebuild_ms_all_pairs")):
         eng.set_cells(enabled=cells)
         eng.rebuild(jid, 1)
@@ -313,53 +336,129 @@ def bytes_per_bead_step(avg):
     return 32 + 32 + 12 + 8 * n_b + 12 * n_n + 4 * n_p + 8 + 16
 
 
-def cpu_baseline(jobs, target_s, threads):
-    """The oracle on a bounded sample of the mix (test infrastructure)."""
+def cpu_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return os.cpu_count() or 1, model
+
+
+def _oracle_params(O, p):
+    return O.make_params(**{f: list(getattr(p, f)) if hasattr(getattr(p, f), "__len__") else getattr(p, f)
+                            for f, _ in O.OrParams._fields_})
+
+
+def cpu_rate(O, sysm, p, target_s, threads):
+    """bead-steps/s of the oracle (the reference kernel bodies as host
+    loops, OpenMP over beads) stepping one job: a first 200-step list
+    period sizes a window of whole periods of about target_s seconds."""
+    sim = O.Sim(_oracle_params(O, p), sysm)
+    t = sim.bench(PERIOD, threads)
+    periods = max(1, int(target_s / max(t, 1e-6)))
+    t = sim.bench(periods * PERIOD, threads)
+    return sysm.n * periods * PERIOD / t, periods * PERIOD
+
+
+def cpu_baseline(jobs, c0, target_s, threads):
+    """The oracle on this host (test infrastructure), all threads and one
+    thread: four jobs of the 512-chain mix run one after another, and the
+    single N = 1,000 chain."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
     rng = np.random.default_rng(7)
     pick = rng.choice(len(jobs), size=min(4, len(jobs)), replace=False)
-    total_bs, total_t = 0.0, 0.0
-    for k in pick:
-        sysm, p = jobs[k]
-        op = O.make_params(**{f: getattr(p, f) if not hasattr(getattr(p, f), "__len__") else list(getattr(p, f))
-                              for f, _ in O.OrParams._fields_})
-        sim = O.Sim(op, sysm)
-        t = sim.bench(100, threads)  # one possible-pairs and ten Verlet rebuilds
-        per_step = max(t / 100, 1e-6)
-        steps = max(10, int(target_s / len(pick) / per_step))
-        t = sim.bench(steps, threads)
-        total_bs += sysm.n * steps
-        total_t += t
-    sample = f"{len(pick)} jobs of the mix (N={','.join(str(jobs[k][0].n) for k in pick)}), " \
-             f"~{target_s:.0f} s of stepping incl. list rebuilds"
-    return total_bs / total_t, sample
+    out = {}
+    for nt in (threads, 1):
+        bs, tt = 0.0, 0.0
+        for k in pick:
+            sysm, p = jobs[k]
+            r, steps = cpu_rate(O, sysm, p, target_s / len(pick) / 2, nt)
+            bs += sysm.n * steps
+            tt += sysm.n * steps / r
+        out[nt] = bs / tt
+    c0_all, c0_steps = cpu_rate(O, c0[0], c0[1], target_s / 4, threads)
+    c0_one, _ = cpu_rate(O, c0[0], c0[1], target_s / 4, 1)
+    nproc, model = cpu_info()
+    sample = (f"{len(pick)} jobs of the mix (N={','.join(str(jobs[k][0].n) for k in pick)}) one after another, "
+              f"whole 200-step list periods (~{target_s / 2:.0f} s per thread count, rebuilds included)")
+    return {
+        "value": out[threads], "unit": "bead-steps/s", "cores": threads, "kind": "port", "sample": sample,
+        "one_thread_value": out[1], "nproc": nproc, "cpu_model": model,
+        "c0": {"n_beads": c0[0].n, "value_all_threads": c0_all, "value_one_thread": c0_one,
+               "sample_steps": c0_steps, "note": "10^5-step run timed on a sample of whole periods"},
+    }
+
+
+def c0_system(S):
+    """One chain of N = 1,000 beads (seeded compact globule, natives
+    |i-j| > 2, r < 8 A), bead 0 fixed and bead 999 pulled at 1 um/s by a
+    35 pN/nm spring, T = 300 K, h = 40 ps."""
+    n = 1000
+    sysm = S.synth_chain(n, seed=1000)
+    sysm.fixed = np.array([0], np.int32)
+    sysm.pulled = np.array([n - 1], np.int32)
+    return sysm, job_params(S, n, 1000, 1.0, 35.0)
+
+
+def c0_phase(S, c0, steps):
+    """The N = 1,000 chain for `steps` steps alone on the GPU: the plain
+    engine (one launch per kernel) and the batched engine with one job
+    (graph replay)."""
+    sysm, p = c0
+    out = {"n_beads": sysm.n, "steps": steps}
+    for name, graph in (("plain", None), ("batched_graph", True)):
+        eng = S.Engine(0, 4 << 30)
+        jid = eng.add_job(p, sysm, steps - 10)
+        eng.synchronize()
+        t0 = time.perf_counter()
+        if graph:
+            eng.set_tuning(0, True)
+            eng.run_blocks(steps // eng.block_steps())
+        else:
+            eng.run_job(jid)
+        eng.synchronize()
+        dt = time.perf_counter() - t0
+        assert eng.status(jid)[0] == steps
+        eng.close()
+        out[name] = {"bead_steps_per_s": sysm.n * steps / dt, "us_per_step": dt / steps * 1e6}
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4000)
-    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=4000,
+                    help="timed steps, rounded up to whole 200-step list periods")
+    ap.add_argument("--warmup", type=int, default=200, help="untimed steps, rounded up to whole list periods")
     ap.add_argument("--jobs", type=int, default=512)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--plain-jobs", type=int, default=12)
     ap.add_argument("--plain-steps", type=int, default=400)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=16.0)
+    ap.add_argument("--c0-steps", type=int, default=100000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-hi", action="store_true")
     ap.add_argument("--no-capsid", action="store_true")
+    ap.add_argument("--no-c0", action="store_true")
     ap.add_argument("--hi-only", action="store_true",
                     help="run only the exact-HI phase (Cholesky timing and integrator, no covariance check) "
                          "and print its JSON; for profiling the factorisation")
-    ap.add_argument("--serve-jobs", type=int, default=60)
-    ap.add_argument("--serve-steps-lo", type=int, default=2000)
-    ap.add_argument("--serve-steps-hi", type=int, default=20000)
+    ap.add_argument("--serve-jobs", type=int, default=40)
+    ap.add_argument("--serve-steps-lo", type=int, default=10000)
+    ap.add_argument("--serve-steps-hi", type=int, default=100000)
     ap.add_argument("--serve-load", type=float, default=0.6)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--serve-plain-window", type=int, default=1000,
+                    help="steps each serving request is timed alone on the plain engine")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build"}))
@@ -380,18 +479,23 @@ def main():
     copy_gbps = S.copy_bandwidth(1 << 30, 20)
     print(f"# streaming copy: {copy_gbps:.0f} GB/s", file=sys.stderr, flush=True)
     eng = S.Engine(0, 96 << 30)
-    ids = [eng.add_job(p, sysm, 100000) for sysm, p in jobs]
+    ids = [eng.add_job(p, sysm, 1000000) for sysm, p in jobs]
     L = eng.block_steps()
-    tuned = eng.autotune()
+    assert PERIOD % L == 0
+    tune = eng.autotune()  # the jobs do not advance
+    tuned = tune["shape"]["max_blocks"]
     eng.set_tuning(tuned, not args.no_graph)
-    warm_blocks = max(1, args.warmup // L)
-    eng.run_blocks(warm_blocks)
+    print(f"# autotune: {tune}", file=sys.stderr, flush=True)
+    # warm-up: whole list periods, so every block variant (Verlet only,
+    # with possible pairs, with the row sort) is captured before timing
+    warm = whole_periods(args.warmup)
+    eng.run_blocks(warm // L)
     eng.synchronize()
-    nblocks = max(1, args.steps // L)
-    steps = nblocks * L
-    print(f"# L={L} tuned max_blocks={tuned}; timing {steps} steps", file=sys.stderr, flush=True)
+    steps = whole_periods(args.steps)
+    nblocks = steps // L
+    print(f"# L={L}; warm-up {warm} steps; timing {steps} steps", file=sys.stderr, flush=True)
 
-    # timed region: graph replays
+    # timed region: graph replays over whole list periods
     eng.synchronize()
     t0 = time.perf_counter()
     eng.run_blocks(nblocks)
@@ -399,11 +503,12 @@ def main():
     dt = time.perf_counter() - t0
     value = beads * steps / dt
 
-    # roofline: the same blocks with HIP events around every step-kernel
-    # launch on the engine's stream
+    # roofline: whole periods again, eager, HIP events on the engine's
+    # stream around every block's ten step-kernel launches (nothing else is
+    # launched between them)
     eng.profile(True)
-    prof_blocks = max(1, min(nblocks, 200))
-    eng.run_blocks(prof_blocks)
+    prof_steps = whole_periods(min(steps, 2000))
+    eng.run_blocks(prof_steps // L)
     eng.synchronize()
     launches, ms = eng.profile_read()
     eng.profile(False)
@@ -423,20 +528,22 @@ def main():
             traffic = None
     eng.close()
 
-    # plain engine: a seeded sample of the same jobs, each run alone
+    # plain engine: a seeded sample of the same jobs, each run alone over
+    # whole list periods from its first step
     rng = np.random.default_rng(args.seed + 1)
     pick = rng.choice(len(jobs), size=min(args.plain_jobs, len(jobs)), replace=False)
+    plain_steps = whole_periods(args.plain_steps)
     pe = S.Engine(0, 32 << 30)
     plain_bs, plain_t = 0.0, 0.0
     for k in pick:
         sysm, p = jobs[k]
-        jid = pe.add_job(p, sysm, args.plain_steps - 10)
+        jid = pe.add_job(p, sysm, plain_steps - 10)
         pe.synchronize()
         t0 = time.perf_counter()
         pe.run_job(jid)
         pe.synchronize()
         plain_t += time.perf_counter() - t0
-        plain_bs += sysm.n * args.plain_steps
+        plain_bs += sysm.n * plain_steps
         pe.remove_job(jid)
     pe.close()
     plain_value = plain_bs / plain_t
@@ -445,12 +552,20 @@ def main():
     if not args.no_serve:
         print("# serving phase", file=sys.stderr, flush=True)
         serving, smix = serving_phase(S, args, value)
-        serial_s, model = plain_serial_estimate(S, smix, jobs)
-        serving["plain_serial_estimate_s"] = serial_s
-        serving["plain_model"] = model
+        serial_s, measured = plain_serial_measured(S, smix, args.serve_plain_window)
+        serving["plain_serial_s"] = serial_s
+        serving["plain_serial_method"] = (f"each request alone on the plain engine for min(its steps, "
+                                          f"{whole_periods(args.serve_plain_window)}) steps, time per step "
+                                          f"scaled to its step count ({measured} steps timed)")
         serving["speedup_vs_plain_serial"] = serial_s / serving["makespan_s"]
-        print(f"# serving: p50 {serving['p50_latency_s']:.3f} s p99 {serving['p99_latency_s']:.3f} s",
-              file=sys.stderr, flush=True)
+        print(f"# serving: p50 {serving['p50_latency_s']:.3f} s p99 {serving['p99_latency_s']:.3f} s, "
+              f"{serving['speedup_vs_plain_serial']:.2f}x serial plain", file=sys.stderr, flush=True)
+
+    c0 = c0_system(S)
+    c0_gpu = None
+    if not args.no_c0:
+        print("# C0 phase", file=sys.stderr, flush=True)
+        c0_gpu = c0_phase(S, c0, args.c0_steps)
 
     capsid = None
     if not args.no_capsid:
@@ -464,9 +579,9 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        cv, sample = cpu_baseline(jobs, args.cpu_seconds, threads)
-        cpu = {"value": cv, "unit": "bead-steps/s", "cores": threads, "kind": "port", "sample": sample}
+        print("# CPU baseline", file=sys.stderr, flush=True)
+        threads = os.cpu_count() or 1
+        cpu = cpu_baseline(jobs, c0, args.cpu_seconds, threads)
 
     out = {
         "metric": METRIC,
@@ -474,7 +589,7 @@ def main():
         "unit": "bead-steps/s",
         "n_gpus": 1,
         "steps": steps,
-        "warmup": warm_blocks * L,
+        "warmup": warm,
         "ms_per_step": dt / steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
@@ -485,17 +600,25 @@ def main():
             "workload": "batched replicas: 512 SOP chains, N log-uniform [200,5000], force-ramp 0.1-10 um/s, "
                         "10-70 pN/nm, T=300K, h=40ps, Verlet/possible rebuild 10/100 steps",
             "jobs": len(jobs), "beads": beads, "block_steps": L, "max_blocks": tuned,
-            "graph": not args.no_graph,
+            "graph": not args.no_graph, "steps_requested": args.steps, "warmup_requested": args.warmup,
+            "tuning": tune,
+            "window": f"whole {PERIOD}-step list periods (steps and warm-up rounded up)",
         },
-        "plain": {"value": plain_value, "jobs_sampled": int(len(pick)), "steps_per_job": args.plain_steps,
+        "plain": {"value": plain_value, "jobs_sampled": int(len(pick)), "steps_per_job": plain_steps,
                   "batched_over_plain": value / plain_value},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": traffic, "kernel": kernel,
-                     "avg_launch_ms": avg_ms, "launches": launches, "bytes_per_bead_step": bpb,
+                     "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "traffic_source": "profiles/step_kernel_traffic.json (rocprofv3 --pmc FETCH_SIZE and "
+                                       "WRITE_SIZE of this bench, scaled by the copy kernel's counters; per bead "
+                                       "x beads of this run)",
+                     "kernel": kernel, "avg_launch_ms": avg_ms, "launches": launches,
+                     "timing": f"HIP events around each block's step launches, {prof_steps} eager steps",
+                     "bytes_per_bead_step": bpb,
                      "copy_kernel_gbps": copy_gbps, "frac_of_copy": achieved / (copy_gbps * 1e9),
                      "avg_lists": {"covalent": avg_lists[0], "native": avg_lists[1], "verlet": avg_lists[2],
                                    "possible": avg_lists[3]}},
         "serving": serving,
+        "c0_single_chain": c0_gpu,
         "capsid_indentation": capsid,
         "hi_cholesky": hi,
         "cpu_baseline": cpu,

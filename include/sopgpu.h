@@ -20,6 +20,16 @@
  *   energies        SOPPotential::computeEnergy / sumEnergies gsop.cu:255-271
  *   RNG             initRand / generateSeeds / rforce, src/Integrators/ht.cu
  *   config runs     sop-gpu main(), src/sop-gpu.cpp:18-65
+ *   plugins         SOPPotential / SOPUpdater / SOPIntegrator registries
+ *                   and potentialByName / updaterByName, src/gsop.h:69-105,
+ *                   src/gsop.cu:31-34,80-101,237-253 (C++ classes over
+ *                   these entry points: sop_plugin.h)
+ *
+ * Threading: a handle (engine, server, simulation) is used by one host
+ * thread at a time -- none of them is thread-safe, as the reference's global
+ * state is not; use one handle per thread.  The error message of
+ * sop_last_error() is kept per thread.  Handles on the same GPU may live in
+ * different threads: each has its own stream and device memory.
  */
 #ifndef SOPGPU_H
 #define SOPGPU_H
@@ -81,6 +91,12 @@ typedef struct sop_params {
     float hi_epsmax;
     int hi_chol_freq;       /* refactorise every k steps (reference: 1) */
     int hi_faithful_blocks; /* apply L's 3x3 blocks as integrateCholesky_L reads them */
+    /* host plugins (sop_plugin.h): the smallest frequency of updaters the
+     * engine does not run (0: none), which bounds the stride as every
+     * updater's frequency does (src/gsop.cu:116-122); host_forces != 0: a
+     * host potential adds forces every step (sop_sim_add_forces) */
+    int updater_freq_min;
+    int host_forces;
 } sop_params;
 
 /* Topology and coordinates of one system in .top order
@@ -146,8 +162,28 @@ int sop_engine_set_cells(sop_engine* e, int min_beads, int enabled);
  * coordinates staged in LDS (default, enabled = 1) or, enabled = 0, with
  * every neighbour gather from global memory.  Results are identical. */
 int sop_engine_set_lds_step(sop_engine* e, int enabled);
-/* Pick max_blocks by timing candidate grids on the admitted jobs. */
-int sop_engine_autotune(sop_engine* e, int* max_blocks_out);
+/* Launch shapes of the step and list kernels; none changes a result (the
+ * trajectories are the same bits under every shape).  No counterpart in the
+ * reference, whose launches are one thread per bead in blocks of
+ * `block_size` (src/gsop.cuh:13). */
+typedef struct sop_tuning {
+    int max_blocks;   /* persistent grid of the global-gather step kernel (jobs above 5120 beads; 0: one
+                         workgroup per 4 tiles) */
+    int unit_rows;    /* Verlet rows per LDS step unit, the threads of its workgroup: 512, 768 or 1024 */
+    int shared_units; /* rows left over by LDS jobs packed together into shared units (1) or one job per unit (0) */
+    int pair_batch;   /* Verlet entries fetched ahead per lane in the LDS step: 4, 6 or 8 */
+    int verlet_chunk; /* 64-entry chunks of a possible-list row in flight in the LDS Verlet build: 4 or 8 */
+    int sort_every;   /* Verlet rows re-sorted by length at every k-th Verlet rebuild */
+} sop_tuning;
+int sop_engine_get_shape(sop_engine* e, sop_tuning* out);
+int sop_engine_set_shape(sop_engine* e, const sop_tuning* shape);
+/* Searches the shapes on the admitted jobs and keeps the fastest: each
+ * candidate runs real blocks (whole periods of the jobs' list cadences)
+ * from a device snapshot of every job, restored afterwards, so the jobs do
+ * not advance.  Reports the chosen shape, the ms per step of the shape the
+ * search started from and of the chosen one, and the candidates timed. */
+int sop_engine_autotune(sop_engine* e, sop_tuning* best, double* ms_per_step_start, double* ms_per_step_best,
+                        int* trials);
 
 /* Batched stepping: every unfinished job advances one block of L steps
  * (L = gcd of the jobs' strides) per call, all jobs in one step kernel
@@ -237,6 +273,59 @@ int sop_check_division(int device, unsigned int lo, unsigned int hi, const float
  * parameter, topology or coordinate file. */
 int sop_run_config(const char* path, int noverride, const char* const* names, const char* const* values);
 
+/* The sop-gpu program with its plugin registries (src/gsop.cu:45-163).
+ * sop_sim_create reads the parameter file (names/values override it), the
+ * topology and coordinates, and builds the built-in potentials, updaters and
+ * integrator in initFF's order (gsop.cu:80-101); the host may then append
+ * its own plugins; sop_sim_run admits the trajectories on the GPU, runs
+ * runGPU's loop and writes the outputs; it may be called once.  Errors:
+ * SOP_E_CONFIG with the message (a bad parameter, a plugin's failure). */
+typedef struct sop_sim sop_sim;
+int sop_sim_create(const char* path, int noverride, const char* const* names, const char* const* values,
+                   sop_sim** out);
+int sop_sim_run(sop_sim* s);
+void sop_sim_destroy(sop_sim* s);
+
+/* Host plugins (SOPUpdater / SOPPotential, src/gsop.h:69-94), appended to
+ * the registries in call order.  update() is called every stride steps and
+ * checks its own frequency (sop_sim_state), compute() every step before
+ * the integrator; a nonzero return ends the run with the message given to
+ * sop_sim_fail().  energy(traj, id) gives the potential's term id of
+ * trajectory traj for the energy .dat (energies_count terms, 0 for none).
+ * release(user), if given, is called when the simulation is destroyed, or
+ * at once when registration fails.  Registration is refused once running. */
+typedef int (*sop_plugin_fn)(sop_sim* s, void* user);
+typedef float (*sop_energy_fn)(sop_sim* s, int traj, int id, void* user);
+typedef void (*sop_release_fn)(void* user);
+int sop_sim_add_updater(sop_sim* s, const char* name, int frequency, sop_plugin_fn update, void* user,
+                        sop_release_fn release);
+int sop_sim_add_potential(sop_sim* s, const char* name, int energies_count, sop_plugin_fn compute,
+                          sop_energy_fn energy, void* user, sop_release_fn release);
+/* A plugin's failure message (the DIE() of the reference). */
+void sop_sim_fail(const char* message);
+
+/* The registries: counts, names and frequencies in order, and the lookups
+ * of potentialByName / updaterByName (SOP_E_ARG "Unable to find requested
+ * potential 'x'" when absent); the integrator's name and time step. */
+int sop_sim_potential_count(const sop_sim* s, int* count);
+int sop_sim_updater_count(const sop_sim* s, int* count);
+const char* sop_sim_potential_name(const sop_sim* s, int index);
+const char* sop_sim_updater_name(const sop_sim* s, int index);
+int sop_sim_updater_frequency(const sop_sim* s, int index, int* frequency);
+int sop_sim_potential_by_name(const sop_sim* s, const char* name, int* index);
+int sop_sim_updater_by_name(const sop_sim* s, const char* name, int* index);
+const char* sop_sim_integrator_name(const sop_sim* s);
+/* SOPPotential::getEnergy of potential `index` at the last output step. */
+int sop_sim_get_energy(sop_sim* s, int index, int traj, int id, float* energy);
+
+/* State for plugins: the current step, beads per trajectory and
+ * trajectories; the coordinates of trajectory traj now (n x 4); forces to
+ * add to trajectory traj in the current step (n x 4, w ignored; summed over
+ * the potentials that add them, applied by the step's integrator). */
+int sop_sim_state(const sop_sim* s, long long* step, int* n, int* ntr);
+int sop_sim_get_coords(sop_sim* s, int traj, float* out4);
+int sop_sim_add_forces(sop_sim* s, int traj, const float* f4);
+
 /* Continuous-batching request loop (no counterpart in the reference, which
  * runs one system or identical replicas per process, src/sop-gpu.cpp:18-65).
  * submit copies the request and queues it; pump admits queued requests in
@@ -248,6 +337,11 @@ typedef struct sop_server sop_server;
 int sop_server_create(int device, unsigned long long hbm_budget, sop_server** out);
 void sop_server_destroy(sop_server* s);
 int sop_server_set_tuning(sop_server* s, int max_blocks, int use_graph);
+/* The launch shapes of the server's engine (sop_engine_set_shape), and
+ * the autotuner on the jobs admitted so far (sop_engine_autotune). */
+int sop_server_set_shape(sop_server* s, const sop_tuning* shape);
+int sop_server_autotune(sop_server* s, sop_tuning* best, double* ms_per_step_start, double* ms_per_step_best,
+                        int* trials);
 int sop_server_submit(sop_server* s, const sop_params* p, const sop_system* sys, long long numsteps, int* ticket);
 int sop_server_pump(sop_server* s, int nblocks, int* running, int* queued);
 int sop_server_poll(sop_server* s, int ticket, int* state, double* t_submit, double* t_start, double* t_done);

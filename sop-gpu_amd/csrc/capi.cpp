@@ -80,6 +80,7 @@ int sop_engine_job_bytes(const sop_params* p, const sop_system* s, unsigned long
 
 int sop_engine_set_tuning(sop_engine* e, int max_blocks, int use_graph) {
     ENG(e);
+    if (max_blocks < 0) return sop::fail(SOP_E_ARG, "max_blocks < 0");
     E.set_tuning(max_blocks, use_graph);
     return 0;
 }
@@ -96,12 +97,36 @@ int sop_engine_set_cells(sop_engine* e, int min_beads, int enabled) {
     return 0;
 }
 
-int sop_engine_autotune(sop_engine* e, int* max_blocks_out) {
+static sop_tuning to_c(const sop::Tuning& t) {
+    return sop_tuning{t.max_blocks, t.unit_rows, t.shared_units, t.pair_batch, t.verlet_chunk, t.sort_every};
+}
+
+int sop_engine_get_shape(sop_engine* e, sop_tuning* out) {
     ENG(e);
-    int b = 0;
-    int rc = E.autotune(&b);
-    if (max_blocks_out) *max_blocks_out = b;
-    return rc;
+    if (!out) return sop::fail(SOP_E_ARG, "null output");
+    *out = to_c(E.tuning());
+    return 0;
+}
+
+int sop_engine_set_shape(sop_engine* e, const sop_tuning* t) {
+    ENG(e);
+    if (!t) return sop::fail(SOP_E_ARG, "null shape");
+    return E.set_shape(sop::Tuning{t->max_blocks, t->unit_rows, t->shared_units, t->pair_batch, t->verlet_chunk,
+                                   t->sort_every});
+}
+
+int sop_engine_autotune(sop_engine* e, sop_tuning* best, double* ms_start, double* ms_best, int* trials) {
+    ENG(e);
+    sop::Tuning t{};
+    double a = 0.0, b = 0.0;
+    int n = 0;
+    const int rc = E.autotune(&t, &a, &b, &n);
+    if (rc) return rc;
+    if (best) *best = to_c(t);
+    if (ms_start) *ms_start = a;
+    if (ms_best) *ms_best = b;
+    if (trials) *trials = n;
+    return 0;
 }
 
 int sop_engine_run_blocks(sop_engine* e, int nblocks) {

@@ -49,7 +49,8 @@ static constexpr size_t kAlign = 256;
 static constexpr size_t kChunk = size_t(1) << 30;
 
 Arena::~Arena() {
-    for (auto& c : chunks_) (void)hipFree(c.base);
+    for (auto& c : chunks_)
+        if (c.base) (void)hipFree(c.base);
 }
 
 void* Arena::alloc(size_t bytes) {
@@ -57,6 +58,7 @@ void* Arena::alloc(size_t bytes) {
     if (bytes == 0) bytes = kAlign;
     for (size_t ci = 0; ci < chunks_.size(); ++ci) {
         auto& c = chunks_[ci];
+        if (!c.base) continue;  // trimmed
         for (auto it = c.free.begin(); it != c.free.end(); ++it) {
             if (it->second >= bytes) {
                 size_t off = it->first, len = it->second;
@@ -64,6 +66,7 @@ void* Arena::alloc(size_t bytes) {
                 if (len > bytes) c.free[off + bytes] = len - bytes;
                 void* p = c.base + off;
                 live_[p] = {(int)ci, bytes};
+                c.live++;
                 in_use_ += bytes;
                 return p;
             }
@@ -80,6 +83,7 @@ void* Arena::alloc(size_t bytes) {
     Chunk c;
     c.base = base;
     c.size = want;
+    c.live = 1;
     if (want > bytes) c.free[bytes] = want - bytes;
     chunks_.push_back(c);
     reserved_ += want;
@@ -95,6 +99,7 @@ void Arena::release(void* p) {
     auto& c = chunks_[it->second.first];
     size_t off = (char*)p - c.base, len = it->second.second;
     in_use_ -= len;
+    c.live--;
     live_.erase(it);
     auto nx = c.free.lower_bound(off);
     if (nx != c.free.end() && off + len == nx->first) {
@@ -109,6 +114,20 @@ void Arena::release(void* p) {
         }
     }
     c.free[off] = len;
+}
+
+size_t Arena::trim() {
+    size_t freed = 0;
+    for (auto& c : chunks_) {
+        if (!c.base || c.live) continue;
+        (void)hipFree(c.base);
+        c.base = nullptr;
+        c.free.clear();
+        reserved_ -= c.size;
+        freed += c.size;
+        c.size = 0;
+    }
+    return freed;
 }
 
 // Largest float x with sqrtf(x) < t (-1 when no x >= 0 qualifies).
@@ -448,6 +467,9 @@ static int build_job(const sop_params& p, const sop_system& s, long long numstep
     mn(p.dcdfreq);
     if (p.heating) mn(p.tempFreq);
     if (p.hi_on && !p.hi_exact) mn(p.hi_epsilon_freq);
+    if (p.updater_freq_min < 0) return fail(SOP_E_ARG, "updater frequencies must be positive");
+    if (p.updater_freq_min > 0) mn(p.updater_freq_min);  // updaters run by the host
+    if (p.host_forces) J.flags |= JF_EXTRA_FORCE;
     J.stride = st;
     if (numsteps < 0) return fail(SOP_E_ARG, "numsteps must be >= 0");
     // the outer loop runs i = 0 .. numsteps/stride inclusive (gsop.cu:129)
@@ -484,6 +506,7 @@ static unsigned long long built_bytes(const BuiltJob& b, bool pulling, bool inde
     t += round_up((n + 63) / 64 * 64 * b.pair_cap * es) + round_up(n * b.pp_ld * es) + round_up(n * b.excl_cap * 4);
     t += round_up(std::max<size_t>(1, b.pulled.size()) * 4);
     t += round_up(n * 12);                // energies
+    if (b.dj.flags & JF_EXTRA_FORCE) t += round_up(n * 16);  // forces of host potentials
     if (n >= (unsigned long long)kCellMinBeads && n <= 64ull * kMaxCellWords)
         t += round_up((n + 64) * 4) + round_up((n + 65) * 4) + 3 * round_up(n * 4);  // cell grid
     if (b.dj.flags & JF_HI) {
@@ -515,8 +538,12 @@ int Engine::add_job(const sop_params& p, const sop_system& s, long long numsteps
     bool oom = false;
     auto A = [&](size_t bytes) -> void* {
         void* q = arena_.alloc(bytes);
-        if (!q) oom = true;
-        else hj.blocks.push_back(q);
+        if (!q) {
+            oom = true;
+        } else {
+            hj.blocks.push_back(q);
+            hj.sizes.push_back(bytes);
+        }
         return q;
     };
     DevJob& J = b.dj;
@@ -528,7 +555,7 @@ int Engine::add_job(const sop_params& p, const sop_system& s, long long numsteps
     J.ext = p.pulling ? (float4*)A(n * 16) : nullptr;
     J.bflag = p.pulling ? (unsigned char*)A(n) : nullptr;
     J.tipF = p.indentation ? (float4*)A(n * 16) : nullptr;
-    J.fextra = nullptr;
+    J.fextra = (J.flags & JF_EXTRA_FORCE) ? (float4*)A(n * 16) : nullptr;
     J.cov_cnt = (int*)A(n * 4);
     J.cov_j = (int*)A(n * b.cov_cap * 4);
     J.cov_j16 = J.idx16 ? (unsigned short*)A(n * b.cov_cap * 2) : nullptr;
@@ -618,6 +645,7 @@ int Engine::add_job(const sop_params& p, const sop_system& s, long long numsteps
         H2D(J.bflag, flags.data(), flags.size());
     }
     if (J.tipF) H2D(J.tipF, zeros.data(), n * 16);
+    if (J.fextra) H2D(J.fextra, zeros.data(), n * 16);
     H2D(J.cov_cnt, b.cov_cnt.data(), nbytes(b.cov_cnt));
     H2D(J.cov_j, b.cov_j.data(), nbytes(b.cov_j));
     std::vector<unsigned short> cov16, nat16;
@@ -843,11 +871,12 @@ int Engine::sync_tables() {
     return 0;
 }
 
-// The LDS step plan of the LDS-step jobs: each whole kLdsUnit-row block of
-// a job is a unit; the rows left over (or a whole small job) are packed,
-// largest first, into units of at most kLdsUnit threads (segments rounded
-// up to 64 rows) and kLdsStepBeads staged beads.  Returns the largest
-// number of beads a unit stages.
+// The LDS step plan of the LDS-step jobs: each whole block of unit_rows
+// rows of a job is a unit; the rows left over (or a whole small job) are
+// packed, largest first, into units of at most unit_rows threads (segments
+// rounded up to 64 rows) and kLdsStepBeads staged beads -- or, without
+// shared units, each gets a unit of its own.  Returns the largest number of
+// beads a unit stages.
 int Engine::plan_lds_units(std::vector<int2>& units, std::vector<int4>& segs) const {
     struct Part {
         int job, row, rows, n;
@@ -856,13 +885,14 @@ int Engine::plan_lds_units(std::vector<int2>& units, std::vector<int4>& segs) co
     int lds = 0;
     units.clear();
     segs.clear();
+    const int U = shape_.unit_rows;
     for (int k = 0; k < (int)jobs_.size(); ++k) {
         const DevJob& J = jobs_[k].dj;
         if (!J.lds_step) continue;
         int r = 0;
-        for (; r + kLdsUnit <= J.n; r += kLdsUnit) {
+        for (; r + U <= J.n; r += U) {
             units.push_back(make_int2((int)segs.size(), 1));
-            segs.push_back(make_int4(k, r, kLdsUnit, 0));
+            segs.push_back(make_int4(k, r, U, 0));
             lds = std::max(lds, J.n);
         }
         if (r < J.n) parts.push_back({k, r, J.n - r, J.n});
@@ -877,7 +907,7 @@ int Engine::plan_lds_units(std::vector<int2>& units, std::vector<int4>& segs) co
         const int span = (parts[q].rows + 63) / 64 * 64;
         Bin* fit = nullptr;
         for (auto& b : bins)
-            if (b.threads + span <= kLdsUnit && b.beads + parts[q].n <= kLdsStepBeads) {
+            if (shared_units_ && b.threads + span <= U && b.beads + parts[q].n <= kLdsStepBeads) {
                 fit = &b;
                 break;
             }
@@ -979,7 +1009,7 @@ int Engine::lists_due(int slot_only) const {
         if (s >= hj.end_step || s % hj.dj.stride != 0) continue;
         if (s % hj.dj.pp_freq == 0) m |= 1;
         if (s % hj.dj.pairs_freq == 0) m |= 2;
-        if (hj.dj.sort_rows && s % ((long long)hj.dj.pairs_freq * kSortEvery) == 0) m |= 4;
+        if (hj.dj.sort_rows && s % ((long long)hj.dj.pairs_freq * shape_.sort_every) == 0) m |= 4;
     }
     return m;
 }
@@ -1020,9 +1050,9 @@ int Engine::launch_block(int slot_only) {
                                 sort && jobs_[slot_only].dj.sort_rows ? (jobs_[slot_only].n + kLdsUnit - 1) / kLdsUnit
                                                                       : 0,
                                 d_sunits1_, d_segs1_, jobs_[slot_only].dj.lds_step ? nsunits1_ : 0,
-                                jobs_[slot_only].n, 0, stream_)
+                                jobs_[slot_only].n, 0, stream_, shape_)
                 : launch_verlet(jobs, st, tiles, nt, any_idx32_, any_global_, d_units_, sort ? nunits_ : 0,
-                                d_sunits_, d_segs_, nsunits_, lds_beads_, 0, stream_);
+                                d_sunits_, d_segs_, nsunits_, lds_beads_, 0, stream_, shape_);
         if (int rc = check(e, "verlet")) return rc;
     }
     const bool tea = slot_only >= 0 ? ((jobs_[slot_only].dj.flags & JF_HI) && !(jobs_[slot_only].dj.flags & JF_HI_EXACT))
@@ -1055,9 +1085,9 @@ int Engine::launch_block(int slot_only) {
         if (int rc = check(slot_only >= 0
                                ? launch_step(jobs, st, tiles, nt, !jobs_[slot_only].dj.lds_step, d_sunits1_, d_segs1_,
                                              jobs_[slot_only].dj.lds_step ? nsunits1_ : 0, jobs_[slot_only].n, k,
-                                             max_blocks_, stream_)
+                                             max_blocks_, stream_, shape_)
                                : launch_step(jobs, st, tiles, nt, any_global_, d_sunits_, d_segs_, nsunits_, lds_beads_,
-                                             k, max_blocks_, stream_),
+                                             k, max_blocks_, stream_, shape_),
                            "step"))
             return rc;
         if (prof_end) {
@@ -1096,7 +1126,7 @@ int Engine::tiles_for_slot(int slot, int* nt) {
     const int n = jobs_[slot].n;
     const int need = (n + 63) / 64;
     *nt = need;
-    if (tiles1_n_ == n) return 0;
+    if (tiles1_n_ == n && tiles1_rows_ == shape_.unit_rows) return 0;
     if (need > cap_tiles1_) {
         (void)hipFree(d_tiles1_);
         d_tiles1_ = nullptr;
@@ -1117,20 +1147,24 @@ int Engine::tiles_for_slot(int slot, int* nt) {
         if (int rc = check(hipMalloc(&d_units1_, nu * sizeof(int2)), "unit table")) return rc;
         cap_units1_ = nu;
     }
-    std::vector<int2> u1(nu), su1(nu);
-    std::vector<int4> sg1(nu);
-    for (int u = 0; u < nu; ++u) {
-        u1[u] = make_int2(0, kLdsUnit * u);
+    std::vector<int2> u1(nu);
+    for (int u = 0; u < nu; ++u) u1[u] = make_int2(0, kLdsUnit * u);  // row-sort units
+    // step units of the job alone: unit_rows rows each, as in the batched plan
+    const int U = shape_.unit_rows, ns = (n + U - 1) / U;
+    std::vector<int2> su1(ns);
+    std::vector<int4> sg1(ns);
+    for (int u = 0; u < ns; ++u) {
         su1[u] = make_int2(u, 1);
-        sg1[u] = make_int4(0, kLdsUnit * u, std::min(kLdsUnit, n - kLdsUnit * u), 0);
+        sg1[u] = make_int4(0, U * u, std::min(U, n - U * u), 0);
     }
     if (int rc = check(hipMemcpy(d_units1_, u1.data(), nu * sizeof(int2), hipMemcpyHostToDevice), "unit upload"))
         return rc;
     if (int rc = check(upload_grow(d_sunits1_, cap_sunits1_, su1, stream_), "step units")) return rc;
     if (int rc = check(upload_grow(d_segs1_, cap_segs1_, sg1, stream_), "step segments")) return rc;
     if (int rc = synchronize()) return rc;
-    nsunits1_ = nu;
+    nsunits1_ = ns;
     tiles1_n_ = n;
+    tiles1_rows_ = U;
     return 0;
 }
 
@@ -1148,8 +1182,7 @@ int Engine::stage_possible() {
 int Engine::stage_verlet() {
     if (int rc = sync_tables()) return rc;
     return check(launch_verlet(d_jobs_, d_state_, d_tiles_, ntiles_, any_idx32_, any_global_, d_units_, nunits_,
-                               d_sunits_, d_segs_,
-                               nsunits_, lds_beads_, 0, stream_),
+                               d_sunits_, d_segs_, nsunits_, lds_beads_, 0, stream_, shape_),
                  "verlet");
 }
 int Engine::stage_tea_update() {
@@ -1161,7 +1194,7 @@ int Engine::stage_tea_update() {
 int Engine::stage_step(int k) {
     if (int rc = sync_tables()) return rc;
     if (int rc = check(launch_step(d_jobs_, d_state_, d_tiles_, ntiles_, any_global_, d_sunits_, d_segs_, nsunits_,
-                                   lds_beads_, k, max_blocks_, stream_),
+                                   lds_beads_, k, max_blocks_, stream_, shape_),
                        "step"))
         return rc;
     if (any_tea_)
@@ -1287,6 +1320,25 @@ int Engine::get_coords(int id, float* out4) {
     if (int rc = read_state(slot, &s)) return rc;
     const HostJob& hj = jobs_[slot];
     return check(hipMemcpy(out4, hj.dj.coord[s.step & 1], (size_t)hj.n * 16, hipMemcpyDeviceToHost), "coords read");
+}
+
+int Engine::get_coords_at(int id, long long s, float* out4) {
+    HostJob* hj = find(id);
+    if (!hj) return fail(SOP_E_ARG, "no such job");
+    if (int rc = synchronize()) return rc;
+    return check(hipMemcpy(out4, hj->dj.coord[s & 1], (size_t)hj->n * 16, hipMemcpyDeviceToHost), "coords read");
+}
+
+int Engine::upload_forces(int id, const float* f4) {
+    HostJob* hj = find(id);
+    if (!hj) return fail(SOP_E_ARG, "no such job");
+    if (!hj->dj.fextra) return fail(SOP_E_STATE, "job takes no host forces (host_forces off)");
+    // the last step that read the array cleared it; the copy is ordered
+    // after that step and before the next on the engine's stream
+    if (int rc = check(hipMemcpyAsync(hj->dj.fextra, f4, (size_t)hj->n * 16, hipMemcpyHostToDevice, stream_),
+                       "forces upload"))
+        return rc;
+    return synchronize();
 }
 
 int Engine::set_coords(int id, const float* in4) {
@@ -1416,7 +1468,15 @@ int Engine::compute_forces(int id, float* out4) {
         if (int rc = check(hipMalloc(&d_scratch4_, (size_t)hj.n * 16), "scratch")) return rc;
         scratch4_n_ = hj.n;
     }
-    if (int rc = check(launch_forces(d_jobs_, d_state_, slot, hj.n, d_scratch4_, stream_), "forces")) return rc;
+    // the step kernel the job is stepped by (LDS or global gathers), in its
+    // forces-only instantiation, over the job's own tiles / units
+    int nt = 0;
+    if (int rc = tiles_for_slot(slot, &nt)) return rc;
+    const bool lds = hj.dj.lds_step != 0;
+    if (int rc = check(launch_forces(d_jobs_ + slot, d_state_ + slot, d_tiles1_, nt, !lds, d_sunits1_, d_segs1_,
+                                     lds ? nsunits1_ : 0, hj.n, d_scratch4_, stream_),
+                       "forces"))
+        return rc;
     if (int rc = synchronize()) return rc;
     return check(hipMemcpy(out4, d_scratch4_, (size_t)hj.n * 16, hipMemcpyDeviceToHost), "forces read");
 }
@@ -1465,7 +1525,7 @@ int Engine::rebuild_lists(int id, int mask) {
                                          d_units1_,
                                          jobs_[slot].dj.sort_rows ? (jobs_[slot].n + kLdsUnit - 1) / kLdsUnit : 0,
                                          d_sunits1_, d_segs1_, jobs_[slot].dj.lds_step ? nsunits1_ : 0,
-                                         jobs_[slot].n, 1, stream_),
+                                         jobs_[slot].n, 1, stream_, shape_),
                            "verlet"))
             return rc;
     return synchronize();
@@ -1489,43 +1549,184 @@ int Engine::list_stats(double* avg4, long long* beads) {
     return 0;
 }
 
-// Times the step kernel over real blocks (results do not depend on the
-// launch shape, so tuning runs are ordinary progress) for a ladder of
-// persistent grid sizes and keeps the fastest.
-int Engine::autotune(int* best_out) {
-    if (int rc = sync_tables()) return rc;
-    if (jobs_.empty()) {
-        *best_out = max_blocks_;
-        return 0;
-    }
-    const int full = (ntiles_ + 3) / 4;
-    std::vector<int> cand = {0};
-    for (int b : {256, 512, 1024, 2048, 4096, 8192})
-        if (b < full) cand.push_back(b);
-    const bool saved_prof = profiling_;
-    int best = 0;
-    double best_ms = 1e30;
-    for (int c : cand) {
-        max_blocks_ = c;
-        profiling_ = true;
-        long long nl;
-        double ms;
-        profile_read(&nl, &ms);
-        bool done = true;
-        for (auto& j : jobs_)
-            if (j.step < j.end_step) done = false;
-        if (done) break;
-        if (int rc = run_blocks(2)) return rc;
-        profile_read(&nl, &ms);
-        if (nl > 0 && ms / nl < best_ms) {
-            best_ms = ms / nl;
-            best = c;
-        }
-    }
-    profiling_ = saved_prof;
-    max_blocks_ = best;
+Tuning Engine::tuning() const {
+    return Tuning{max_blocks_, shape_.unit_rows, shared_units_ ? 1 : 0, shape_.pair_batch, shape_.verlet_chunk,
+                  shape_.sort_every};
+}
+
+int Engine::set_shape(const Tuning& t) {
+    const bool ok = t.max_blocks >= 0 && (t.unit_rows == 512 || t.unit_rows == 768 || t.unit_rows == 1024) &&
+                    (t.pair_batch == 4 || t.pair_batch == 6 || t.pair_batch == 8) &&
+                    (t.verlet_chunk == 4 || t.verlet_chunk == 8) && t.sort_every >= 1 && t.sort_every <= 1000;
+    if (!ok) return fail(SOP_E_ARG, "launch shape out of range");
+    max_blocks_ = t.max_blocks;
+    shape_.unit_rows = t.unit_rows;
+    shape_.pair_batch = t.pair_batch;
+    shape_.verlet_chunk = t.verlet_chunk;
+    shape_.sort_every = t.sort_every;
+    shared_units_ = t.shared_units != 0;
+    tables_dirty_ = true;  // the unit plan depends on unit_rows and shared_units
+    tiles1_n_ = -1;
     drop_graph();
-    *best_out = best;
+    return 0;
+}
+
+// Every job's arena blocks and the state table, copied into one device
+// buffer (allocated here; the caller frees it with hipFree).
+int Engine::snapshot(void** out) {
+    *out = nullptr;
+    if (int rc = sync_tables()) return rc;
+    if (int rc = synchronize()) return rc;
+    size_t total = jobs_.size() * sizeof(DevJobState);
+    for (auto& j : jobs_)
+        for (size_t b : j.sizes) total += b;
+    char* buf = nullptr;
+    if (int rc = check(hipMalloc(&buf, total), "tuning snapshot")) return rc;
+    size_t off = 0;
+    int err = 0;
+    for (auto& j : jobs_)
+        for (size_t k = 0; k < j.blocks.size() && !err; ++k) {
+            err = check(hipMemcpyAsync(buf + off, j.blocks[k], j.sizes[k], hipMemcpyDeviceToDevice, stream_), "snapshot");
+            off += j.sizes[k];
+        }
+    if (!err)
+        err = check(hipMemcpyAsync(buf + off, d_state_, jobs_.size() * sizeof(DevJobState), hipMemcpyDeviceToDevice,
+                                   stream_),
+                    "snapshot");
+    if (!err) err = synchronize();
+    if (err) {
+        (void)hipFree(buf);
+        return err;
+    }
+    *out = buf;
+    return 0;
+}
+
+// Puts the snapshot back (arrays, state table, host step mirrors).
+int Engine::restore(void* snap, const std::vector<long long>& steps) {
+    if (int rc = synchronize()) return rc;
+    if (int rc = sync_tables()) return rc;  // slots as at the snapshot: the job set is unchanged
+    const char* buf = static_cast<const char*>(snap);
+    size_t off = 0;
+    for (auto& j : jobs_)
+        for (size_t k = 0; k < j.blocks.size(); ++k) {
+            if (int rc = check(hipMemcpyAsync(j.blocks[k], buf + off, j.sizes[k], hipMemcpyDeviceToDevice, stream_),
+                               "restore"))
+                return rc;
+            off += j.sizes[k];
+        }
+    if (int rc = check(hipMemcpyAsync(d_state_, buf + off, jobs_.size() * sizeof(DevJobState), hipMemcpyDeviceToDevice,
+                                      stream_),
+                       "restore"))
+        return rc;
+    for (size_t q = 0; q < jobs_.size(); ++q) jobs_[q].step = steps[q];
+    return synchronize();
+}
+
+// GPU time of nblocks eager blocks, from events on the engine's stream.
+int Engine::time_blocks(int nblocks, double* ms) {
+    if (int rc = sync_tables()) return rc;
+    hipEvent_t a = nullptr, b = nullptr;
+    if (int rc = check(hipEventCreate(&a), "event")) return rc;
+    if (int rc = check(hipEventCreate(&b), "event")) {
+        (void)hipEventDestroy(a);
+        return rc;
+    }
+    int rc = check(hipEventRecord(a, stream_), "event");
+    for (int k = 0; k < nblocks && !rc; ++k) {
+        rc = launch_block(-1);
+        for (auto& j : jobs_) j.step = std::min(j.step + L_, j.end_step);
+    }
+    if (!rc) rc = check(hipEventRecord(b, stream_), "event");
+    if (!rc) rc = check(hipEventSynchronize(b), "event");
+    float f = 0.0f;
+    if (!rc) rc = check(hipEventElapsedTime(&f, a, b), "event");
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    *ms = f;
+    return rc;
+}
+
+// Coordinate descent over the launch shapes (each dimension in turn, the
+// others held at their best so far), every candidate timed on the same
+// window of real blocks from the same snapshot: whole periods of the jobs'
+// list cadences (possible pairs, Verlet rebuilds, the longest row-sort
+// cadence tried), so each candidate pays its share of every kernel.
+int Engine::autotune(Tuning* best_out, double* ms_default, double* ms_best, int* trials_out) {
+    if (hipSetDevice(device_) != hipSuccess) return fail(SOP_E_HIP, "hipSetDevice failed");
+    if (int rc = sync_tables()) return rc;
+    const Tuning start = tuning();
+    if (best_out) *best_out = start;
+    if (trials_out) *trials_out = 0;
+    if (jobs_.empty()) return 0;
+    bool lds = false;
+    long long window = L_;
+    for (auto& j : jobs_) {
+        if (j.step >= j.end_step) continue;
+        if (j.dj.lds_step) lds = true;
+        window = std::max<long long>(window, j.dj.pp_freq);
+        if (j.dj.sort_rows) window = std::max<long long>(window, (long long)j.dj.pairs_freq * 40);
+    }
+    window = std::min<long long>((window + L_ - 1) / L_ * L_, 4000);
+    const int nblocks = (int)std::max<long long>(1, window / L_);
+    std::vector<long long> steps;
+    for (auto& j : jobs_) steps.push_back(j.step);
+    const bool saved_graph = use_graph_, saved_prof = profiling_;
+    profiling_ = false;
+    void* snap = nullptr;
+    if (int rc = snapshot(&snap)) return rc;
+    int trials = 0;
+    auto trial = [&](const Tuning& t, double* per_step) -> int {
+        if (int rc = set_shape(t)) return rc;
+        if (int rc = restore(snap, steps)) return rc;
+        double ms = 0.0;
+        if (int rc = time_blocks(nblocks, &ms)) return rc;
+        *per_step = ms / ((double)nblocks * L_);
+        ++trials;
+        return 0;
+    };
+    Tuning best = start;
+    double best_ms = 0.0;
+    int rc = trial(best, &best_ms);
+    const double def_ms = best_ms;
+    auto consider = [&](Tuning t) {
+        if (rc) return;
+        double ms = 0.0;
+        rc = trial(t, &ms);
+        if (!rc && ms < best_ms * 0.995) {  // a real gain, beyond run-to-run noise
+            best = t;
+            best_ms = ms;
+        }
+    };
+    if (lds) {
+        for (int v : {768, 512})
+            if (v != best.unit_rows) { Tuning t = best; t.unit_rows = v; consider(t); }
+        { Tuning t = best; t.shared_units = !best.shared_units; consider(t); }
+        for (int v : {4, 8})
+            if (v != best.pair_batch) { Tuning t = best; t.pair_batch = v; consider(t); }
+        { Tuning t = best; t.verlet_chunk = best.verlet_chunk == 8 ? 4 : 8; consider(t); }
+        for (int v : {10, 40})
+            if (v != best.sort_every) { Tuning t = best; t.sort_every = v; consider(t); }
+    }
+    if (any_global_) {
+        const int full = (ntiles_ + 3) / 4;
+        for (int b : {256, 1024, 4096})
+            if (b < full && b != best.max_blocks) { Tuning t = best; t.max_blocks = b; consider(t); }
+        if (best.max_blocks != 0) { Tuning t = best; t.max_blocks = 0; consider(t); }
+    }
+    // the jobs as they were, under the chosen shape
+    const int rc2 = set_shape(rc ? start : best);
+    const int rc3 = restore(snap, steps);
+    (void)hipFree(snap);
+    use_graph_ = saved_graph;
+    profiling_ = saved_prof;
+    if (rc) return rc;
+    if (rc2) return rc2;
+    if (rc3) return rc3;
+    if (best_out) *best_out = best;
+    if (ms_default) *ms_default = def_ms;
+    if (ms_best) *ms_best = best_ms;
+    if (trials_out) *trials_out = trials;
     return 0;
 }
 

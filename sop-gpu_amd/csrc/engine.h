@@ -20,6 +20,7 @@
 #include "hi.h"
 #include "hostutil.h"
 #include "jobdesc.h"
+#include "kernels.h"
 
 namespace sop {
 
@@ -35,13 +36,18 @@ class Arena {
     ~Arena();
     void* alloc(size_t bytes);  // nullptr when the budget is exhausted
     void release(void* p);
+    // Returns the chunks that hold no live block to the device; the bytes
+    // they reserved count against the budget no longer.  Returns the bytes
+    // freed.
+    size_t trim();
     size_t reserved() const { return reserved_; }
     size_t in_use() const { return in_use_; }
 
    private:
     struct Chunk {
-        char* base;
+        char* base;   // null once trimmed (the slot is kept: live_ holds chunk indices)
         size_t size;
+        size_t live = 0;  // blocks allocated from the chunk
         std::map<size_t, size_t> free;  // offset -> length
     };
     std::vector<Chunk> chunks_;
@@ -58,8 +64,19 @@ struct HostJob {
     long long end_step = 0;
     std::vector<void*> blocks;
     unsigned long long bytes = 0;
+    std::vector<size_t> sizes;  // bytes of each block (snapshots for the autotuner)
     HiExactWork hw{};      // exact-HI factorisation workspace (hw.M null otherwise)
     int tile0 = 0, ntile = 0;  // the job's range in the tile table
+};
+
+// Launch shapes the autotuner searches; none of them changes a result.
+struct Tuning {
+    int max_blocks;    // persistent grid of the global-gather step kernel (0: one workgroup per 4 tiles)
+    int unit_rows;     // Verlet rows per LDS step unit (threads of its workgroup)
+    int shared_units;  // rows left over by LDS jobs packed into shared units (else one job per unit)
+    int pair_batch;    // Verlet entries fetched ahead per lane in the LDS step
+    int verlet_chunk;  // 64-entry chunks in flight per row in the LDS Verlet build
+    int sort_every;    // Verlet rows re-sorted at every k-th Verlet rebuild
 };
 
 class Engine {
@@ -69,6 +86,8 @@ class Engine {
 
     int add_job(const sop_params& p, const sop_system& s, long long numsteps, int* id);
     int remove_job(int id);
+    // Frees arena chunks that hold no job's arrays (see Arena::trim).
+    size_t trim_arena() { return arena_.trim(); }
     static int job_bytes(const sop_params& p, const sop_system& s, unsigned long long* bytes);
 
     int run_blocks(int nblocks);
@@ -83,6 +102,10 @@ class Engine {
     // read-back / hooks
     int get_coords(int id, float* out4);
     int set_coords(int id, const float* in4);
+    // coordinates of local step s (inside a block: the buffer step s reads)
+    int get_coords_at(int id, long long s, float* out4);
+    // forces added to every bead in the job's next step (host potentials)
+    int upload_forces(int id, const float* f4);
     int get_array(int id, int what, void* out);
     int list_cap(int id, int which, int* cap);
     int get_list(int id, int which, int* counts, int* entries);
@@ -114,7 +137,12 @@ class Engine {
     void set_exact_mode(int mode) { exact_mode_ = mode; }
     int get_hi(int id, double* eps, float* beta);
     int get_hi_factor(int id, float* L);
-    int autotune(int* best);
+    // Searches the launch shapes on the admitted jobs: each candidate runs
+    // real blocks (whole list periods) from a snapshot of every job's device
+    // arrays, which is restored afterwards, so the jobs do not advance.
+    int autotune(Tuning* best, double* ms_default, double* ms_best, int* trials);
+    Tuning tuning() const;
+    int set_shape(const Tuning& t);
     void profile(bool on);
     void profile_read(long long* launches, double* ms);
 
@@ -172,7 +200,14 @@ class Engine {
     int cap_sunits_ = 0, cap_segs_ = 0, nsunits_ = 0;
     int2* d_sunits1_ = nullptr;
     int4* d_segs1_ = nullptr;
-    int cap_sunits1_ = 0, cap_segs1_ = 0, nsunits1_ = 0;  // bitmap words of the largest job with a cell grid (0: none or disabled)
+    int cap_sunits1_ = 0, cap_segs1_ = 0, nsunits1_ = 0;
+    int tiles1_rows_ = 0;  // unit rows the single-job plan was built for
+    StepShape shape_;
+    bool shared_units_ = true;
+    // autotuner snapshot of the jobs' device arrays and state table
+    int snapshot(void** buf);
+    int restore(void* buf, const std::vector<long long>& steps);
+    int time_blocks(int nblocks, double* ms);
     bool uses_cells(size_t n) const;
     int cell_words_of(const DevJob& J) const;
     int exact_mode_ = SOP_CHOL_BF16X3;

@@ -1257,8 +1257,11 @@ hipError_t launch_chol_step(const DevJob* jobs, const DevJobState* st, const int
     hipLaunchKernelGGL(k_chol_D, dim3(ntiles * kHiSplit), dim3(1024), 0, stream, jobs, st, tiles, ntiles, kstep);
     if (w) {
         hipLaunchKernelGGL(k_chol_build, dim3(ntiles, ntiles), dim3(256), 0, stream, jobs, st, tiles, kstep);
-        (void)launch_chol_factor(w->M, w->ld, w->mp, w->mode, w->linv, w->phi, w->plo, nullptr, stream, w->aux,
-                                 w->ev_f, w->ev_t2);
+        // the factorisation's error covers the event record/wait pairs that
+        // order its two streams: report it before the noise reads the factor
+        const hipError_t e = launch_chol_factor(w->M, w->ld, w->mp, w->mode, w->linv, w->phi, w->plo, nullptr,
+                                                stream, w->aux, w->ev_f, w->ev_t2);
+        if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(k_chol_L, dim3(ntiles * 16), dim3(256), 0, stream, jobs, st, tiles, ntiles, kstep);
     return hipGetLastError();

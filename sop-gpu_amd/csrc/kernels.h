@@ -6,6 +6,18 @@
 
 namespace sop {
 
+// Launch shapes of the LDS step and list kernels that the autotuner picks
+// (none of them changes any result): threads of a step unit (its Verlet
+// rows), Verlet entries fetched ahead per lane in the step, 64-entry chunks
+// in flight per row in the Verlet build, and the row-sort cadence (every
+// k-th Verlet rebuild).
+struct StepShape {
+    int unit_rows = kLdsUnit;
+    int pair_batch = 6;
+    int verlet_chunk = 8;
+    int sort_every = kSortEvery;
+};
+
 // One Langevin step (local step = state.step + kstep) of every job that owns
 // a tile.  max_blocks caps the persistent grid (0 = one block per 4 tiles).
 // Jobs flagged lds_step are stepped by the LDS kernel over `units` (first
@@ -14,7 +26,7 @@ namespace sop {
 // tile kernel (launched when any_global is set).
 hipError_t launch_step(const DevJob* jobs, const DevJobState* st, const int2* tiles, int ntiles, int any_global,
                        const int2* units, const int4* segs, int nunits, int lds_beads, int kstep, int max_blocks,
-                       hipStream_t stream);
+                       hipStream_t stream, const StepShape& shape);
 // List rebuilds for jobs whose cadence is due (force != 0: rebuild all).
 // cell_words: bitmap words (ceil(n/64)) of the largest job with a cell grid,
 // 0 to build every job's possible list by the all-pairs scan.
@@ -34,13 +46,17 @@ constexpr int kMaxCellWords = 8192;
 // jobs by the tile kernel); any_global: some job is outside that plan.
 hipError_t launch_verlet(const DevJob* jobs, DevJobState* st, const int2* tiles, int ntiles, int any_idx32,
                          int any_global, const int2* units, int nunits, const int2* sunits, const int4* segs,
-                         int nsunits, int lds_beads, int force, hipStream_t stream);
+                         int nsunits, int lds_beads, int force, hipStream_t stream, const StepShape& shape);
 // Per-job updaters at a block start, and the step counter advance.
 hipError_t launch_job_update(const DevJob* jobs, DevJobState* st, int njobs, hipStream_t stream, int mask = 7);
 hipError_t launch_job_advance(const DevJob* jobs, DevJobState* st, int njobs, int nsteps, hipStream_t stream);
 // Output / parity hooks.
 hipError_t launch_energies(const DevJob* jobs, const DevJobState* st, int job, int n, hipStream_t stream);
-hipError_t launch_forces(const DevJob* jobs, const DevJobState* st, int job, int n, float4* out, hipStream_t stream);
+// Forces only (the step kernels' force terms, no state change) of the jobs
+// of the given tiles / LDS plan, into out[bead] (one job per launch).
+hipError_t launch_forces(const DevJob* jobs, const DevJobState* st, const int2* tiles, int ntiles, int any_global,
+                         const int2* units, const int4* segs, int nunits, int lds_beads, float4* out,
+                         hipStream_t stream);
 hipError_t launch_rng_probe(uint4* state, int n, int count, int gaussian, float* out, hipStream_t stream);
 // Counts (into *bad, device) the b bit patterns in [lo, hi) times numerators
 // a[0..na) (device) whose short division differs from the IEEE quotient.

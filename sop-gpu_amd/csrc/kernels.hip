@@ -206,7 +206,10 @@ __device__ __forceinline__ void bonded_terms(const DevJob& J, int nc, int nn, in
 // (global memory, or the job's coordinates staged in LDS), indentation
 // walls, then noise, the fixed/pulled rules and the Euler update, written to
 // `cout` in global memory.  Hydrodynamic jobs stop after the forces.
-template <int PB>
+// FO (forces only): the same force terms, written to `cout` (indexed by
+// bead) before the noise, with no state changed -- the parity hook of the
+// force gather runs the kernel the step runs.
+template <int PB, bool FO = false>
 __device__ __forceinline__ void step_bead(const DevJob& J, const DevJobState& S, long long s, int i, int r, int n,
                                           const float4* __restrict__ cin, float4* __restrict__ cout) {
     const float4 c = cin[i];
@@ -214,7 +217,7 @@ __device__ __forceinline__ void step_bead(const DevJob& J, const DevJobState& S,
     if (J.flags & JF_EXTRA_FORCE) {
         const float4 e = J.fextra[i];
         fx = e.x; fy = e.y; fz = e.z;
-        J.fextra[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if constexpr (!FO) J.fextra[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
     // FENE and native LJ 12-6: for sorted rows from the row-ordered copies
     // (a wavefront's 64 rows read 64 consecutive words per slot), else in
@@ -253,7 +256,7 @@ __device__ __forceinline__ void step_bead(const DevJob& J, const DevJobState& S,
         fin.x += dfx;
         fin.y += dfy;
         fin.z += dfz;
-        J.tipF[i] = fin;
+        if constexpr (!FO) J.tipF[i] = fin;
         x = c.x - S.micaR[0];
         y = c.y - S.micaR[1];
         z = c.z - S.micaR[2];
@@ -266,6 +269,10 @@ __device__ __forceinline__ void step_bead(const DevJob& J, const DevJobState& S,
         fx += J.micaN[0] * sw + dfx;
         fy += J.micaN[1] * sw + dfy;
         fz += J.micaN[2] * sw + dfz;
+    }
+    if constexpr (FO) {
+        cout[i] = make_float4(fx, fy, fz, 0.0f);
+        return;
     }
     // hydrodynamic jobs: noise and forces for the HI integrator
     // (integrateTea_prepare, bdhitea_kernel.cu:33-65); the update itself
@@ -305,8 +312,12 @@ __device__ __forceinline__ void step_bead(const DevJob& J, const DevJobState& S,
     cout[i] = cn;
 }
 
+// FO: forces only, into fout (bead-indexed), for every job of the tiles
+// whatever its step (nothing is advanced).
+template <bool FO>
 __global__ __launch_bounds__(256) void k_step(const DevJob* __restrict__ jobs, const DevJobState* __restrict__ st,
-                                              const int2* __restrict__ tiles, int ntiles, int kstep) {
+                                              const int2* __restrict__ tiles, int ntiles, int kstep,
+                                              float4* __restrict__ fout) {
     const int lane = threadIdx.x & 63;
     const TileRange tr = xcd_tiles(ntiles);
     for (int t = tr.begin; t < tr.end; t += tr.step) {
@@ -314,11 +325,11 @@ __global__ __launch_bounds__(256) void k_step(const DevJob* __restrict__ jobs, c
         const DevJob& J = jobs[td.x];
         if (J.lds_step) continue;  // stepped by k_step_lds
         const long long s = st[td.x].step + kstep;
-        if (s >= J.end_step) continue;
+        if (!FO && s >= J.end_step) continue;
         const int r = td.y + lane;  // Verlet row; the lane steps bead perm[r]
         const int n = J.n;
         if (r >= n) continue;
-        step_bead<8>(J, st[td.x], s, J.perm[r], r, n, J.coord[s & 1], J.coord[(s + 1) & 1]);
+        step_bead<8, FO>(J, st[td.x], s, J.perm[r], r, n, J.coord[s & 1], FO ? fout : J.coord[(s + 1) & 1]);
     }
 }
 
@@ -330,9 +341,15 @@ __global__ __launch_bounds__(256) void k_step(const DevJob* __restrict__ jobs, c
 // coordinates in LDS (coalesced), so every neighbour gather of the step
 // reads LDS.  Segments start at multiples of 64 threads: no wavefront spans
 // two jobs.
-__global__ __launch_bounds__(1024, 8) void k_step_lds(const DevJob* __restrict__ jobs, const DevJobState* __restrict__ st,
-                                                   const int2* __restrict__ units, const int4* __restrict__ segs,
-                                                   int nunits, int kstep) {
+// PB: Verlet entries fetched ahead per lane (pair_loop).  With PB <= 6 the
+// kernel is built for two 1024-thread workgroups per CU (64 VGPRs), with
+// PB = 8 for one (128 VGPRs); two 80 KB LDS stagings fit a CU either way.
+template <bool FO, int PB>
+__global__ __launch_bounds__(1024, PB <= 6 ? 8 : 4) void k_step_lds(const DevJob* __restrict__ jobs,
+                                                                 const DevJobState* __restrict__ st,
+                                                                 const int2* __restrict__ units,
+                                                                 const int4* __restrict__ segs, int nunits, int kstep,
+                                                                 float4* __restrict__ fout) {
     extern __shared__ float4 lds_coord[];
     for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
         const int2 ud = units[u];  // first segment, segments
@@ -341,7 +358,7 @@ __global__ __launch_bounds__(1024, 8) void k_step_lds(const DevJob* __restrict__
             const int4 sg = segs[ud.x + q];
             const DevJob& J = jobs[sg.x];
             const long long s = st[sg.x].step + kstep;
-            if (s >= J.end_step) continue;
+            if (!FO && s >= J.end_step) continue;
             const float4* __restrict__ g = J.coord[s & 1];
             for (int k = threadIdx.x; k < J.n; k += blockDim.x) lds_coord[sg.w + k] = g[k];
         }
@@ -354,8 +371,9 @@ __global__ __launch_bounds__(1024, 8) void k_step_lds(const DevJob* __restrict__
                 const DevJob& J = jobs[sg.x];
                 const long long s = st[sg.x].step + kstep;
                 const int r = sg.y + t;  // Verlet row; the thread steps bead perm[r]
-                if (t < sg.z && s < J.end_step)
-                    step_bead<6>(J, st[sg.x], s, J.perm[r], r, J.n, lds_coord + sg.w, J.coord[(s + 1) & 1]);
+                if (t < sg.z && (FO || s < J.end_step))
+                    step_bead<PB, FO>(J, st[sg.x], s, J.perm[r], r, J.n, lds_coord + sg.w,
+                                      FO ? fout : J.coord[(s + 1) & 1]);
                 break;
             }
             t -= span;
@@ -878,11 +896,11 @@ struct VerletLds<int> {
 };
 
 // w: this wavefront's index among the WAVES wavefronts that build the tile.
-template <typename IdxT, int WAVES>
+// CH: chunks of 64 possible-list entries whose loads are in flight together.
+template <typename IdxT, int WAVES, int CH = 8>
 __device__ __forceinline__ void verlet_tile(const DevJob& J, DevJobState& S, const float4* __restrict__ cur, int i0,
                                             IdxT* __restrict__ rows, int* __restrict__ cnt_s, int w) {
     constexpr int E = VerletLds<IdxT>::entries, RS = VerletLds<IdxT>::stride;
-    constexpr int CH = 8;  // chunks of 64 entries whose loads are in flight together
     constexpr int RPW = 64 / WAVES;  // rows per wave
     const int lane = threadIdx.x & 63;
     const int n = J.n;
@@ -1009,7 +1027,9 @@ __device__ __forceinline__ void verlet_store(const DevJob& J, int i0, const IdxT
     // the four waves' maxima differ only in which beads they saw: all waves
     // see all 64 beads here (b = lane), so maxc is the tile's
     const auto pj = gmem(static_cast<IdxT*>(J.pair_j));
-    const int lim = min(c, E);
+    // a row that outgrew its capacity (flag bit 2 is set) stores only its
+    // first cap entries: pair_j holds pair_cap slots
+    const int lim = min(c, min(E, cap));
     for (int k = kq; k < maxc; k += WAVES)
         if (k < lim) pj[(size_t)k * J.pair_ld + i0 + b] = rows[b * RS + k];
 }
@@ -1025,14 +1045,15 @@ __device__ __forceinline__ void verlet_store(const DevJob& J, int i0, const IdxT
 // reads them in row order), so it runs at every kSortEvery-th Verlet
 // rebuild only: list lengths drift little in between.
 __global__ __launch_bounds__(1024) void k_sort_rows(const DevJob* __restrict__ jobs, const DevJobState* __restrict__ st,
-                                                    const int2* __restrict__ units, int nunits, int force) {
+                                                    const int2* __restrict__ units, int nunits, int sort_every,
+                                                    int force) {
     __shared__ unsigned long long key[kLdsUnit];
     const int t = threadIdx.x;
     for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
         const int2 ud = units[u];
         const DevJob& J = jobs[ud.x];
         if (!J.sort_rows) continue;
-        if (!force && !due(J, st[ud.x].step, J.pairs_freq * kSortEvery)) continue;  // a Verlet rebuild step
+        if (!force && !due(J, st[ud.x].step, J.pairs_freq * sort_every)) continue;  // a Verlet rebuild step
         const int m = min(kLdsUnit, J.n - ud.y), r = ud.y + t;
         __syncthreads();  // the previous unit's keys are read
         key[t] = t < m ? ((unsigned long long)(unsigned)(J.pair_cap - J.pair_cnt[r]) << 32) | (unsigned)J.perm[r]
@@ -1107,6 +1128,7 @@ __global__ __launch_bounds__(256) void k_verlet(const DevJob* __restrict__ jobs,
 // rows each) and writes each tile slot-major as k_verlet does.  Same
 // lists, same order.
 constexpr int kVerletRowsBytes = 64 * VerletLds<unsigned short>::stride * 2;
+template <int CH>
 __global__ __launch_bounds__(1024) void k_verlet_lds(const DevJob* __restrict__ jobs, DevJobState* __restrict__ st,
                                                      const int2* __restrict__ units, const int4* __restrict__ segs,
                                                      int nunits, int force) {
@@ -1135,7 +1157,7 @@ __global__ __launch_bounds__(1024) void k_verlet_lds(const DevJob* __restrict__ 
             for (int t0 = 0; t0 < sg.z; t0 += 128) {
                 const int i0 = sg.y + t0 + 64 * grp;
                 const bool mine = t0 + 64 * grp < sg.z;
-                if (mine) verlet_tile<unsigned short, 8>(J, S, coords + sg.w, i0, rows, cnt_s[grp], gt >> 6);
+                if (mine) verlet_tile<unsigned short, 8, CH>(J, S, coords + sg.w, i0, rows, cnt_s[grp], gt >> 6);
                 __syncthreads();
                 if (mine) verlet_store<unsigned short, 8>(J, i0, rows, cnt_s[grp], gt);
                 __syncthreads();
@@ -1290,84 +1312,6 @@ __global__ __launch_bounds__(256) void k_energies(const DevJob* __restrict__ job
     J.energies[2 * n + i] = pot;
 }
 
-// Per-bead forces only (no integration, no state change): the parity hook
-// for the force gather.  Same arithmetic as k_step.
-__global__ __launch_bounds__(256) void k_forces(const DevJob* __restrict__ jobs, const DevJobState* __restrict__ st,
-                                                int job, float4* __restrict__ out) {
-    const DevJob& J = jobs[job];
-    const int n = J.n;
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;  // Verlet row of bead i
-    if (r >= n) return;
-    const int i = J.perm[r];
-    const long long s = st[job].step;
-    const float4* __restrict__ cin = J.coord[s & 1];
-    const float4 c = cin[i];
-    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    for (int k = 0; k < J.cov_cnt[i]; ++k) {
-        const int j = J.cov_j[(size_t)k * n + i];
-        const float r0 = J.cov_r0[(size_t)k * n + i];
-        const float4 o = cin[j];
-        const float x = o.x - c.x, y = o.y - c.y, z = o.z - c.z;
-        float w = sqrtf(x * x + y * y + z * z);
-        w = w - r0;
-        w = J.ks * w / ((w + r0) * (1.0f - w * w / J.R2));
-        fx += w * x;
-        fy += w * y;
-        fz += w * z;
-    }
-    for (int k = 0; k < J.nat_cnt[i]; ++k) {
-        const int j = J.nat_j[(size_t)k * n + i];
-        const float2 par = J.nat_par[(size_t)k * n + i];
-        const float4 o = cin[j];
-        const float x = o.x - c.x, y = o.y - c.y, z = o.z - c.z;
-        float q = par.x / (x * x + y * y + z * z);
-        const float p6 = q * q * q;
-        q = q * (p6 * p6 - p6) * par.y;
-        fx += x * q;
-        fy += y * q;
-        fz += z * q;
-    }
-    for (int k = 0; k < J.pair_cnt[r]; ++k) {
-        const int j = list_at(J.pair_j, J.idx16, (size_t)k * J.pair_ld + r);
-        const float4 o = cin[j];
-        const float x = o.x - c.x, y = o.y - c.y, z = o.z - c.z;
-        const float d2 = x * x + y * y + z * z;
-        if (d2 < J.rc2) {
-            float q = J.a2 / d2;
-            q = q * q;
-            q = J.m6 * q * q;
-            fx += x * q;
-            fy += y * q;
-            fz += z * q;
-        }
-    }
-    if (J.flags & JF_INDENTATION) {
-        const DevJobState& S = st[job];
-        float x = c.x - S.tip[0], y = c.y - S.tip[1], z = c.z - S.tip[2];
-        const float rr = sqrtf(x * x + y * y + z * z);
-        float r2 = rr - J.tipR;
-        r2 = 1.0f / r2;
-        r2 = r2 * r2;
-        float r6 = r2 * r2 * r2;
-        float dw = J.tipAp * r6 + J.tipBp;
-        dw = dw * r6 / (rr * (rr - J.tipR));
-        const float dfx = x * dw, dfy = y * dw, dfz = z * dw;
-        x = c.x - S.micaR[0];
-        y = c.y - S.micaR[1];
-        z = c.z - S.micaR[2];
-        float h = x * J.micaN[0] + y * J.micaN[1] + z * J.micaN[2];
-        h = 1.0f / h;
-        h = h * h;
-        r6 = h * h * h;
-        float sw = J.surfAp * r6 + J.surfBp;
-        sw = sw * r6 * h;
-        fx += J.micaN[0] * sw + dfx;
-        fy += J.micaN[1] * sw + dfy;
-        fz += J.micaN[2] * sw + dfz;
-    }
-    out[i] = make_float4(fx, fy, fz, 0.0f);
-}
-
 // Self-check of div_rn_fast against the IEEE division: every float bit
 // pattern b in [lo, hi) (inside the fast range) with each numerator a[k];
 // counts the results that differ.
@@ -1431,14 +1375,34 @@ static int grid_for(int ntiles, int max_blocks) {
 
 hipError_t launch_step(const DevJob* jobs, const DevJobState* st, const int2* tiles, int ntiles, int any_global,
                        const int2* units, const int4* segs, int nunits, int lds_beads, int kstep, int max_blocks,
-                       hipStream_t stream) {
+                       hipStream_t stream, const StepShape& shape) {
     if (ntiles <= 0) return hipSuccess;
     if (any_global)
-        hipLaunchKernelGGL(k_step, dim3(grid_for(ntiles, max_blocks)), dim3(256), 0, stream, jobs, st, tiles, ntiles,
-                           kstep);
+        hipLaunchKernelGGL(k_step<false>, dim3(grid_for(ntiles, max_blocks)), dim3(256), 0, stream, jobs, st, tiles,
+                           ntiles, kstep, nullptr);
+    if (nunits > 0) {
+        const dim3 g(nunits), b(shape.unit_rows);
+        const size_t lds = (size_t)lds_beads * sizeof(float4);
+        if (shape.pair_batch == 4)
+            hipLaunchKernelGGL((k_step_lds<false, 4>), g, b, lds, stream, jobs, st, units, segs, nunits, kstep, nullptr);
+        else if (shape.pair_batch == 8)
+            hipLaunchKernelGGL((k_step_lds<false, 8>), g, b, lds, stream, jobs, st, units, segs, nunits, kstep, nullptr);
+        else
+            hipLaunchKernelGGL((k_step_lds<false, 6>), g, b, lds, stream, jobs, st, units, segs, nunits, kstep, nullptr);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_forces(const DevJob* jobs, const DevJobState* st, const int2* tiles, int ntiles, int any_global,
+                         const int2* units, const int4* segs, int nunits, int lds_beads, float4* out,
+                         hipStream_t stream) {
+    if (ntiles <= 0) return hipSuccess;
+    if (any_global)
+        hipLaunchKernelGGL(k_step<true>, dim3(grid_for(ntiles, 0)), dim3(256), 0, stream, jobs, st, tiles, ntiles, 0,
+                           out);
     if (nunits > 0)
-        hipLaunchKernelGGL(k_step_lds, dim3(nunits), dim3(kLdsUnit), (size_t)lds_beads * sizeof(float4), stream, jobs,
-                           st, units, segs, nunits, kstep);
+        hipLaunchKernelGGL((k_step_lds<true, 6>), dim3(nunits), dim3(kLdsUnit), (size_t)lds_beads * sizeof(float4),
+                           stream, jobs, st, units, segs, nunits, 0, out);
     return hipGetLastError();
 }
 
@@ -1470,9 +1434,11 @@ hipError_t launch_possible(const DevJob* jobs, DevJobState* st, const int2* tile
 
 hipError_t launch_verlet(const DevJob* jobs, DevJobState* st, const int2* tiles, int ntiles, int any_idx32,
                          int any_global, const int2* units, int nunits, const int2* sunits, const int4* segs,
-                         int nsunits, int lds_beads, int force, hipStream_t stream) {
+                         int nsunits, int lds_beads, int force, hipStream_t stream, const StepShape& shape) {
     if (ntiles <= 0) return hipSuccess;
-    if (nunits > 0) hipLaunchKernelGGL(k_sort_rows, dim3(nunits), dim3(kLdsUnit), 0, stream, jobs, st, units, nunits, force);
+    if (nunits > 0)
+        hipLaunchKernelGGL(k_sort_rows, dim3(nunits), dim3(kLdsUnit), 0, stream, jobs, st, units, nunits,
+                           shape.sort_every, force);
     int blocks = ntiles;
     if (blocks >= 8) blocks = (blocks + 7) / 8 * 8;
     const size_t lds = any_idx32 ? 64 * VerletLds<int>::stride * sizeof(int)
@@ -1480,9 +1446,15 @@ hipError_t launch_verlet(const DevJob* jobs, DevJobState* st, const int2* tiles,
     if (any_global || nsunits <= 0)
         hipLaunchKernelGGL(k_verlet, dim3(blocks), dim3(256), lds, stream, jobs, st, tiles, ntiles, force,
                            nsunits > 0);
-    if (nsunits > 0)
-        hipLaunchKernelGGL(k_verlet_lds, dim3(nsunits), dim3(1024), 2 * (size_t)kVerletRowsBytes + (size_t)lds_beads * 16,
-                           stream, jobs, st, sunits, segs, nsunits, force);
+    if (nsunits > 0) {
+        const size_t lds = 2 * (size_t)kVerletRowsBytes + (size_t)lds_beads * 16;
+        if (shape.verlet_chunk == 4)
+            hipLaunchKernelGGL(k_verlet_lds<4>, dim3(nsunits), dim3(1024), lds, stream, jobs, st, sunits, segs, nsunits,
+                               force);
+        else
+            hipLaunchKernelGGL(k_verlet_lds<8>, dim3(nsunits), dim3(1024), lds, stream, jobs, st, sunits, segs, nsunits,
+                               force);
+    }
     return hipGetLastError();
 }
 
@@ -1503,16 +1475,11 @@ hipError_t launch_energies(const DevJob* jobs, const DevJobState* st, int job, i
     return hipGetLastError();
 }
 
-hipError_t launch_forces(const DevJob* jobs, const DevJobState* st, int job, int n, float4* out, hipStream_t stream) {
-    hipLaunchKernelGGL(k_forces, dim3((n + 255) / 256), dim3(256), 0, stream, jobs, st, job, out);
-    return hipGetLastError();
-}
-
 hipError_t launch_rng_probe(uint4* state, int n, int count, int gaussian, float* out, hipStream_t stream) {
     hipLaunchKernelGGL(k_rng_probe, dim3((n + 255) / 256), dim3(256), 0, stream, state, n, count, gaussian, out);
     return hipGetLastError();
 }
 
-const void* step_kernel_symbol() { return reinterpret_cast<const void*>(&k_step); }
+const void* step_kernel_symbol() { return reinterpret_cast<const void*>(&k_step<false>); }
 
 }  // namespace sop

@@ -159,14 +159,20 @@ int sop_server_pump(sop_server* s, int nblocks, int* running, int* queued) {
         const sop_system v = r.view();
         int id = 0;
         int rc = E.add_job(r.p, v, r.numsteps, &id);
+        if (rc == SOP_E_MEM) {
+            // the budget has room by the byte count, but the arena's chunks
+            // do not: return the empty ones to the device and try once more
+            if (E.trim_arena() > 0) rc = E.add_job(r.p, v, r.numsteps, &id);
+        }
         s->queue.pop_front();
-        if (rc == SOP_E_MEM) {  // arena fragmentation: retry after retirements
+        if (rc == SOP_E_MEM && !s->running.empty()) {  // retry after retirements free more chunks
             s->queue.push_front(r.ticket);
             break;
         }
-        if (rc) {
+        if (rc) {  // a bad request, or no room with nothing else admitted: it fails
             r.state = -1;
             r.error = sop_last_error();
+            r.t_done = s->now();
             continue;
         }
         r.job = id;
@@ -254,6 +260,25 @@ int sop_server_stats(sop_server* s, double* bead_steps, double* busy_seconds, in
     if (admitted) *admitted = s->admitted;
     if (finished) *finished = s->finished;
     if (peak_bytes) *peak_bytes = s->peak;
+    return 0;
+}
+
+int sop_server_set_shape(sop_server* s, const sop_tuning* t) {
+    if (!s || !t) return sop::fail(SOP_E_ARG, "null argument");
+    return s->eng->set_shape(sop::Tuning{t->max_blocks, t->unit_rows, t->shared_units, t->pair_batch,
+                                         t->verlet_chunk, t->sort_every});
+}
+
+int sop_server_autotune(sop_server* s, sop_tuning* best, double* ms_start, double* ms_best, int* trials) {
+    if (!s) return sop::fail(SOP_E_ARG, "null server");
+    sop::Tuning t{};
+    double a = 0.0, b = 0.0;
+    int n = 0;
+    if (int rc = s->eng->autotune(&t, &a, &b, &n)) return rc;
+    if (best) *best = sop_tuning{t.max_blocks, t.unit_rows, t.shared_units, t.pair_batch, t.verlet_chunk, t.sort_every};
+    if (ms_start) *ms_start = a;
+    if (ms_best) *ms_best = b;
+    if (trials) *trials = n;
     return 0;
 }
 

@@ -10,6 +10,7 @@
 // launches the step.  Trajectories of the many-runs mode are separate jobs
 // of one engine, seeded from one seed table as the reference seeds its
 // concatenated bead array.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -18,6 +19,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/sop_plugin.h"
 #include "../../include/sopgpu.h"
 #include "config.h"
 #include "engine.h"
@@ -26,45 +28,30 @@
 
 namespace sop {
 
-class SOPPotential {
-   public:
-    virtual ~SOPPotential() {}
-    std::string name;
-    virtual void compute() {}
-    virtual int getEnergiesCount() const { return 0; }
-    virtual float getEnergy(int traj, int id) const { return 0.0f; }
-};
-
-class SOPUpdater {
-   public:
-    virtual ~SOPUpdater() {}
-    std::string name;
-    int frequency = 1;
-    virtual void update() = 0;
-};
-
-class SOPIntegrator {
-   public:
-    virtual ~SOPIntegrator() {}
-    std::string name;
-    float h = 0.0f;
-    virtual void integrate() = 0;
-};
-
 struct Sim {
     Config cfg;
     Topology top;
     std::unique_ptr<Engine> eng;
-    std::vector<int> jobs;  // one per trajectory
+    std::vector<int> jobs;  // one per trajectory, admitted when the run starts
     int Ntr = 1, firstrun = 0, nav = 1000, N = 0, device = 0;
     std::string device_name;
     long long numsteps = 0, step = 0;
     int substep = 0;
-    bool hi = false, tea = false;
+    bool hi = false, tea = false, running = false, ran = false;
+    time_t t_start = 0;
     sop_params p{};
+    sop_sim* handle = nullptr;
     std::vector<std::unique_ptr<SOPPotential>> potentials;
     std::vector<std::unique_ptr<SOPUpdater>> updaters;
     std::unique_ptr<SOPIntegrator> integrator;
+    size_t builtin_updaters = 0;  // the registry's first entries are the program's own
+    // inputs of the trajectories (admitted by run())
+    std::vector<std::vector<float>> xyz;
+    std::vector<uint32_t> seeds;
+    std::vector<int> fixed, pulled;
+    // forces host potentials add in the current step, per trajectory
+    std::vector<std::vector<float>> host_forces;
+    std::vector<char> host_forces_set;
     // per-output-step caches
     long long energy_step = -1;
     std::vector<float> energies;  // [traj*3 + term]
@@ -98,13 +85,70 @@ struct Sim {
     }
 };
 
+// A plugin's failure message, set by sop_sim_fail() in a callback.
+static thread_local std::string g_plugin_error;
+
+// ---- host plugins (sop_sim_add_updater / sop_sim_add_potential) ----------
+struct HostUpdater : SOPUpdater {
+    sop_sim* h;
+    sop_plugin_fn fn;
+    void* user;
+    sop_release_fn rel;
+    HostUpdater(sop_sim* h_, const char* n, int f, sop_plugin_fn fn_, void* u, sop_release_fn r)
+        : h(h_), fn(fn_), user(u), rel(r) {
+        name = n;
+        frequency = f;
+    }
+    ~HostUpdater() override {
+        if (rel) rel(user);
+    }
+    void update() override {
+        g_plugin_error.clear();
+        if (fn(h, user))
+            throw ConfigError(g_plugin_error.empty() ? "updater '" + name + "' failed" : g_plugin_error);
+    }
+};
+
+struct HostPotential : SOPPotential {
+    sop_sim* h;
+    sop_plugin_fn fn;
+    sop_energy_fn efn;
+    void* user;
+    sop_release_fn rel;
+    int count;
+    HostPotential(sop_sim* h_, const char* n, int c, sop_plugin_fn fn_, sop_energy_fn e, void* u, sop_release_fn r)
+        : h(h_), fn(fn_), efn(e), user(u), rel(r), count(c) {
+        name = n;
+    }
+    ~HostPotential() override {
+        if (rel) rel(user);
+    }
+    void compute() override {
+        g_plugin_error.clear();
+        if (fn && fn(h, user))
+            throw ConfigError(g_plugin_error.empty() ? "potential '" + name + "' failed" : g_plugin_error);
+    }
+    int getEnergiesCount() const override { return count; }
+    float getEnergy(int traj, int id) override { return efn ? efn(h, traj, id, user) : 0.0f; }
+};
+
 // ---- potentials (creation order of initFF, gsop.cu:80-85) ----------------
+// The built-in potentials compute inside the fused step kernel, so their
+// compute() launches nothing.
 struct TermPotential : SOPPotential {
     Sim& sim;
     int term;
     TermPotential(Sim& s, const char* n, int t) : sim(s), term(t) { name = n; }
+    void compute() override {}
     int getEnergiesCount() const override { return 1; }
-    float getEnergy(int traj, int) const override { return sim.energies[3 * traj + term]; }
+    float getEnergy(int traj, int) override { return sim.energies[3 * traj + term]; }
+};
+
+// Indentation and pulling add their forces in the step kernel and report
+// no energy term (indentation.h, pulling.h).
+struct KernelPotential : SOPPotential {
+    explicit KernelPotential(const char* n) { name = n; }
+    void compute() override {}
 };
 
 // ---- updaters --------------------------------------------------------------
@@ -563,7 +607,16 @@ struct StepIntegrator : SOPIntegrator {  // LangevinIntegrator / TeaIntegrator
         name = n;
         h = s.p.timestep;
     }
-    void integrate() override { sim.check(sim.eng->stage_step(sim.substep++)); }
+    void integrate() override {
+        // forces the host potentials added for this step
+        for (int t = 0; t < sim.Ntr && !sim.host_forces_set.empty(); ++t) {
+            if (!sim.host_forces_set[t]) continue;
+            sim.check(sim.eng->upload_forces(sim.jobs[t], sim.host_forces[t].data()));
+            std::fill(sim.host_forces[t].begin(), sim.host_forces[t].end(), 0.0f);
+            sim.host_forces_set[t] = 0;
+        }
+        sim.check(sim.eng->stage_step(sim.substep++));
+    }
 };
 
 // ---- configuration -> sop_params (the plugins' PARAMETER()s) ---------------
@@ -672,10 +725,14 @@ static void read_params(const Config& c, sop_params& p, int& nav) {
     }
 }
 
-int run_config(const std::string& path, const std::vector<std::pair<std::string, std::string>>& overrides) {
-    const time_t t_start = std::time(nullptr);
+// initGPU + initFF (src/gsop.cu:45-106, src/sop-gpu.cpp:18-45): parameters,
+// topology, coordinates and seeds of every trajectory, the engine, and the
+// built-in plugins in initFF's order.  The trajectories are admitted on the
+// GPU by run_sim(), once the host has added its plugins.
+static void create_sim(Sim& sim, const std::string& path,
+                       const std::vector<std::pair<std::string, std::string>>& overrides) {
+    sim.t_start = std::time(nullptr);
     std::printf("===========================\nSOP-GPU (MI355X engine) %s\n===========================\n", sop_version());
-    Sim sim;
     sim.cfg.load(path);
     for (auto& kv : overrides) sim.cfg.set(kv.first, kv.second);
     Config& c = sim.cfg;
@@ -699,39 +756,38 @@ int run_config(const std::string& path, const std::vector<std::pair<std::string,
     sim.N = (int)sim.top.atoms.size();
     const int N = sim.N;
     // coordinates of every trajectory (gsop.cu:185-199, sop-gpu.cpp:41-45)
-    std::vector<std::vector<float>> xyz(sim.Ntr);
+    sim.xyz.assign(sim.Ntr, {});
     for (int t = 0; t < sim.Ntr; ++t) {
         const std::string cf = string_replace(c.str("coordinates"), "<run>", std::to_string(t + sim.firstrun));
         read_coordinates(cf, sim.top);
-        xyz[t].resize(4 * (size_t)N);
+        sim.xyz[t].resize(4 * (size_t)N);
         for (int i = 0; i < N; ++i) {
-            xyz[t][4 * i] = (float)sim.top.atoms[i].x;
-            xyz[t][4 * i + 1] = (float)sim.top.atoms[i].y;
-            xyz[t][4 * i + 2] = (float)sim.top.atoms[i].z;
-            xyz[t][4 * i + 3] = 0.0f;
+            sim.xyz[t][4 * i] = (float)sim.top.atoms[i].x;
+            sim.xyz[t][4 * i + 1] = (float)sim.top.atoms[i].y;
+            sim.xyz[t][4 * i + 2] = (float)sim.top.atoms[i].z;
+            sim.xyz[t][4 * i + 3] = 0.0f;
         }
     }
     // seed + firstrun, one seed table over the concatenated beads (langevin.cu:23-24)
     const int seed = (int)(c.has("seed") ? c.integer("seed") : (long long)std::time(nullptr)) + sim.firstrun;
     sim.p.seed = seed;
-    std::vector<uint32_t> seeds(4 * (size_t)N * sim.Ntr);
-    generate_seeds(seed, N * sim.Ntr, seeds.data());
-    std::vector<int> fixed, pulled;
+    sim.seeds.resize(4 * (size_t)N * sim.Ntr);
+    generate_seeds(seed, N * sim.Ntr, sim.seeds.data());
     if (sim.p.pulling) {
-        fixed = c.int_array("fixed");
-        pulled = c.int_array("pulled");
-        std::printf("%zu resid(s) fixed, %zu pulled.\n", fixed.size(), pulled.size());
+        sim.fixed = c.int_array("fixed");
+        sim.pulled = c.int_array("pulled");
+        std::printf("%zu resid(s) fixed, %zu pulled.\n", sim.fixed.size(), sim.pulled.size());
         // the pulled/fixed flags overwrite the B-factor column (pulling.cu:123-125)
-        for (int i : pulled)
+        for (int i : sim.pulled)
             if (i > N || i < 0)
                 throw ConfigError("Invalid pulling bead #" + std::to_string(i) + "; should be >=0 and <" + std::to_string(N));
-        for (int i : fixed)
+        for (int i : sim.fixed)
             if (i > N || i < 0)
                 throw ConfigError("Invalid fixed bead #" + std::to_string(i) + "; should be >=0 and <" + std::to_string(N));
         for (auto& a : sim.top.atoms) a.beta = 0.0;
-        for (int i : pulled)
+        for (int i : sim.pulled)
             if (i >= 0 && i < N) sim.top.atoms[i].beta = 2.0;
-        for (int i : fixed)
+        for (int i : sim.fixed)
             if (i >= 0 && i < N) sim.top.atoms[i].beta = 1.0;
     }
     if (sim.p.indentation && sim.Ntr > 1)
@@ -745,31 +801,6 @@ int run_config(const std::string& path, const std::vector<std::pair<std::string,
     sim.eng = std::make_unique<Engine>(device, (size_t)(0.9 * (double)free_b));
     sim.eng->set_tuning(0, 0);
     if (c.has("hi_chol_mode")) sim.eng->set_exact_mode((int)c.integer("hi_chol_mode"));
-    for (int t = 0; t < sim.Ntr; ++t) {
-        sop_system s{};
-        s.n = N;
-        s.coords4 = xyz[t].data();
-        s.nbonds = (int)sim.top.bi.size();
-        s.bond_i = sim.top.bi.data();
-        s.bond_j = sim.top.bj.data();
-        s.bond_r0 = sim.top.br0.data();
-        s.nnative = (int)sim.top.ni.size();
-        s.nat_i = sim.top.ni.data();
-        s.nat_j = sim.top.nj.data();
-        s.nat_r0 = sim.top.nr0.data();
-        s.nat_eh = sim.top.neh.data();
-        s.npairs = (int)sim.top.pi.size();
-        s.pair_i = sim.top.pi.data();
-        s.pair_j = sim.top.pj.data();
-        s.nfixed = (int)fixed.size();
-        s.fixed = fixed.data();
-        s.npulled = (int)pulled.size();
-        s.pulled = pulled.data();
-        s.rng_state4 = seeds.data() + 4 * (size_t)N * t;
-        int id = 0;
-        sim.check(sim.eng->add_job(sim.p, s, sim.numsteps, &id));
-        sim.jobs.push_back(id);
-    }
     // plugins in initFF's order (gsop.cu:80-101)
     sim.hi = sim.p.hi_on;
     sim.tea = sim.p.hi_on && !sim.p.hi_exact;
@@ -782,14 +813,10 @@ int run_config(const std::string& path, const std::vector<std::pair<std::string,
         sim.updaters.emplace_back(new IndentationTipUpdater(sim));
         if (show) sim.updaters.emplace_back(new IndentationAminoUpdater(sim));
         else sim.extra_atoms.clear();
-        auto* pot = new SOPPotential;
-        pot->name = "Indentation";
-        sim.potentials.emplace_back(pot);
+        sim.potentials.emplace_back(new KernelPotential("Indentation"));
     }
     if (sim.p.pulling) {
-        auto* pot = new SOPPotential;
-        pot->name = "Pulling";
-        sim.potentials.emplace_back(pot);
+        sim.potentials.emplace_back(new KernelPotential("Pulling"));
         if (sim.p.deltax != 0.0f) sim.updaters.emplace_back(new PullingUpdater(sim));
     }
     sim.updaters.emplace_back(new PossiblepairUpdater(sim));
@@ -799,11 +826,59 @@ int run_config(const std::string& path, const std::vector<std::pair<std::string,
     if (sim.p.heating) sim.updaters.emplace_back(new TemperatureUpdater(sim));
     if (sim.tea) sim.updaters.emplace_back(new TeaUpdater(sim));
     sim.integrator.reset(new StepIntegrator(sim, sim.hi ? "BDHI-TEA" : "Langevin"));
+    sim.builtin_updaters = sim.updaters.size();
+}
+
+// The trajectories on the GPU, runGPU (gsop.cu:111-163) and the final
+// coordinates (sop-gpu.cpp:47-65).
+static void run_sim(Sim& sim) {
+    const Config& c = sim.cfg;
+    const int N = sim.N;
+    // host plugins: their updaters' frequencies bound the stride, their
+    // potentials' forces reach the step through the job's force array
+    int host_min = 0;
+    for (size_t k = sim.builtin_updaters; k < sim.updaters.size(); ++k)
+        host_min = host_min ? std::min(host_min, sim.updaters[k]->frequency) : sim.updaters[k]->frequency;
+    sim.p.updater_freq_min = host_min;
+    bool host_pot = false;
+    for (auto& pot : sim.potentials)
+        if (dynamic_cast<HostPotential*>(pot.get())) host_pot = true;
+    sim.p.host_forces = host_pot ? 1 : 0;
+    if (host_pot) {
+        sim.host_forces.assign(sim.Ntr, std::vector<float>(4 * (size_t)N, 0.0f));
+        sim.host_forces_set.assign(sim.Ntr, 0);
+    }
+    sim.running = true;
+    for (int t = 0; t < sim.Ntr; ++t) {
+        sop_system s{};
+        s.n = N;
+        s.coords4 = sim.xyz[t].data();
+        s.nbonds = (int)sim.top.bi.size();
+        s.bond_i = sim.top.bi.data();
+        s.bond_j = sim.top.bj.data();
+        s.bond_r0 = sim.top.br0.data();
+        s.nnative = (int)sim.top.ni.size();
+        s.nat_i = sim.top.ni.data();
+        s.nat_j = sim.top.nj.data();
+        s.nat_r0 = sim.top.nr0.data();
+        s.nat_eh = sim.top.neh.data();
+        s.npairs = (int)sim.top.pi.size();
+        s.pair_i = sim.top.pi.data();
+        s.pair_j = sim.top.pj.data();
+        s.nfixed = (int)sim.fixed.size();
+        s.fixed = sim.fixed.data();
+        s.npulled = (int)sim.pulled.size();
+        s.pulled = sim.pulled.data();
+        s.rng_state4 = sim.seeds.data() + 4 * (size_t)N * t;
+        int id = 0;
+        sim.check(sim.eng->add_job(sim.p, s, sim.numsteps, &id));
+        sim.jobs.push_back(id);
+    }
 
     const std::string ref = sim.file_param("reffilename", "<name>.ref.pdb");
     std::printf("Saving reference PDB: '%s'.\n", ref.c_str());
     write_pdb(ref, sim.top, sim.extra_atoms);  // holds the last trajectory's coordinates, as in the reference
-    std::printf("System initialization took %.f second(s).\n", difftime(std::time(nullptr), t_start));
+    std::printf("System initialization took %.f second(s).\n", difftime(std::time(nullptr), sim.t_start));
     std::printf("Starting %d runs (%lld steps).\n", sim.Ntr, sim.numsteps);
 
     // runGPU (gsop.cu:111-163)
@@ -812,17 +887,18 @@ int run_config(const std::string& path, const std::vector<std::pair<std::string,
         if (u->frequency < stride) stride = u->frequency;
     // the Verlet list is built once before the loop, from the possible list
     // of the topology's [ pairs ], so the step-0 energies use it (gsop.cu:126)
-    for (auto& u : sim.updaters)
-        if (u->name == "Pairlist") u->update();
+    for (size_t k = 0; k < sim.builtin_updaters; ++k)
+        if (sim.updaters[k]->name == "Pairlist") sim.updaters[k]->update();
     for (long long b = sim.step / stride; b <= sim.numsteps / stride; ++b) {
-        for (auto& u : sim.updaters) u->update();
         sim.substep = 0;
+        for (auto& u : sim.updaters) u->update();
         for (int j = 0; j < stride; ++j) {
             for (auto& pot : sim.potentials) pot->compute();
             sim.integrator->integrate();
         }
         sim.check(sim.eng->stage_advance(stride));
         sim.step += stride;
+        sim.substep = 0;
     }
     const std::string fin = sim.file_param("finalcoord", "<name>_<run>_final.pdb");
     for (int t = 0; t < sim.Ntr; ++t) {
@@ -842,20 +918,203 @@ int run_config(const std::string& path, const std::vector<std::pair<std::string,
         if (ovf & 3) throw ConfigError("a pairlist outgrew its capacity (max_pairs / max_possiblePairs)");
         if (ovf & 12) throw ConfigError("HI tensor check failed (hi_capricious)");
     }
-    return 0;
+    (void)c;
 }
 
 }  // namespace sop
 
-extern "C" int sop_run_config(const char* path, int noverride, const char* const* names, const char* const* values) {
-    if (!path) return sop::fail(SOP_E_ARG, "null path");
+struct sop_sim {
+    std::unique_ptr<sop::Sim> sim;
+};
+
+namespace {
+std::vector<std::pair<std::string, std::string>> override_list(int n, const char* const* names,
+                                                               const char* const* values) {
     std::vector<std::pair<std::string, std::string>> ov;
-    for (int k = 0; k < noverride; ++k) ov.push_back({names[k], values[k]});
+    for (int k = 0; k < n; ++k) ov.push_back({names[k], values[k]});
+    return ov;
+}
+
+template <typename F>
+int guarded(F&& f) {
     try {
-        return sop::run_config(path, ov);
+        return f();
     } catch (const sop::ConfigError& e) {
         return sop::fail(SOP_E_CONFIG, e.what());
     } catch (const std::exception& e) {
         return sop::fail(SOP_E_CONFIG, e.what());
     }
 }
+}  // namespace
+
+extern "C" {
+
+int sop_run_config(const char* path, int noverride, const char* const* names, const char* const* values) {
+    if (!path || (noverride > 0 && (!names || !values))) return sop::fail(SOP_E_ARG, "null path or overrides");
+    return guarded([&] {
+        sop::Sim sim;
+        sop::create_sim(sim, path, override_list(noverride, names, values));
+        sop::run_sim(sim);
+        return 0;
+    });
+}
+
+int sop_sim_create(const char* path, int noverride, const char* const* names, const char* const* values,
+                   sop_sim** out) {
+    if (!out) return sop::fail(SOP_E_ARG, "null output");
+    *out = nullptr;
+    if (!path || (noverride > 0 && (!names || !values))) return sop::fail(SOP_E_ARG, "null path or overrides");
+    auto* h = new sop_sim;
+    h->sim = std::make_unique<sop::Sim>();
+    h->sim->handle = h;
+    const int rc = guarded([&] {
+        sop::create_sim(*h->sim, path, override_list(noverride, names, values));
+        return 0;
+    });
+    if (rc) {
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return 0;
+}
+
+int sop_sim_run(sop_sim* s) {
+    if (!s || !s->sim) return sop::fail(SOP_E_ARG, "null simulation");
+    if (s->sim->ran) return sop::fail(SOP_E_STATE, "the simulation has already run");
+    s->sim->ran = true;
+    return guarded([&] {
+        sop::run_sim(*s->sim);
+        return 0;
+    });
+}
+
+void sop_sim_destroy(sop_sim* s) { delete s; }
+
+void sop_sim_fail(const char* message) { sop::g_plugin_error = message ? message : ""; }
+
+int sop_sim_add_updater(sop_sim* s, const char* name, int frequency, sop_plugin_fn update, void* user,
+                        sop_release_fn release) {
+    int rc = 0;
+    if (!s || !s->sim || !name || !update) rc = sop::fail(SOP_E_ARG, "null argument");
+    else if (frequency <= 0) rc = sop::fail(SOP_E_ARG, "updater frequency must be positive");
+    else if (s->sim->ran) rc = sop::fail(SOP_E_STATE, "plugins are registered before the run");
+    if (rc) {
+        if (release) release(user);
+        return rc;
+    }
+    s->sim->updaters.emplace_back(new sop::HostUpdater(s, name, frequency, update, user, release));
+    return 0;
+}
+
+int sop_sim_add_potential(sop_sim* s, const char* name, int energies_count, sop_plugin_fn compute,
+                          sop_energy_fn energy, void* user, sop_release_fn release) {
+    int rc = 0;
+    if (!s || !s->sim || !name) rc = sop::fail(SOP_E_ARG, "null argument");
+    else if (energies_count < 0 || (energies_count > 0 && !energy)) rc = sop::fail(SOP_E_ARG, "bad energy terms");
+    else if (s->sim->ran) rc = sop::fail(SOP_E_STATE, "plugins are registered before the run");
+    if (rc) {
+        if (release) release(user);
+        return rc;
+    }
+    s->sim->potentials.emplace_back(new sop::HostPotential(s, name, energies_count, compute, energy, user, release));
+    return 0;
+}
+
+int sop_sim_potential_count(const sop_sim* s, int* count) {
+    if (!s || !s->sim || !count) return sop::fail(SOP_E_ARG, "null argument");
+    *count = (int)s->sim->potentials.size();
+    return 0;
+}
+
+int sop_sim_updater_count(const sop_sim* s, int* count) {
+    if (!s || !s->sim || !count) return sop::fail(SOP_E_ARG, "null argument");
+    *count = (int)s->sim->updaters.size();
+    return 0;
+}
+
+const char* sop_sim_potential_name(const sop_sim* s, int index) {
+    if (!s || !s->sim || index < 0 || index >= (int)s->sim->potentials.size()) return nullptr;
+    return s->sim->potentials[index]->name.c_str();
+}
+
+const char* sop_sim_updater_name(const sop_sim* s, int index) {
+    if (!s || !s->sim || index < 0 || index >= (int)s->sim->updaters.size()) return nullptr;
+    return s->sim->updaters[index]->name.c_str();
+}
+
+int sop_sim_updater_frequency(const sop_sim* s, int index, int* frequency) {
+    if (!s || !s->sim || !frequency || index < 0 || index >= (int)s->sim->updaters.size())
+        return sop::fail(SOP_E_ARG, "bad argument");
+    *frequency = s->sim->updaters[index]->frequency;
+    return 0;
+}
+
+int sop_sim_potential_by_name(const sop_sim* s, const char* name, int* index) {
+    if (!s || !s->sim || !name || !index) return sop::fail(SOP_E_ARG, "null argument");
+    for (size_t k = 0; k < s->sim->potentials.size(); ++k)
+        if (s->sim->potentials[k]->name == name) {
+            *index = (int)k;
+            return 0;
+        }
+    return sop::fail(SOP_E_ARG, std::string("Unable to find requested potential '") + name + "'");
+}
+
+int sop_sim_updater_by_name(const sop_sim* s, const char* name, int* index) {
+    if (!s || !s->sim || !name || !index) return sop::fail(SOP_E_ARG, "null argument");
+    for (size_t k = 0; k < s->sim->updaters.size(); ++k)
+        if (s->sim->updaters[k]->name == name) {
+            *index = (int)k;
+            return 0;
+        }
+    return sop::fail(SOP_E_ARG, std::string("Unable to find requested updater '") + name + "'");
+}
+
+const char* sop_sim_integrator_name(const sop_sim* s) {
+    if (!s || !s->sim || !s->sim->integrator) return nullptr;
+    return s->sim->integrator->name.c_str();
+}
+
+int sop_sim_get_energy(sop_sim* s, int index, int traj, int id, float* energy) {
+    if (!s || !s->sim || !energy) return sop::fail(SOP_E_ARG, "null argument");
+    sop::Sim& sim = *s->sim;
+    if (index < 0 || index >= (int)sim.potentials.size() || traj < 0 || traj >= sim.Ntr || id < 0 ||
+        id >= sim.potentials[index]->getEnergiesCount())
+        return sop::fail(SOP_E_ARG, "no such energy term");
+    if (sim.energies.empty()) return sop::fail(SOP_E_STATE, "no energies computed yet");
+    *energy = sim.potentials[index]->getEnergy(traj, id);
+    return 0;
+}
+
+int sop_sim_state(const sop_sim* s, long long* step, int* n, int* ntr) {
+    if (!s || !s->sim) return sop::fail(SOP_E_ARG, "null simulation");
+    if (step) *step = s->sim->step + s->sim->substep;
+    if (n) *n = s->sim->N;
+    if (ntr) *ntr = s->sim->Ntr;
+    return 0;
+}
+
+int sop_sim_get_coords(sop_sim* s, int traj, float* out4) {
+    if (!s || !s->sim || !out4) return sop::fail(SOP_E_ARG, "null argument");
+    sop::Sim& sim = *s->sim;
+    if (traj < 0 || traj >= sim.Ntr) return sop::fail(SOP_E_ARG, "no such trajectory");
+    if (sim.jobs.empty()) {  // not admitted yet: the initial coordinates
+        std::memcpy(out4, sim.xyz[traj].data(), sim.xyz[traj].size() * sizeof(float));
+        return 0;
+    }
+    return sim.eng->get_coords_at(sim.jobs[traj], sim.step + sim.substep, out4);
+}
+
+int sop_sim_add_forces(sop_sim* s, int traj, const float* f4) {
+    if (!s || !s->sim || !f4) return sop::fail(SOP_E_ARG, "null argument");
+    sop::Sim& sim = *s->sim;
+    if (traj < 0 || traj >= sim.Ntr) return sop::fail(SOP_E_ARG, "no such trajectory");
+    if (!sim.running || sim.host_forces.empty())
+        return sop::fail(SOP_E_STATE, "forces are added by a registered potential's compute() during the run");
+    std::vector<float>& acc = sim.host_forces[traj];
+    for (size_t i = 0; i < acc.size(); ++i) acc[i] += (i & 3) == 3 ? 0.0f : f4[i];
+    sim.host_forces_set[traj] = 1;
+    return 0;
+}
+
+}  // extern "C"

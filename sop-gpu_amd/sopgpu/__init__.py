@@ -66,6 +66,7 @@ PARAM_FIELDS = [
     ("hi_on", C.c_int), ("hi_exact", C.c_int), ("hi_a", C.c_float),
     ("hi_epsilon_freq", C.c_int), ("hi_capricious", C.c_int), ("hi_unlisted", C.c_int),
     ("hi_epsmax", C.c_float), ("hi_chol_freq", C.c_int), ("hi_faithful_blocks", C.c_int),
+    ("updater_freq_min", C.c_int), ("host_forces", C.c_int),
 ]
 
 
@@ -93,7 +94,7 @@ DEFAULTS = dict(
     indTipA=0.0, indTipB=1.0, indSurfA=0.0, indSurfB=1.0, indTipZeta=5000.0, indOutputFreq=1000,
     indRetractionStep=-1,
     hi_on=0, hi_exact=0, hi_a=1.8, hi_epsilon_freq=10, hi_capricious=1, hi_unlisted=1, hi_epsmax=999.0,
-    hi_chol_freq=1, hi_faithful_blocks=1,
+    hi_chol_freq=1, hi_faithful_blocks=1, updater_freq_min=0, host_forces=0,
 )
 
 
@@ -201,7 +202,18 @@ _sig("sop_engine_job_bytes", C.c_int, C.POINTER(Params), C.POINTER(SystemStruct)
 _sig("sop_engine_set_tuning", C.c_int, VP, C.c_int, C.c_int)
 _sig("sop_engine_set_cells", C.c_int, VP, C.c_int, C.c_int)
 _sig("sop_engine_set_lds_step", C.c_int, VP, C.c_int)
-_sig("sop_engine_autotune", C.c_int, VP, C.POINTER(C.c_int))
+class Tuning(C.Structure):
+    _fields_ = [("max_blocks", C.c_int), ("unit_rows", C.c_int), ("shared_units", C.c_int),
+                ("pair_batch", C.c_int), ("verlet_chunk", C.c_int), ("sort_every", C.c_int)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+_sig("sop_engine_autotune", C.c_int, VP, C.POINTER(Tuning), C.POINTER(C.c_double), C.POINTER(C.c_double),
+     C.POINTER(C.c_int))
+_sig("sop_engine_get_shape", C.c_int, VP, C.POINTER(Tuning))
+_sig("sop_engine_set_shape", C.c_int, VP, C.POINTER(Tuning))
 _sig("sop_engine_run_blocks", C.c_int, VP, C.c_int)
 _sig("sop_engine_run_job", C.c_int, VP, C.c_int)
 _sig("sop_engine_block_steps", C.c_int, VP, C.POINTER(C.c_int))
@@ -230,6 +242,9 @@ _sig("sop_chol_factor", C.c_int, C.c_int, C.c_int, VP, C.c_int, VP, C.c_int, C.P
 _sig("sop_server_create", C.c_int, C.c_int, C.c_ulonglong, C.POINTER(VP))
 _sig("sop_server_destroy", None, VP)
 _sig("sop_server_set_tuning", C.c_int, VP, C.c_int, C.c_int)
+_sig("sop_server_set_shape", C.c_int, VP, C.POINTER(Tuning))
+_sig("sop_server_autotune", C.c_int, VP, C.POINTER(Tuning), C.POINTER(C.c_double), C.POINTER(C.c_double),
+     C.POINTER(C.c_int))
 _sig("sop_server_submit", C.c_int, VP, C.POINTER(Params), C.POINTER(SystemStruct), C.c_longlong, C.POINTER(C.c_int))
 _sig("sop_server_pump", C.c_int, VP, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int))
 _sig("sop_server_poll", C.c_int, VP, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double),
@@ -250,19 +265,48 @@ _sig("sop_write_pdb", C.c_int, C.c_char_p, VP, C.c_int, C.c_char_p)
 _sig("sop_write_dcd", C.c_int, C.c_char_p, C.c_int, C.c_int, VP, C.c_int, C.c_float)
 _sig("sop_run_config", C.c_int, C.c_char_p, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_char_p))
 
+# host plugins (sop_plugin.h)
+PLUGIN_FN = C.CFUNCTYPE(C.c_int, VP, VP)
+ENERGY_FN = C.CFUNCTYPE(C.c_float, VP, C.c_int, C.c_int, VP)
+RELEASE_FN = C.CFUNCTYPE(None, VP)
+_sig("sop_sim_create", C.c_int, C.c_char_p, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_char_p), C.POINTER(VP))
+_sig("sop_sim_run", C.c_int, VP)
+_sig("sop_sim_destroy", None, VP)
+_sig("sop_sim_fail", None, C.c_char_p)
+_sig("sop_sim_add_updater", C.c_int, VP, C.c_char_p, C.c_int, PLUGIN_FN, VP, RELEASE_FN)
+_sig("sop_sim_add_potential", C.c_int, VP, C.c_char_p, C.c_int, PLUGIN_FN, ENERGY_FN, VP, RELEASE_FN)
+_sig("sop_sim_potential_count", C.c_int, VP, C.POINTER(C.c_int))
+_sig("sop_sim_updater_count", C.c_int, VP, C.POINTER(C.c_int))
+_sig("sop_sim_potential_name", C.c_char_p, VP, C.c_int)
+_sig("sop_sim_updater_name", C.c_char_p, VP, C.c_int)
+_sig("sop_sim_updater_frequency", C.c_int, VP, C.c_int, C.POINTER(C.c_int))
+_sig("sop_sim_potential_by_name", C.c_int, VP, C.c_char_p, C.POINTER(C.c_int))
+_sig("sop_sim_updater_by_name", C.c_int, VP, C.c_char_p, C.POINTER(C.c_int))
+_sig("sop_sim_integrator_name", C.c_char_p, VP)
+_sig("sop_sim_get_energy", C.c_int, VP, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float))
+_sig("sop_sim_state", C.c_int, VP, C.POINTER(C.c_longlong), C.POINTER(C.c_int), C.POINTER(C.c_int))
+_sig("sop_sim_get_coords", C.c_int, VP, C.c_int, VP)
+_sig("sop_sim_add_forces", C.c_int, VP, C.c_int, VP)
+
 EXPORTED = [
     "sop_last_error", "sop_version", "sop_engine_create", "sop_engine_destroy", "sop_engine_add_job",
     "sop_engine_remove_job", "sop_engine_job_bytes", "sop_engine_set_tuning", "sop_engine_set_cells", "sop_engine_set_lds_step", "sop_engine_autotune",
+    "sop_engine_get_shape", "sop_engine_set_shape",
     "sop_engine_run_blocks", "sop_engine_run_job", "sop_engine_block_steps", "sop_engine_synchronize",
     "sop_engine_job_status", "sop_engine_active_jobs", "sop_engine_get_coords", "sop_engine_set_coords",
     "sop_engine_get_temperature", "sop_engine_get_rng", "sop_engine_get_ext", "sop_engine_list_cap",
     "sop_engine_get_list", "sop_engine_get_indentation", "sop_engine_compute_forces",
     "sop_engine_compute_energies", "sop_engine_rebuild_lists", "sop_engine_profile", "sop_engine_profile_read",
     "sop_engine_list_stats", "sop_engine_set_chol_mode", "sop_engine_get_hi", "sop_engine_get_hi_factor",
-    "sop_chol_factor", "sop_server_create", "sop_server_destroy", "sop_server_set_tuning", "sop_server_submit",
+    "sop_chol_factor", "sop_server_create", "sop_server_destroy", "sop_server_set_tuning", "sop_server_set_shape",
+    "sop_server_autotune", "sop_server_submit",
     "sop_server_pump", "sop_server_poll", "sop_server_result", "sop_server_stats", "sop_copy_bandwidth",
     "sop_generate_seeds", "sop_rng_probe", "sop_check_division", "sop_synth_chain", "sop_synth_shell",
     "sop_free_system", "sop_run_config", "sop_read_topology", "sop_write_pdb", "sop_write_dcd",
+    "sop_sim_create", "sop_sim_run", "sop_sim_destroy", "sop_sim_fail", "sop_sim_add_updater",
+    "sop_sim_add_potential", "sop_sim_potential_count", "sop_sim_updater_count", "sop_sim_potential_name",
+    "sop_sim_updater_name", "sop_sim_updater_frequency", "sop_sim_potential_by_name", "sop_sim_updater_by_name",
+    "sop_sim_integrator_name", "sop_sim_get_energy", "sop_sim_state", "sop_sim_get_coords", "sop_sim_add_forces",
 ]
 
 
@@ -408,10 +452,25 @@ class Engine:
     def set_lds_step(self, enabled: bool = True):
         check(lib.sop_engine_set_lds_step(self.h, int(enabled)))
 
-    def autotune(self) -> int:
-        b = C.c_int()
-        check(lib.sop_engine_autotune(self.h, C.byref(b)))
-        return b.value
+    def autotune(self) -> dict:
+        """Searches the launch shapes on the admitted jobs (they do not
+        advance) and keeps the fastest; returns the shape and the ms per step
+        before and after."""
+        t, a, b, n = Tuning(), C.c_double(), C.c_double(), C.c_int()
+        check(lib.sop_engine_autotune(self.h, C.byref(t), C.byref(a), C.byref(b), C.byref(n)))
+        return dict(shape=t.as_dict(), ms_per_step_start=a.value, ms_per_step_best=b.value, trials=n.value)
+
+    def shape(self) -> dict:
+        t = Tuning()
+        check(lib.sop_engine_get_shape(self.h, C.byref(t)))
+        return t.as_dict()
+
+    def set_shape(self, **kw):
+        t = Tuning()
+        check(lib.sop_engine_get_shape(self.h, C.byref(t)))
+        for k, v in kw.items():
+            setattr(t, k, int(v))
+        check(lib.sop_engine_set_shape(self.h, C.byref(t)))
 
     def run_blocks(self, nblocks: int):
         check(lib.sop_engine_run_blocks(self.h, nblocks))
@@ -585,6 +644,15 @@ class Server:
     def set_tuning(self, max_blocks: int = 0, use_graph: bool = True):
         check(lib.sop_server_set_tuning(self.h, max_blocks, int(use_graph)))
 
+    def set_shape(self, shape: dict):
+        t = Tuning(**{k: int(v) for k, v in shape.items()})
+        check(lib.sop_server_set_shape(self.h, C.byref(t)))
+
+    def autotune(self) -> dict:
+        t, a, b, n = Tuning(), C.c_double(), C.c_double(), C.c_int()
+        check(lib.sop_server_autotune(self.h, C.byref(t), C.byref(a), C.byref(b), C.byref(n)))
+        return dict(shape=t.as_dict(), ms_per_step_start=a.value, ms_per_step_best=b.value, trials=n.value)
+
     def submit(self, params: Params, system: System, numsteps: int) -> int:
         st, keep = system.struct()
         t = C.c_int()
@@ -612,3 +680,109 @@ class Server:
         check(lib.sop_server_stats(self.h, C.byref(bs), C.byref(busy), C.byref(adm), C.byref(fin), C.byref(peak)))
         return dict(bead_steps=bs.value, busy_s=busy.value, admitted=adm.value, finished=fin.value,
                     peak_bytes=peak.value)
+
+
+class Simulation:
+    """The sop-gpu program with its plugin registries (sop_sim_*): create
+    from a parameter file, add host updaters / potentials (Python callables
+    wrapped as C callbacks), run once."""
+
+    def __init__(self, path: str, **overrides):
+        names = [k.encode() for k in overrides]
+        values = [str(v).encode() for v in overrides.values()]
+        n = len(names)
+        h = VP()
+        check(lib.sop_sim_create(path.encode(), n, (C.c_char_p * max(n, 1))(*names),
+                                 (C.c_char_p * max(n, 1))(*values), C.byref(h)))
+        self.h = h
+        self._keep = []
+
+    def close(self):
+        if self.h:
+            lib.sop_sim_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @staticmethod
+    def _wrap(fn):
+        def tramp(sim, user):
+            try:
+                r = fn()
+                return 0 if r is None else int(r)
+            except Exception as e:  # the plugin's failure ends the run with its text
+                lib.sop_sim_fail(str(e).encode())
+                return -1
+        return PLUGIN_FN(tramp)
+
+    def add_updater(self, name: str, frequency: int, update):
+        cb = self._wrap(update)
+        self._keep.append(cb)
+        check(lib.sop_sim_add_updater(self.h, name.encode(), frequency, cb, None, RELEASE_FN()))
+
+    def add_potential(self, name: str, compute, energies_count: int = 0, energy=None):
+        cb = self._wrap(compute)
+        ecb = ENERGY_FN(lambda sim, traj, i, user: float(energy(traj, i))) if energy else ENERGY_FN()
+        self._keep += [cb, ecb]
+        check(lib.sop_sim_add_potential(self.h, name.encode(), energies_count, cb, ecb, None, RELEASE_FN()))
+
+    def run(self):
+        check(lib.sop_sim_run(self.h))
+
+    def potentials(self):
+        c = C.c_int()
+        check(lib.sop_sim_potential_count(self.h, C.byref(c)))
+        return [lib.sop_sim_potential_name(self.h, k).decode() for k in range(c.value)]
+
+    def updaters(self):
+        c = C.c_int()
+        check(lib.sop_sim_updater_count(self.h, C.byref(c)))
+        out = []
+        for k in range(c.value):
+            f = C.c_int()
+            check(lib.sop_sim_updater_frequency(self.h, k, C.byref(f)))
+            out.append((lib.sop_sim_updater_name(self.h, k).decode(), f.value))
+        return out
+
+    def potential_by_name(self, name: str) -> int:
+        i = C.c_int()
+        check(lib.sop_sim_potential_by_name(self.h, name.encode(), C.byref(i)))
+        return i.value
+
+    def updater_by_name(self, name: str) -> int:
+        i = C.c_int()
+        check(lib.sop_sim_updater_by_name(self.h, name.encode(), C.byref(i)))
+        return i.value
+
+    def integrator(self) -> str:
+        return lib.sop_sim_integrator_name(self.h).decode()
+
+    def state(self):
+        st, n, ntr = C.c_longlong(), C.c_int(), C.c_int()
+        check(lib.sop_sim_state(self.h, C.byref(st), C.byref(n), C.byref(ntr)))
+        return st.value, n.value, ntr.value
+
+    def coords(self, traj: int = 0) -> np.ndarray:
+        n = self.state()[1]
+        out = np.zeros((n, 4), np.float32)
+        check(lib.sop_sim_get_coords(self.h, traj, _ptr(out)))
+        return out
+
+    def add_forces(self, traj: int, f4: np.ndarray):
+        f = np.ascontiguousarray(f4, dtype=np.float32)
+        check(lib.sop_sim_add_forces(self.h, traj, _ptr(f)))
+
+    def energy(self, index: int, traj: int = 0, term: int = 0) -> float:
+        e = C.c_float()
+        check(lib.sop_sim_get_energy(self.h, index, traj, term, C.byref(e)))
+        return e.value

@@ -57,21 +57,24 @@ def test_exact_zero_temperature_bit_exact(mode):
 @pytest.mark.parametrize("n,mode", [(128, S.CHOL_BF16X3), (128, S.CHOL_F32), (700, S.CHOL_BF16X3),
                                     (700, S.CHOL_F32)])
 def test_engine_factor_covariance(n, mode):
+    """The factor the engine keeps for an exact-HI job: refactorised only at
+    step 0 (hi_chol_freq beyond the run), so it is the factor of the initial
+    structure's RPY matrix, whose FP64 factor it must match."""
     sysm = S.synth_chain(n, seed=72)
-    kw = hi_kw(hi_exact=1, hi_a=1.9, temperature=0.6)
+    kw = hi_kw(hi_exact=1, hi_a=1.9, temperature=0.6, hi_chol_freq=1000)
     with S.Engine(0, 4 << 30) as e:
         e.set_chol_mode(mode)
         jid = e.add_job(S.make_params(**kw), sysm, 0)
-        e.run_job(jid)  # 10 steps; the factor is of the step-9 structure
+        e.run_job(jid)  # 10 steps, the factor is of step 0
         L = e.hi_factor(jid).astype(np.float64)
-        c_prev = None
-    assert np.all(np.isfinite(L))
-    # compare with the FP64 factor of an RPY matrix of the same size
+    assert np.abs(np.triu(L, 1)).max() == 0.0
     D = O.rpy_matrix(sysm.coords, 1.9)
     L64 = np.linalg.cholesky(D)
-    assert np.abs(np.triu(L, 1)).max() == 0.0
-    # the engine's factor is of the structure after 9 steps; its own
-    # reconstruction error is checked through the standalone entry below
+    cov = np.linalg.norm(L @ L.T - D) / np.linalg.norm(D)
+    fac = np.linalg.norm(L - L64) / np.linalg.norm(L64)
+    print(f"n={n} mode={mode}: covariance error {cov:.3e}, factor error {fac:.3e}")
+    assert cov < 1e-5, cov
+    assert fac < 1e-4, fac
 
 
 @pytest.mark.parametrize("n", [384, 1536, 3 * 1000])
@@ -89,12 +92,13 @@ def test_standalone_cholesky_covariance_error(n):
         assert cov_err < tol, (mode, cov_err)
 
 
-def test_exact_noise_statistics_vs_oracle():
+@pytest.mark.parametrize("faithful", [0, 1])
+def test_exact_noise_statistics_vs_oracle(faithful):
     """Finite temperature: the exact path's displacement statistics agree
     with the oracle's over a short run (same seeds, different rounding)."""
     n = 40
     sysm = numpy_chain(n, seed=74)
-    kw = hi_kw(hi_exact=1, temperature=0.6, hi_faithful_blocks=0)
+    kw = hi_kw(hi_exact=1, temperature=0.6, hi_faithful_blocks=faithful)
     with S.Engine(0, 1 << 30) as e:
         jid = e.add_job(S.make_params(**kw), sysm, 0)
         e.run_job(jid)
@@ -104,3 +108,49 @@ def test_exact_noise_statistics_vs_oracle():
     # one block of 10 steps from identical seeds: trajectories agree to FP32
     # rounding of the factor
     np.testing.assert_allclose(c, ref.coords(), rtol=0, atol=5e-4)
+
+
+@pytest.mark.parametrize("faithful", [1, 0])
+@pytest.mark.parametrize("mode", [S.CHOL_BF16X3, S.CHOL_F32])
+def test_exact_finite_temperature_step_vs_oracle(faithful, mode):
+    """One exact-HI step at T = 300 K from a known state, 200 beads (order
+    600): the displacement is (h/zeta)(D F + B rho g) with B the factor read
+    block-symmetrically (the reference's default, bdhitea_kernel.cu:322-337)
+    or the factor itself; the GPU's equals the oracle's and the FP64
+    statement to the factor's rounding."""
+    n = 200
+    sysm = numpy_chain(n, seed=75)
+    kw = hi_kw(hi_exact=1, temperature=0.6, hi_faithful_blocks=faithful, nav=1, outputtiming=1, dcdfreq=1,
+               pairs_freq=1, possiblepairs_freq=1, hi_epsilon_freq=1)
+    x0 = sysm.coords[:, :3].astype(np.float64)
+    with S.Engine(0, 1 << 30) as e:
+        e.set_chol_mode(mode)
+        jid = e.add_job(S.make_params(**kw), sysm, 0)
+        e.run_job(jid)
+        assert e.status(jid)[0] == 1
+        dx_gpu = e.coords(jid)[:, :3].astype(np.float64) - x0
+    ref = O.Sim(O.make_params(**kw), sysm)
+    g, _ = O.gaussians(ref.rng(), 1)
+    f = O.Sim(O.make_params(**kw), sysm)
+    f.build_possible()
+    f.build_verlet()
+    F = f.forces64()
+    assert ref.run(0) == 1
+    dx_ref = ref.coords()[:, :3].astype(np.float64) - x0
+    D = O.rpy_matrix(sysm.coords, kw["hi_a"])
+    L = np.linalg.cholesky(D)
+    if faithful:
+        B = np.zeros_like(L)
+        for i in range(n):
+            for j in range(i + 1):
+                blk = L[3 * i:3 * i + 3, 3 * j:3 * j + 3]
+                B[3 * i:3 * i + 3, 3 * j:3 * j + 3] = np.tril(blk) + np.tril(blk, -1).T
+    else:
+        B = L
+    rho = np.sqrt(2 * 0.6 * kw["zeta"] / kw["timestep"])
+    expect = kw["timestep"] / kw["zeta"] * (D @ F.ravel() + B @ (rho * g[:, 0, :].astype(np.float64)).ravel())
+    expect = expect.reshape(n, 3)
+    ulp = float(np.spacing(np.float32(np.abs(x0).max() + 1.0)))
+    for got in (dx_gpu, dx_ref):
+        np.testing.assert_allclose(got, expect, rtol=5e-5, atol=4 * ulp)
+    np.testing.assert_allclose(dx_gpu, dx_ref, rtol=5e-5, atol=4 * ulp)

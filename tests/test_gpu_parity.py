@@ -308,3 +308,73 @@ def test_pairlist_overflow_is_reported(eng):
         assert eng.status(jid)[2] & 1
     finally:
         eng.remove_job(jid)
+
+
+def test_verlet_overflow_is_capped_and_reported():
+    """A job whose Verlet rows outgrow max_pairs = 16 (a compact 300-bead
+    globule has ~100 partners within 20 A) sets overflow bit 2, keeps the
+    first 16 entries of every row, as the oracle does, and writes nothing
+    past its list; a job batched with it gives the oracle's bits too."""
+    dense, other = numpy_chain(300, seed=61), numpy_chain(250, seed=62)
+    ka = c0_params(max_pairs=16, seed=611)
+    kb = c0_params(seed=622)
+    with S.Engine(0, 1 << 30) as e:
+        a = e.add_job(S.make_params(**ka), dense, 190)
+        b = e.add_job(S.make_params(**kb), other, 190)
+        e.run_blocks(200 // e.block_steps())
+        assert e.status(a)[2] & 2 and e.status(b)[2] == 0
+        cnt, _ = e.list(a, 2)
+        assert cnt.max() == 16
+        ca, cb = e.coords(a), e.coords(b)
+    ra = O.Sim(O.make_params(**ka), dense)
+    ra.run(190)
+    rb = O.Sim(O.make_params(**kb), other)
+    rb.run(190)
+    np.testing.assert_array_equal(ca, ra.coords())
+    np.testing.assert_array_equal(cb, rb.coords())
+
+
+@pytest.mark.parametrize("n,lds", [(257, False), (1000, False), (6000, True)])
+def test_forces_of_the_global_gather_step_kernel(n, lds):
+    """The forces-only pass of the global-gather step kernel (jobs above
+    5,120 beads, or the LDS path switched off) equals the oracle's FP32
+    forces bit for bit."""
+    sysm = S.synth_chain(n, seed=n)
+    kw = c0_params(pairs_threshold=32.0)
+    with S.Engine(0, 2 << 30) as e:
+        e.set_lds_step(lds)
+        jid = e.add_job(S.make_params(**kw), sysm, 0)
+        e.rebuild(jid, 3)
+        f = e.forces(jid)
+    ref = O.Sim(O.make_params(**kw), sysm)
+    ref.build_possible()
+    ref.build_verlet()
+    np.testing.assert_array_equal(f, ref.forces32())
+
+
+def test_heating_schedule_bit_exact():
+    """The heating schedule (src/Integrators/langevin.cu:68-82: T raised by
+    deltaT every tempFreq steps) run on the device in the job updater:
+    coordinates, RNG state and temperature accumulator equal the oracle's,
+    alone and batched with a job that does not heat."""
+    n = 300
+    sysm = _pull_system(n, 14)
+    kw = c0_params(heating=1, initialT=0.1, deltaT=0.12, tempFreq=60, nav=30, outputtiming=30, dcdfreq=30,
+                   pulling=1, fixedEnd=0, pulledEnd=n - 1, deltax=0.02, pullFreq=30, seed=4343)
+    other = _pull_system(200, 15)
+    ko = c0_params(pulling=1, fixedEnd=0, pulledEnd=199, deltax=0.01, pullFreq=100, seed=4344)
+    ref = O.Sim(O.make_params(**kw), sysm)
+    ref.run(590)
+    with S.Engine(0, 1 << 30) as e:
+        a = e.add_job(S.make_params(**kw), sysm, 590)
+        e.run_job(a)
+        np.testing.assert_array_equal(e.coords(a), ref.coords())
+        np.testing.assert_array_equal(e.rng(a), ref.rng())
+        np.testing.assert_array_equal(e.temperature(a), ref.temperature())
+    with S.Engine(0, 1 << 30) as e:
+        a = e.add_job(S.make_params(**kw), sysm, 590)
+        b = e.add_job(S.make_params(**ko), other, 590)
+        L = e.block_steps()
+        e.run_blocks(600 // L)
+        assert e.status(a)[0] == 600
+        np.testing.assert_array_equal(e.coords(a), ref.coords())

@@ -49,3 +49,28 @@ def test_server_results_equal_alone():
         ref = O.Sim(O.make_params(**kw), sysm)
         ref.run(steps)
         np.testing.assert_array_equal(got, ref.coords())
+
+
+def test_server_admits_a_larger_job_after_a_smaller_one_retires():
+    """Budget below 1 GiB (each arena block gets a chunk of its own size):
+    the larger request fits only once the smaller one has retired and its
+    chunks are returned, and both finish with the oracle's bits."""
+    small, large = numpy_chain(100, seed=95), numpy_chain(400, seed=96)
+    ks = c0_params(seed=951)
+    kl = c0_params(seed=961)
+    bs = S.job_bytes(S.make_params(**ks), small)
+    bl = S.job_bytes(S.make_params(**kl), large)
+    assert bl > bs
+    with S.Server(0, bl + bs // 2) as srv:
+        t1 = srv.submit(S.make_params(**ks), small, 190)
+        t2 = srv.submit(S.make_params(**kl), large, 190)
+        for _ in range(2000):
+            running, queued = srv.pump(5)
+            if running == 0 and queued == 0:
+                break
+        assert srv.poll(t1)[0] == 2 and srv.poll(t2)[0] == 2, (srv.poll(t1), srv.poll(t2))
+        got = [srv.result(t1), srv.result(t2)]
+    for (sysm, kw), c in zip(((small, ks), (large, kl)), got):
+        r = O.Sim(O.make_params(**kw), sysm)
+        r.run(190)
+        np.testing.assert_array_equal(c, r.coords())
