@@ -59,7 +59,8 @@ struct or_sim {
     float hOverZeta, tempNorm, var;
     /* pulling */
     int nfixed, npulled; int *fixed, *pulled;
-    float pullVec[3], cant0[3], cant[3], extForce[3];
+    /* per trajectory, as cantCoord0/cantCoord/pullVector/extForce[traj] (pulling.cu:66-85) */
+    float (*pullVec)[3], (*cant0)[3], (*cant)[3], (*extForce)[3];
     int pullingUpdater;
     /* indentation */
     float tip[3], chip[3], chip0[3], dir[3], micaR[3], micaR0[3], micaN[3];
@@ -384,22 +385,27 @@ or_sim* or_create(const or_params* p, int N, int ntr, const float* coords4,
         s->nfixed = nfixed; s->npulled = npulled;
         s->fixed = (int*)xcalloc(nfixed, sizeof(int)); s->pulled = (int*)xcalloc(npulled, sizeof(int));
         memcpy(s->fixed, fixed, nfixed * sizeof(int)); memcpy(s->pulled, pulled, npulled * sizeof(int));
-        /* one pulling geometry (ntr identical copies share the topology;
-         * each trajectory has its own vector in the reference, the oracle
-         * keeps trajectory 0's for the ntr=1 case it is used for). */
-        const f4* c = s->coord;
-        if (p->pullDirectionVector) {
-            s->pullVec[0] = p->pullVector[0]; s->pullVec[1] = p->pullVector[1]; s->pullVec[2] = p->pullVector[2];
-        } else {
-            float vx = c[p->pulledEnd].x - c[p->fixedEnd].x;
-            float vy = c[p->pulledEnd].y - c[p->fixedEnd].y;
-            float vz = c[p->pulledEnd].z - c[p->fixedEnd].z;
-            float v = sqrtf(vx * vx + vy * vy + vz * vz);
-            s->pullVec[0] = vx / v; s->pullVec[1] = vy / v; s->pullVec[2] = vz / v;
+        /* each trajectory's pulling geometry from its own coordinates
+         * (pulling.cu:62-92) */
+        s->pullVec = (float(*)[3])xcalloc(ntr, sizeof(float[3]));
+        s->cant0 = (float(*)[3])xcalloc(ntr, sizeof(float[3]));
+        s->cant = (float(*)[3])xcalloc(ntr, sizeof(float[3]));
+        s->extForce = (float(*)[3])xcalloc(ntr, sizeof(float[3]));
+        for (int t = 0; t < ntr; t++) {
+            const f4* c = s->coord + (size_t)t * N;
+            if (p->pullDirectionVector) {
+                s->pullVec[t][0] = p->pullVector[0]; s->pullVec[t][1] = p->pullVector[1]; s->pullVec[t][2] = p->pullVector[2];
+            } else {
+                float vx = c[p->pulledEnd].x - c[p->fixedEnd].x;
+                float vy = c[p->pulledEnd].y - c[p->fixedEnd].y;
+                float vz = c[p->pulledEnd].z - c[p->fixedEnd].z;
+                float v = sqrtf(vx * vx + vy * vy + vz * vz);
+                s->pullVec[t][0] = vx / v; s->pullVec[t][1] = vy / v; s->pullVec[t][2] = vz / v;
+            }
+            s->cant0[t][0] = s->cant[t][0] = c[p->pulledEnd].x;
+            s->cant0[t][1] = s->cant[t][1] = c[p->pulledEnd].y;
+            s->cant0[t][2] = s->cant[t][2] = c[p->pulledEnd].z;
         }
-        s->cant0[0] = s->cant[0] = c[p->pulledEnd].x;
-        s->cant0[1] = s->cant[1] = c[p->pulledEnd].y;
-        s->cant0[2] = s->cant[2] = c[p->pulledEnd].z;
         for (int t = 0; t < ntr; t++) {
             for (int k = 0; k < npulled; k++) {
                 int i = pulled[k];
@@ -409,7 +415,7 @@ or_sim* or_create(const or_params* p, int N, int ntr, const float* coords4,
                     return NULL;
                 }
                 f4* e = &s->ext[t * N + i];
-                e->x = s->pullVec[0] * p->fconst; e->y = s->pullVec[1] * p->fconst; e->z = s->pullVec[2] * p->fconst; e->w = 2.0f;
+                e->x = s->pullVec[t][0] * p->fconst; e->y = s->pullVec[t][1] * p->fconst; e->z = s->pullVec[t][2] * p->fconst; e->w = 2.0f;
             }
             for (int k = 0; k < nfixed; k++) {
                 int i = fixed[k];
@@ -493,6 +499,7 @@ void or_destroy(or_sim* s) {
     free(s->pair_cnt); free(s->pair_j); free(s->pp_cnt); free(s->pp_j);
     free(s->ni); free(s->nj); free(s->nr0); free(s->fixed); free(s->pulled); free(s->dat);
     free(s->prow); free(s->irow);
+    free(s->pullVec); free(s->cant0); free(s->cant); free(s->extForce);
     free(s->rforce); free(s->mforce); free(s->mforce0); free(s->hcoord); free(s->ci); free(s->eps); free(s->beta);
     free(s->eps_traj); free(s->L);
     free(s);
@@ -1265,29 +1272,31 @@ static void pulling_update(or_sim* s) {
     float xt = s->p.deltax * (float)(s->step / s->p.pullFreq);
     for (int t = 0; t < s->ntr; t++) {
         f4 cn = s->coord[N * t + s->p.pulledEnd];
-        s->extForce[0] = s->p.k_trans * (s->cant[0] - cn.x);
-        s->extForce[1] = s->p.k_trans * (s->cant[1] - cn.y);
-        s->extForce[2] = s->p.k_trans * (s->cant[2] - cn.z);
-        s->cant[0] = s->cant0[0] + xt * s->pullVec[0];
-        s->cant[1] = s->cant0[1] + xt * s->pullVec[1];
-        s->cant[2] = s->cant0[2] + xt * s->pullVec[2];
+        float* const F = s->extForce[t];
+        float* const cant = s->cant[t];
+        const float* const pv = s->pullVec[t];
+        F[0] = s->p.k_trans * (cant[0] - cn.x);
+        F[1] = s->p.k_trans * (cant[1] - cn.y);
+        F[2] = s->p.k_trans * (cant[2] - cn.z);
+        cant[0] = s->cant0[t][0] + xt * pv[0];
+        cant[1] = s->cant0[t][1] + xt * pv[1];
+        cant[2] = s->cant0[t][2] + xt * pv[2];
         for (int k = 0; k < s->npulled; k++) {
             f4* e = &s->ext[t * N + s->pulled[k]];
-            e->x = s->extForce[0]; e->y = s->extForce[1]; e->z = s->extForce[2]; e->w = 2.0f;
+            e->x = F[0]; e->y = F[1]; e->z = F[2]; e->w = 2.0f;
         }
         /* savePullingData (pulling.cu:166-187): the row of the pass's
          * coordinates and force, every pullFreq steps */
         if (s->step % s->p.pullFreq == 0) {
             const f4 a = s->coord[N * t + s->p.pulledEnd], b = s->coord[N * t + s->p.fixedEnd];
             const float vx = a.x - b.x, vy = a.y - b.y, vz = a.z - b.z;
-            const float* pv = s->pullVec;
             double* r = push_row(&s->prow, &s->nprow, &s->capprow, 8);
             r[0] = t;
             r[1] = (double)s->step;
             r[2] = sqrtf(vx * vx + vy * vy + vz * vz);
             r[3] = vx * pv[0] + vy * pv[1] + vz * pv[2];
-            r[4] = s->extForce[0] * pv[0] + s->extForce[1] * pv[1] + s->extForce[2] * pv[2];
-            r[5] = s->extForce[0]; r[6] = s->extForce[1]; r[7] = s->extForce[2];
+            r[4] = F[0] * pv[0] + F[1] * pv[1] + F[2] * pv[2];
+            r[5] = F[0]; r[6] = F[1]; r[7] = F[2];
         }
     }
 }
@@ -1452,7 +1461,8 @@ long long or_run_f64(or_sim* s, long long numsteps, double* cout) {
     long long done = 0;
     /* the lists are built from float images of the double coordinates */
 #define SYNC_F32() do { for (int i_ = 0; i_ < n; i_++) { s->coord[i_].x = (float)x[3*i_]; s->coord[i_].y = (float)x[3*i_+1]; s->coord[i_].z = (float)x[3*i_+2]; } } while (0)
-    double cant[3] = {s->cant[0], s->cant[1], s->cant[2]};
+    double cant[3] = {0.0, 0.0, 0.0};
+    if (s->cant) for (int d = 0; d < 3; d++) cant[d] = s->cant[0][d];
     double ext[3] = {0, 0, 0};
     int have_ext = 0;
     if (s->step % p->pairs_freq == 0) or_build_verlet(s);
@@ -1463,7 +1473,7 @@ long long or_run_f64(or_sim* s, long long numsteps, double* cout) {
             ext[1] = p->k_trans * (cant[1] - x[3 * e + 1]);
             ext[2] = p->k_trans * (cant[2] - x[3 * e + 2]);
             double xt = (double)p->deltax * (double)(s->step / p->pullFreq);
-            for (int d = 0; d < 3; d++) cant[d] = s->cant0[d] + xt * s->pullVec[d];
+            for (int d = 0; d < 3; d++) cant[d] = s->cant0[0][d] + xt * s->pullVec[0][d];
             have_ext = 1;
         }
         SYNC_F32();

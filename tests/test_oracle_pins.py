@@ -45,7 +45,12 @@ def _rel(a, b):
 # ---------------------------------------------------------------------------
 # force-ramp cantilever
 # ---------------------------------------------------------------------------
-def test_cantilever_force_uses_the_previous_cantilever_position():
+@pytest.mark.parametrize("ntr", [1, 2])
+def test_cantilever_force_uses_the_previous_cantilever_position(ntr):
+    """Each trajectory pulls with its own cantilever (cantCoord[traj],
+    pulling.cu:66-85,152-165): in a many-runs set the second trajectory's
+    force still uses its own previous position, not one the first
+    trajectory's pass has just advanced."""
     n = 40
     sysm = numpy_chain(n, seed=3)
     sysm.fixed = np.array([0], np.int32)
@@ -53,7 +58,7 @@ def test_cantilever_force_uses_the_previous_cantilever_position():
     k_trans, deltax, freq = 0.05, 0.5, 20
     kw = c0_params(temperature=0.0, pulling=1, fixedEnd=0, pulledEnd=n - 1, deltax=deltax, pullFreq=freq,
                    k_trans=k_trans)
-    s = O.Sim(O.make_params(**kw), sysm)
+    s = O.Sim(O.make_params(**kw), sysm, ntr=ntr)
     stride = s.stride()
     assert stride == 10  # passes every 10 steps, the ramp every 20
     x0 = sysm.coords[:, :3].astype(np.float32)
@@ -65,17 +70,22 @@ def test_cantilever_force_uses_the_previous_cantilever_position():
     for blk in range(8):
         step = blk * stride
         xc = s.coords()[:, :3].astype(np.float64)
-        xp = xc[n - 1]  # x_pulled at the pass
         s.run(step)  # one block: the pass at `step`, then `stride` steps
-        f = s.ext()[n - 1]
-        assert f[3] == 2.0
-        expect = k_trans * (cant - xp)
-        np.testing.assert_allclose(f[:3], expect, rtol=2e-6, atol=1e-7)
-        if step % freq == 0:  # a pull .dat row: end-to-end, its projection, F . n, F
-            e2e = xp - xc[0]
-            rows.append([0, step, np.linalg.norm(e2e), e2e @ nhat, expect @ nhat, *expect])
+        ext = s.ext()
+        for t in range(ntr):
+            xp = xc[t * n + n - 1]  # x_pulled of trajectory t at the pass
+            f = ext[t * n + n - 1]
+            assert f[3] == 2.0
+            expect = k_trans * (cant - xp)
+            np.testing.assert_allclose(f[:3], expect, rtol=2e-6, atol=1e-7, err_msg="trajectory %d" % t)
+            if step % freq == 0:  # a pull .dat row: end-to-end, its projection, F . n, F
+                e2e = xp - xc[t * n]
+                rows.append([t, step, np.linalg.norm(e2e), e2e @ nhat, expect @ nhat, *expect])
         cant = cant0 + deltax * (step // freq) * nhat
-    np.testing.assert_allclose(s.pull_rows(), np.array(rows), rtol=1e-5, atol=1e-6)
+    got = s.pull_rows()
+    order = np.lexsort((got[:, 1], got[:, 0]))
+    want = np.array(rows)
+    np.testing.assert_allclose(got[order], want[np.lexsort((want[:, 1], want[:, 0]))], rtol=1e-5, atol=1e-6)
     # the ramp has moved the cantilever 3 deltax by step 70
     assert np.linalg.norm(cant - cant0) == pytest.approx(3 * deltax, rel=1e-6)
 
