@@ -242,6 +242,22 @@ int sop_engine_get_hi_factor(sop_engine* e, int job_id, float* L);
  * system integrateCholesky_decompose (bdhitea_kernel.cu:154-190), which is
  * limited to 3N <= 384. */
 int sop_chol_factor(int device, int n, const float* A, int mode, float* L, int repeats, double* ms_per_factor);
+/* The same factorisation with nothing allocated.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * M is device memory the caller owns: an n x n SPD matrix, row-major with
+ * leading dimension ld floats (the lower triangle is read), with room for mp
+ * rows, mp = n rounded up to a multiple of 64, and ld >= mp.  On return (in
+ * `stream` order; the call does not wait) its lower triangle holds L, equal
+ * to sop_chol_factor's bit for bit; the upper triangle is left as it was and
+ * rows n..mp-1 hold the identity padding.  stream is a hipStream_t (NULL:
+ * the null stream); `device` becomes the calling thread's current device.
+ * The workspace is one static device array per GPU (about 25 MB, held until
+ * the process ends), enough for n <= 12288; a larger n is refused with
+ * SOP_E_ARG.  Calls on one GPU share that array and the last call's contents
+ * stay in it: calls that overlap in time on one GPU race, so issue them on
+ * one stream or synchronise between them. */
+int sop_chol_factor_noalloc(int device, int n, float* M, unsigned long long ld, int mode, void* stream);
 
 /* Streaming copy of `bytes` (read once, written once) repeated `repeats`
  * times: achieved HBM GB/s (read + write), the reference point for the

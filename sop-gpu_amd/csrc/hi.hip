@@ -29,9 +29,13 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
+#include <string>
 
+#include "../../include/sopgpu.h"
 #include "devmath.h"
 #include "hi.h"
+#include "hostutil.h"
 #include "jobdesc.h"
 
 namespace sop {
@@ -1202,7 +1206,7 @@ static void trailing_update(float* M, size_t ld, int rs, int cs, int ce, int mp,
 // main stream (T1) and the rest of the trailing matrix on `aux` (T2), so
 // panel p+1 is factored while T2(p) runs.  T1(p+1) and T2(p) both write
 // panel p+2's columns, so the main stream waits for T2(p) before T1(p+1).
-// With aux == nullptr everything runs on `stream`.
+// With aux == nullptr the same updates run in order on `stream`.
 hipError_t launch_chol_factor(float* M, size_t ld, int mp, int mode, double* linv, unsigned short* phi,
                               unsigned short* plo, const int* flag, hipStream_t stream, hipStream_t aux,
                               hipEvent_t ev_f, hipEvent_t ev_t2) {
@@ -1241,8 +1245,9 @@ hipError_t launch_chol_factor(float* M, size_t ld, int mp, int mode, double* lin
             trailing_update(M, ld, s1, s1, mp, mp, c0, kw, mode, ph, pl, aux);
             if ((e = hipEventRecord(ev_t2, aux)) != hipSuccess) return e;
             t2_pending = true;
-        } else {
-            trailing_update(M, ld, s0, s0, mp, mp, c0, kw, mode, ph, pl, stream);
+        } else {  // the same two updates in stream order
+            trailing_update(M, ld, s0, s0, s1, mp, c0, kw, mode, ph, pl, stream, true);
+            trailing_update(M, ld, s1, s1, mp, mp, c0, kw, mode, ph, pl, stream);
         }
     }
     if (t2_pending) {
@@ -1268,3 +1273,85 @@ hipError_t launch_chol_step(const DevJob* jobs, const DevJobState* st, const int
 }
 
 }  // namespace sop
+
+// ---------------------------------------------------------------------------
+// Factorisation on caller-owned device memory with nothing allocated
+// (sop_chol_factor_noalloc): the workspace of launch_chol_factor -- the FP64
+// inverses of a panel's diagonal blocks and the two split-BF16 panel
+// buffers -- sized for matrices up to kNoAllocMaxOrder, lives in one static
+// device array that every GPU the library runs on holds until the process
+// ends.  Its device address is taken once per device with
+// hipGetSymbolAddress; the array's name is never used as an address on the
+// host.
+// ---------------------------------------------------------------------------
+namespace sop {
+
+constexpr int kNoAllocMaxOrder = 12288;  // 3N for N = 4,096 beads
+constexpr size_t kNoAllocPanelBytes = (size_t)2 * kNoAllocMaxOrder * kCholPanel * sizeof(unsigned short);
+constexpr size_t kNoAllocScratchBytes = kCholLinvBytes + 2 * kNoAllocPanelBytes;  // linv | phi | plo
+static_assert(kCholLinvBytes % 256 == 0 && kNoAllocPanelBytes % 256 == 0, "scratch sections stay 256-byte aligned");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on `device` (the current device).
+hipError_t scratch_on(int device, unsigned char** out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch_ptr[device] = static_cast<unsigned char*>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
+
+// Rows n..mp-1 of M become the identity padding: zero left of the
+// diagonal, one on it (the allocating entry point pads the same way).
+__global__ __launch_bounds__(256) void k_chol_pad(float* __restrict__ M, size_t ld, int n, int mp) {
+    const int r = n + (int)blockIdx.y;
+    const int c = (int)(blockIdx.x * 256 + threadIdx.x);
+    if (r >= mp || c > r) return;
+    M[(size_t)r * ld + c] = c == r ? 1.0f : 0.0f;
+}
+
+}  // namespace
+
+}  // namespace sop
+
+extern "C" int sop_chol_factor_noalloc(int device, int n, float* M, unsigned long long ld, int mode, void* stream) {
+    using namespace sop;
+    if (n <= 0 || !M) return fail(SOP_E_ARG, "cholesky (no allocation): bad arguments");
+    if (n > kNoAllocMaxOrder)
+        return fail(SOP_E_ARG, "cholesky (no allocation): order " + std::to_string(n) + " exceeds the maximum " +
+                                   std::to_string(kNoAllocMaxOrder));
+    if (mode != SOP_CHOL_BF16X3 && mode != SOP_CHOL_F32) return fail(SOP_E_ARG, "cholesky (no allocation): unknown mode");
+    const int mp = (n + 63) / 64 * 64;
+    if (ld < (unsigned long long)mp)
+        return fail(SOP_E_ARG, "cholesky (no allocation): leading dimension " + std::to_string(ld) +
+                                   " is below the padded order " + std::to_string(mp));
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(SOP_E_HIP, "no HIP device available");
+    if (device < 0 || device >= ndev || device >= kMaxDevices) return fail(SOP_E_ARG, "device index out of range");
+    if (hipSetDevice(device) != hipSuccess) return fail(SOP_E_HIP, "hipSetDevice failed");
+    unsigned char* base = nullptr;
+    hipError_t e = scratch_on(device, &base);
+    if (e != hipSuccess)
+        return fail(SOP_E_HIP, std::string("cholesky (no allocation): scratch address: ") + hipGetErrorString(e));
+    double* linv = reinterpret_cast<double*>(base);
+    unsigned short* phi = reinterpret_cast<unsigned short*>(base + kCholLinvBytes);
+    unsigned short* plo = reinterpret_cast<unsigned short*>(base + kCholLinvBytes + kNoAllocPanelBytes);
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    if (mp > n) hipLaunchKernelGGL(k_chol_pad, dim3((mp + 255) / 256, mp - n), dim3(256), 0, s, M, (size_t)ld, n, mp);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = launch_chol_factor(M, (size_t)ld, mp, mode, linv, phi, plo, nullptr, s);
+    if (e != hipSuccess) return fail(SOP_E_HIP, std::string("cholesky (no allocation): ") + hipGetErrorString(e));
+    return 0;
+}

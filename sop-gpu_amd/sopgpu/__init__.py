@@ -239,6 +239,7 @@ _sig("sop_engine_set_chol_mode", C.c_int, VP, C.c_int)
 _sig("sop_engine_get_hi", C.c_int, VP, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_float))
 _sig("sop_engine_get_hi_factor", C.c_int, VP, C.c_int, VP)
 _sig("sop_chol_factor", C.c_int, C.c_int, C.c_int, VP, C.c_int, VP, C.c_int, C.POINTER(C.c_double))
+_sig("sop_chol_factor_noalloc", C.c_int, C.c_int, C.c_int, VP, C.c_ulonglong, C.c_int, VP)
 _sig("sop_server_create", C.c_int, C.c_int, C.c_ulonglong, C.POINTER(VP))
 _sig("sop_server_destroy", None, VP)
 _sig("sop_server_set_tuning", C.c_int, VP, C.c_int, C.c_int)
@@ -298,7 +299,7 @@ EXPORTED = [
     "sop_engine_get_list", "sop_engine_get_indentation", "sop_engine_compute_forces",
     "sop_engine_compute_energies", "sop_engine_rebuild_lists", "sop_engine_profile", "sop_engine_profile_read",
     "sop_engine_list_stats", "sop_engine_set_chol_mode", "sop_engine_get_hi", "sop_engine_get_hi_factor",
-    "sop_chol_factor", "sop_server_create", "sop_server_destroy", "sop_server_set_tuning", "sop_server_set_shape",
+    "sop_chol_factor", "sop_chol_factor_noalloc", "sop_server_create", "sop_server_destroy", "sop_server_set_tuning", "sop_server_set_shape",
     "sop_server_autotune", "sop_server_submit",
     "sop_server_pump", "sop_server_poll", "sop_server_result", "sop_server_stats", "sop_copy_bandwidth",
     "sop_generate_seeds", "sop_rng_probe", "sop_check_division", "sop_synth_chain", "sop_synth_shell",
