@@ -145,6 +145,7 @@ def serving_phase(S, args, capacity):
     tickets = [None] * len(mix)
     nxt = 0
     t0 = time.perf_counter()
+    last_note = 0.0
     while True:
         now = time.perf_counter() - t0
         while nxt < len(mix) and arrivals[nxt] <= now:
@@ -152,6 +153,10 @@ def serving_phase(S, args, capacity):
             tickets[nxt] = srv.submit(p, sysm, steps)
             nxt += 1
         running, queued = srv.pump(1)
+        if now - last_note >= 15.0:
+            last_note = now
+            print(f"# serving: {now:.0f} s, {nxt} submitted, {running} running, {queued} queued",
+                  file=sys.stderr, flush=True)
         if nxt == len(mix) and running == 0 and queued == 0:
             break
         if running == 0 and queued == 0 and nxt < len(mix):
@@ -416,10 +421,13 @@ def c0_phase(S, c0, steps):
     for name, graph in (("plain", None), ("batched_graph", True)):
         eng = S.Engine(0, 4 << 30)
         jid = eng.add_job(p, sysm, steps - 10)
+        tune = eng.autotune()  # launch shapes for this one job; it does not advance
+        out["tuning"] = tune
+        if graph:
+            eng.set_tuning(tune["shape"]["max_blocks"], True)
         eng.synchronize()
         t0 = time.perf_counter()
         if graph:
-            eng.set_tuning(0, True)
             eng.run_blocks(steps // eng.block_steps())
         else:
             eng.run_job(jid)
@@ -508,8 +516,11 @@ def main():
     # launched between them)
     eng.profile(True)
     prof_steps = whole_periods(min(steps, 2000))
+    eng.synchronize()
+    t_eager = time.perf_counter()
     eng.run_blocks(prof_steps // L)
     eng.synchronize()
+    t_eager = time.perf_counter() - t_eager
     launches, ms = eng.profile_read()
     eng.profile(False)
     avg_ms = ms / max(launches, 1)
@@ -613,6 +624,7 @@ def main():
                                        "x beads of this run)",
                      "kernel": kernel, "avg_launch_ms": avg_ms, "launches": launches,
                      "timing": f"HIP events around each block's step launches, {prof_steps} eager steps",
+                     "eager_ms_per_step": t_eager / prof_steps * 1e3,
                      "bytes_per_bead_step": bpb,
                      "copy_kernel_gbps": copy_gbps, "frac_of_copy": achieved / (copy_gbps * 1e9),
                      "avg_lists": {"covalent": avg_lists[0], "native": avg_lists[1], "verlet": avg_lists[2],

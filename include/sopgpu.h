@@ -169,7 +169,7 @@ int sop_engine_set_lds_step(sop_engine* e, int enabled);
 typedef struct sop_tuning {
     int max_blocks;   /* persistent grid of the global-gather step kernel (jobs above 5120 beads; 0: one
                          workgroup per 4 tiles) */
-    int unit_rows;    /* Verlet rows per LDS step unit, the threads of its workgroup: 512, 768 or 1024 */
+    int unit_rows;    /* Verlet rows per LDS step unit, the threads of its workgroup: a multiple of 64, 64..1024 */
     int shared_units; /* rows left over by LDS jobs packed together into shared units (1) or one job per unit (0) */
     int pair_batch;   /* Verlet entries fetched ahead per lane in the LDS step: 4, 6 or 8 */
     int verlet_chunk; /* 64-entry chunks of a possible-list row in flight in the LDS Verlet build: 4 or 8 */
@@ -237,8 +237,9 @@ int sop_engine_get_hi(sop_engine* e, int job_id, double* epsilon, float* beta);
 /* the job's current Cholesky factor, 3n x 3n row-major, upper part zero */
 int sop_engine_get_hi_factor(sop_engine* e, int job_id, float* L);
 /* Standalone factorisation of an n x n SPD matrix (row-major; the lower
- * triangle is read) into L (n x n, upper zero); repeats timed runs with
- * HIP events, average in *ms_per_factor.  Replaces the one-block-per-
+ * triangle is read) into L (n x n, upper zero); with repeats > 0, one
+ * untimed run and then `repeats` runs timed with HIP events, average in
+ * *ms_per_factor.  Replaces the one-block-per-
  * system integrateCholesky_decompose (bdhitea_kernel.cu:154-190), which is
  * limited to 3N <= 384. */
 int sop_chol_factor(int device, int n, const float* A, int mode, float* L, int repeats, double* ms_per_factor);
@@ -247,12 +248,13 @@ int sop_chol_factor(int device, int n, const float* A, int mode, float* L, int r
  * graph capture, tight serving loops, harnesses that forbid hipMalloc).
  * M is device memory the caller owns: an n x n SPD matrix, row-major with
  * leading dimension ld floats (the lower triangle is read), with room for mp
- * rows, mp = n rounded up to a multiple of 64, and ld >= mp.  On return (in
+ * rows, mp = n rounded up to a multiple of 64, ld >= mp, ld a multiple of 4
+ * and M 16-byte aligned.  On return (in
  * `stream` order; the call does not wait) its lower triangle holds L, equal
  * to sop_chol_factor's bit for bit; the upper triangle is left as it was and
  * rows n..mp-1 hold the identity padding.  stream is a hipStream_t (NULL:
  * the null stream); `device` becomes the calling thread's current device.
- * The workspace is one static device array per GPU (about 25 MB, held until
+ * The workspace is one static device array per GPU (about 57 MB, held until
  * the process ends), enough for n <= 12288; a larger n is refused with
  * SOP_E_ARG.  Calls on one GPU share that array and the last call's contents
  * stay in it: calls that overlap in time on one GPU race, so issue them on

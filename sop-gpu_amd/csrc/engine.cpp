@@ -182,7 +182,8 @@ Engine::~Engine() {
         (void)hipStreamDestroy(aux_stream_);
     }
     if (ev_f_) (void)hipEventDestroy(ev_f_);
-    if (ev_t2_) (void)hipEventDestroy(ev_t2_);
+    for (hipEvent_t ev : ev_t2_)
+        if (ev) (void)hipEventDestroy(ev);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -513,7 +514,8 @@ static unsigned long long built_bytes(const BuiltJob& b, bool pulling, bool inde
         t += 3 * round_up(n * 16) + round_up(n * 4);  // rforce, mforce, ci, eps
         if (b.dj.flags & JF_HI_EXACT) {
             const unsigned long long mp = (3 * n + 63) / 64 * 64;
-            t += round_up(n * 16) + round_up(mp * mp * 4) + round_up(kCholLinvBytes) + 2 * round_up(2 * mp * kCholPanel * 2);
+            t += round_up(n * 16) + round_up(mp * mp * 4) + round_up(kCholLinvBytes) + round_up(chol_bbuf_bytes(mp)) +
+                 2 * round_up(chol_panel_bytes(mp));
         }
     }
     return t;
@@ -608,19 +610,22 @@ int Engine::add_job(const sop_params& p, const sop_system& s, long long numsteps
             hw.ld = mp;
             hw.mp = (int)mp;
             hw.mode = exact_mode_;
-            hw.linv = (double*)A(kCholLinvBytes);
-            hw.phi = (unsigned short*)A(2 * mp * kCholPanel * 2);
-            hw.plo = (unsigned short*)A(2 * mp * kCholPanel * 2);
+            hw.work.linv = (double*)A(kCholLinvBytes);
+            hw.work.bbuf = (float*)A(chol_bbuf_bytes(mp));
+            hw.work.phi = (unsigned short*)A(chol_panel_bytes(mp));
+            hw.work.plo = (unsigned short*)A(chol_panel_bytes(mp));
             hw.chol_freq = J.hi_chol_freq;
             if (!aux_stream_) {  // the factorisation's lookahead stream, shared by the exact-HI jobs
                 if (hipStreamCreateWithFlags(&aux_stream_, hipStreamNonBlocking) != hipSuccess ||
                     hipEventCreateWithFlags(&ev_f_, hipEventDisableTiming) != hipSuccess ||
-                    hipEventCreateWithFlags(&ev_t2_, hipEventDisableTiming) != hipSuccess)
+                    hipEventCreateWithFlags(&ev_t2_[0], hipEventDisableTiming) != hipSuccess ||
+                    hipEventCreateWithFlags(&ev_t2_[1], hipEventDisableTiming) != hipSuccess)
                     oom = true;
             }
             hw.aux = aux_stream_;
             hw.ev_f = ev_f_;
-            hw.ev_t2 = ev_t2_;
+            hw.ev_t2[0] = ev_t2_[0];
+            hw.ev_t2[1] = ev_t2_[1];
         }
     }
     if (oom) {
@@ -932,6 +937,12 @@ int Engine::plan_lds_units(std::vector<int2>& units, std::vector<int4>& segs) co
 }
 
 void Engine::drop_graph() {
+    bool any = false;
+    for (auto g : graphs_) any = any || g != nullptr;
+    if (!any) return;
+    // a replay of these graphs may still run on the stream: let it finish
+    // before the executable graphs are destroyed
+    (void)hipStreamSynchronize(stream_);
     for (auto& g : graphs_)
         if (g) {
             (void)hipGraphExecDestroy(g);
@@ -1555,7 +1566,7 @@ Tuning Engine::tuning() const {
 }
 
 int Engine::set_shape(const Tuning& t) {
-    const bool ok = t.max_blocks >= 0 && (t.unit_rows == 512 || t.unit_rows == 768 || t.unit_rows == 1024) &&
+    const bool ok = t.max_blocks >= 0 && t.unit_rows >= 64 && t.unit_rows <= 1024 && t.unit_rows % 64 == 0 &&
                     (t.pair_batch == 4 || t.pair_batch == 6 || t.pair_batch == 8) &&
                     (t.verlet_chunk == 4 || t.verlet_chunk == 8) && t.sort_every >= 1 && t.sort_every <= 1000;
     if (!ok) return fail(SOP_E_ARG, "launch shape out of range");
@@ -1699,7 +1710,8 @@ int Engine::autotune(Tuning* best_out, double* ms_default, double* ms_best, int*
         }
     };
     if (lds) {
-        for (int v : {768, 512})
+        // smaller units spread a few small jobs over more CUs
+        for (int v : {1024, 768, 512, 256, 128})
             if (v != best.unit_rows) { Tuning t = best; t.unit_rows = v; consider(t); }
         { Tuning t = best; t.shared_units = !best.shared_units; consider(t); }
         for (int v : {4, 8})

@@ -162,8 +162,8 @@ class Engine {
 
     int device_;
     hipStream_t stream_ = nullptr;
-    hipStream_t aux_stream_ = nullptr;  // lookahead stream of the exact-HI factorisation
-    hipEvent_t ev_f_ = nullptr, ev_t2_ = nullptr;
+    hipStream_t aux_stream_ = nullptr;  // trailing-update stream of the exact-HI factorisation
+    hipEvent_t ev_f_ = nullptr, ev_t2_[2] = {nullptr, nullptr};
     Arena arena_;
     std::vector<HostJob> jobs_;
     int next_id_ = 1;

@@ -14,18 +14,21 @@
 //
 // The exact path factorises the RPY matrix (3N x 3N, padded to a multiple
 // of 64 with an identity block) by a blocked Cholesky over outer panels of
-// kCholPanel columns:
-//   the panel's diagonal region, left-looking over its 64x64 blocks in one
-//   workgroup: each diagonal block factored in FP64 in LDS, with its
-//   inverse (FP64), and the blocks below it in the region solved against
-//   it in FP64;
-//   the panel's rows below the region, 64 per workgroup, solved the same
-//   way with the inverses (X = (A - X' L'^T) Li^T, FP64 accumulation),
-//   also emitted as split-BF16 pairs (hi = bf16(x), lo = bf16(x - hi));
-//   the trailing update C -= P_I P_J^T on MFMA, either split-BF16 with three
-//   products (hi*hi + hi*lo + lo*hi, FP32 accumulate) or FP32 MFMA, the
-//   next panel's columns first and the rest on a second stream, overlapped
-//   with the next panel's factorisation.
+// kCholPanel columns, each factored by 64-column steps (k_chol_step), one
+// launch per step over the 64-row blocks from the step's diagonal down:
+//   the previous step's column block is solved against the inverse of its
+//   diagonal block (X = B Li^T on FP64 MFMA) and emitted in FP32 and as
+//   split-BF16 pairs (hi = bf16(x), lo = bf16(x - hi));
+//   the step's column block is updated by the columns to its left that no
+//   trailing update has applied yet (the previous panel's and this
+//   panel's), on MFMA -- split-BF16 with three products (hi*hi + hi*lo +
+//   lo*hi, FP32 accumulate) or FP32 -- with the K range split over the
+//   workgroup's four wavefronts;
+//   the diagonal workgroup factors its block in FP64 (16-column passes in
+//   registers, rank-16 updates and the inverse's off-diagonal blocks on FP64
+//   MFMA) and forms the inverse the next step solves with.
+// The rest of the trailing matrix is updated once per outer panel by wide
+// MFMA kernels on a second stream, overlapped with the next panel's steps.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -42,6 +45,7 @@ namespace sop {
 
 typedef short bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 struct F6 {
     float xx, xy, xz, yy, yz, zz;
@@ -504,35 +508,32 @@ __device__ __forceinline__ double readlane_d(double v, int lane) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// 1/sqrt(p) in FP64: the hardware estimate refined by two Newton steps
-// (y <- y (3 - p y^2) / 2), which leaves it within a few ulp of FP64.
+// 1/sqrt(p) in FP64: the hardware estimate (relative error below 2^-22)
+// refined by one Newton step (y <- y (3 - p y^2) / 2), which leaves a
+// relative error near 2^-44 -- far below the FP32 rounding of the factor
+// it feeds, while the pivot chain of the diagonal factorisation stays short.
 __device__ __forceinline__ double rsqrt_d(double p) {
-    double y = __builtin_amdgcn_rsq(p);
-    y = y * fma(-0.5 * p * y, y, 1.5);
-    y = y * fma(-0.5 * p * y, y, 1.5);
-    return y;
+    const double y = __builtin_amdgcn_rsq(p);
+    return y * fma(-0.5 * p * y, y, 1.5);
 }
 
 // 16-column steps of the diagonal factorisation, run by one wavefront on
 // registers: lanes 0..15 hold the rows of the 16x16 diagonal sub-block,
 // lanes 16..63 the rows below it.  Step J (a template parameter, so every
 // register index is a constant) scales column J by 1/L[J][J] and subtracts
-// L[r][J] L[c][J] from the columns c > J, with L[c][J] broadcast from the
-// wavefront's LDS row `col`: for lanes below the sub-block this is the
+// L[r][J] L[c][J] from the columns c > J, with L[c][J] read from lane c
+// (readlane, no LDS round trip): for lanes below the sub-block this is the
 // forward substitution of the panel solve, so both run as one pass.  Lanes
 // 0..15 accumulate scratch above their diagonal, which is never written
-// back.  rinv[J] receives 1/L[J][J].
+// back.  y[J] receives 1/L[J][J] (uniform).
 template <int J>
-__device__ __forceinline__ void chol16_step(double (&v)[16], int r, double* __restrict__ col,
-                                            double* __restrict__ rinv) {
-    const double y = rsqrt_d(readlane_d(v[J], J));
-    v[J] *= y;  // lane J: L[J][J] = p / sqrt(p); below: a[r][J] / L[J][J]
-    if (r == J) rinv[J] = y;
+__device__ __forceinline__ void chol16_step(double (&v)[16], double (&y)[16]) {
+    y[J] = rsqrt_d(readlane_d(v[J], J));
+    v[J] *= y[J];  // lane J: L[J][J] = p / sqrt(p); below: a[r][J] / L[J][J]
     if constexpr (J + 1 < 16) {
-        col[r] = v[J];
 #pragma unroll
-        for (int c = J + 1; c < 16; ++c) v[c] = fma(-v[J], col[c], v[c]);
-        chol16_step<J + 1>(v, r, col, rinv);
+        for (int c = J + 1; c < 16; ++c) v[c] = fma(-v[J], readlane_d(v[J], c), v[c]);
+        chol16_step<J + 1>(v, y);
     }
 }
 
@@ -541,89 +542,78 @@ static constexpr int LDT = NB + 2;      // row stride of the transposed FP64 blo
 
 // Factor the 64x64 FP64 block s (lower triangle, row-major) in place with
 // 256 threads: four 16-column sub-blocks, each factored together with the
-// rows below it by wavefront 0 in registers (no barrier inside), then a
-// rank-16 update of the rest of the block by all threads.  dinv receives
-// the reciprocals of the diagonal; col is a 64-double LDS scratch row.
-__device__ __forceinline__ void potrf64(double (*s)[LDS_ROW], double* __restrict__ dinv, double* __restrict__ col) {
-    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+// rows below it by wavefront 0 in registers (no barrier inside), then the
+// rank-16 update of the 16x16 blocks of the rest of the lower triangle on
+// FP64 MFMA, one block per wavefront at a time.  dinv receives the
+// reciprocals of the diagonal.
+__device__ __forceinline__ void potrf64(double (*s)[LDS_ROW], double* __restrict__ dinv) {
+    const int tid = threadIdx.x, l = tid & 63, w = wave_of(), lr = l & 15, lk = l >> 4;
 #pragma unroll 1
     for (int sb = 0; sb < NB / 16; ++sb) {
         const int q0 = 16 * sb;
         if (tid < 64) {
             const int r = tid, row = q0 + r;  // lanes 0..15: sub-block rows; 16..63: rows below
             const bool live = row < NB;
-            double v[16];
+            double v[16], y[16];
 #pragma unroll
             for (int c = 0; c < 16; ++c) v[c] = live && (r >= 16 || c <= r) ? s[row][q0 + c] : 0.0;
-            chol16_step<0>(v, r, col, dinv + q0);
+            chol16_step<0>(v, y);
             if (live) {
 #pragma unroll
                 for (int c = 0; c < 16; ++c)
                     if (r >= 16 || c <= r) s[row][q0 + c] = v[c];
             }
+            if (r == 0) {
+#pragma unroll
+                for (int c = 0; c < 16; ++c) dinv[q0 + c] = y[c];
+            }
         }
         __syncthreads();
-        const int b0 = q0 + 16;  // rank-16 update of rows/columns b0 .. 63
-        if (b0 < NB) {
-            double acc[3][3];
+        // C_IJ -= L_I L_J^T for the blocks sb < J <= I <= 3 (tiles numbered
+        // row by row), with L_I the rows 16I.. of column block sb
+        const int nbt = 3 - sb, ntile = nbt * (nbt + 1) / 2;
+        for (int tt = w; tt < ntile; tt += 4) {
+            int bi = 0;
+            while ((bi + 1) * (bi + 2) / 2 <= tt) ++bi;
+            const int I = sb + 1 + bi, Jb = sb + 1 + (tt - bi * (bi + 1) / 2);
+            f64x4 acc;
 #pragma unroll
-            for (int a = 0; a < 3; ++a)
+            for (int v = 0; v < 4; ++v) acc[v] = s[16 * I + lk + 4 * v][16 * Jb + lr];
 #pragma unroll
-                for (int b = 0; b < 3; ++b) acc[a][b] = 0.0;
+            for (int k0 = 0; k0 < 16; k0 += 4)
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-s[16 * I + lr][q0 + k0 + lk], s[16 * Jb + lr][q0 + k0 + lk],
+                                                           acc, 0, 0, 0);
 #pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                double lr[3], lc[3];
-#pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    const int r = b0 + ty + 16 * a, c = b0 + tx + 16 * a;
-                    lr[a] = r < NB ? s[r][q0 + q] : 0.0;
-                    lc[a] = c < NB ? s[c][q0 + q] : 0.0;
-                }
-#pragma unroll
-                for (int a = 0; a < 3; ++a)
-#pragma unroll
-                    for (int b = 0; b < 3; ++b) acc[a][b] = fma(lr[a], lc[b], acc[a][b]);
-            }
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-#pragma unroll
-                for (int b = 0; b < 3; ++b) {
-                    const int r = b0 + ty + 16 * a, c = b0 + tx + 16 * b;
-                    if (r < NB && c <= r) s[r][c] -= acc[a][b];
-                }
-            __syncthreads();
+            for (int v = 0; v < 4; ++v) s[16 * I + lk + 4 * v][16 * Jb + lr] = acc[v];
         }
+        __syncthreads();
     }
 }
 
-// Off-diagonal 16x16 blocks of the inverse at distance D = I - J:
-// Li_IJ = -Li_II (sum_{J<=K<I} L_IK Li_KJ), thread (ty, tx) one element of
-// each block, sums split over two accumulators.
+// Off-diagonal 16x16 blocks of the inverse at distance D = I - J, on FP64
+// MFMA, one block per wavefront: T = sum_{J<=K<I} L_IK Li_KJ (K = 16D
+// columns of L against rows of Li^T), then Li_IJ = -Li_II T, with T passed
+// through the wavefront's own LDS tile.
 template <int D>
 __device__ __forceinline__ void inverse_level(const double (*s)[LDS_ROW], double* __restrict__ LiT,
-                                              double (*T)[16][17], int ty, int tx) {
+                                              double (*T)[16][17]) {
+    const int l = threadIdx.x & 63, w = wave_of(), lr = l & 15, lk = l >> 4;
+    if (w + D < 4) {
+        const int J = w, I = w + D;
+        f64x4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int J = 0; J + D < 4; ++J) {
-        const int I = J + D;
-        double a0 = 0.0, a1 = 0.0;
+        for (int k0 = 0; k0 < 16 * D; k0 += 4)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(s[16 * I + lr][16 * J + k0 + lk],
+                                                       LiT[(16 * J + lr) * LDT + 16 * J + k0 + lk], acc, 0, 0, 0);
 #pragma unroll
-        for (int t = 16 * J; t < 16 * I; t += 2) {
-            a0 = fma(s[16 * I + ty][t], LiT[(16 * J + tx) * LDT + t], a0);
-            a1 = fma(s[16 * I + ty][t + 1], LiT[(16 * J + tx) * LDT + t + 1], a1);
-        }
-        T[J][ty][tx] = a0 + a1;
-    }
-    __syncthreads();
+        for (int v = 0; v < 4; ++v) T[w][lk + 4 * v][lr] = acc[v];
+        f64x4 li = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int J = 0; J + D < 4; ++J) {
-        const int I = J + D;
-        double a0 = 0.0, a1 = 0.0;
+        for (int k0 = 0; k0 < 16; k0 += 4)
+            li = __builtin_amdgcn_mfma_f64_16x16x4f64(-LiT[(16 * I + k0 + lk) * LDT + 16 * I + lr], T[w][k0 + lk][lr],
+                                                      li, 0, 0, 0);
 #pragma unroll
-        for (int t = 0; t < 16; t += 2) {
-            a0 = fma(LiT[(16 * I + t) * LDT + 16 * I + ty], T[J][t][tx], a0);
-            a1 = fma(LiT[(16 * I + t + 1) * LDT + 16 * I + ty], T[J][t + 1][tx], a1);
-        }
-        LiT[(16 * J + tx) * LDT + 16 * I + ty] = -(a0 + a1);
+        for (int v = 0; v < 4; ++v) LiT[(16 * J + lr) * LDT + 16 * I + lk + 4 * v] = li[v];
     }
     __syncthreads();
 }
@@ -639,210 +629,27 @@ __device__ __forceinline__ void tri_inverse64(const double (*s)[LDS_ROW], double
     for (int e = tid; e < NB * LDT; e += 256) LiT[e] = 0.0;
     __syncthreads();
     {
+        // column c of inv(L_II): y_t = (e_c - sum_{u<t} L_tu y_u) / L_tt, the
+        // sums accumulated right-looking so each y_t waits for one product
         const int I = tid >> 6, c = tid & 63, q0 = 16 * I;
         if (c < 16) {
-            double y[16];
+            double acc[16], y[16];
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                double acc = 0.0;
+            for (int r = 0; r < 16; ++r) acc[r] = 0.0;
 #pragma unroll
-                for (int t = 0; t < r; ++t) acc = fma(s[q0 + r][q0 + t], y[t], acc);
-                y[r] = r < c ? 0.0 : (r == c ? dinv[q0 + r] : -acc * dinv[q0 + r]);
+            for (int t = 0; t < 16; ++t) {
+                y[t] = t < c ? 0.0 : (t == c ? dinv[q0 + t] : -acc[t] * dinv[q0 + t]);
+#pragma unroll
+                for (int r = t + 1; r < 16; ++r) acc[r] = fma(s[q0 + r][q0 + t], y[t], acc[r]);
             }
 #pragma unroll
             for (int r = 0; r < 16; ++r) LiT[(q0 + c) * LDT + q0 + r] = y[r];
         }
     }
     __syncthreads();
-    const int ty = tid >> 4, tx = tid & 15;
-    inverse_level<1>(s, LiT, T, ty, tx);
-    inverse_level<2>(s, LiT, T, ty, tx);
-    inverse_level<3>(s, LiT, T, ty, tx);
-}
-
-// The 64x64 block of M at rows r0.., columns kr.. (column block k of the
-// panel starting at column c0) minus sum_{j<k} X_j L_j^T over the panel's
-// first 64k columns, where X are rows r0.. and L rows kr.. of the panel.
-// Wave w computes rows 16w..16w+15 on MFMA, from the split-BF16 copy of the
-// panel (three products) or from M in FP32.  The result goes to LDS in
-// FP64, row-major (dst[r*dld + c]) or transposed (dst[c*dld + r]).
-// v_mfma_f32_16x16x32_bf16 lane layout: A[m=l%16][k=8(l/16)+0..7],
-// B[k=8(l/16)+0..7][n=l%16], C[4(l/16)+0..3][l%16]; v_mfma_f32_16x16x4_f32:
-// A[m=l%16][k=l/16], B[k=l/16][n=l%16], C as above.
-template <int MODE>
-__device__ __forceinline__ void block_update(const float* __restrict__ M, size_t ld, size_t r0, size_t kr,
-                                             size_t c0, int k, const unsigned short* __restrict__ phi,
-                                             const unsigned short* __restrict__ plo, double* __restrict__ dst,
-                                             int dld, bool trans) {
-    const int l = threadIdx.x & 63, w = wave_of();
-    const int lr = l & 15;
-    f32x4 acc[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const int K = NB * k;
-    if constexpr (MODE == SOP_CHOL_BF16X3) {
-        const int lk = (l >> 4) * 8;
-        const size_t arow = (r0 + 16 * w + lr) * NB2 + lk;
-#pragma unroll 1
-        for (int jc = 0; jc < K; jc += NB)
-#pragma unroll
-        for (int kc = jc; kc < jc + NB; kc += 32) {
-            const bf16x8 ah = *reinterpret_cast<const bf16x8*>(phi + arow + kc);
-            const bf16x8 al = *reinterpret_cast<const bf16x8*>(plo + arow + kc);
-            bf16x8 bh[4], bl[4];
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const size_t brow = (kr + 16 * t + lr) * NB2 + lk + kc;
-                bh[t] = *reinterpret_cast<const bf16x8*>(phi + brow);
-                bl[t] = *reinterpret_cast<const bf16x8*>(plo + brow);
-            }
-#pragma unroll
-            for (int t = 0; t < 4; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[t], acc[t], 0, 0, 0);
-#pragma unroll
-            for (int t = 0; t < 4; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[t], acc[t], 0, 0, 0);
-#pragma unroll
-            for (int t = 0; t < 4; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[t], acc[t], 0, 0, 0);
-        }
-    } else {
-        const int lk = l >> 4;
-        const float* __restrict__ arow = M + (r0 + 16 * w + lr) * ld + c0 + lk;
-#pragma unroll 1
-        for (int jc = 0; jc < K; jc += NB)
-#pragma unroll 4
-        for (int kc = jc; kc < jc + NB; kc += 4) {
-            const float av = arow[kc];
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const float bv = M[(kr + 16 * t + lr) * ld + c0 + lk + kc];
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc[t], 0, 0, 0);
-            }
-        }
-    }
-#pragma unroll
-    for (int t = 0; t < 4; ++t)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int row = 16 * w + 4 * (l >> 4) + q, col = 16 * t + lr;
-            const double v = (double)(M[(r0 + row) * ld + kr + col] - acc[t][q]);
-            dst[trans ? col * dld + row : row * dld + col] = v;
-        }
-}
-
-// X = A Li^T for one 64x64 block in FP64: At[t*LDT + r] = A[r][t] and
-// LiT[t*LDT + c] = Li[c][t] (zero for t > c).  Thread (rb, cb) computes
-// rows 4rb..4rb+3 and columns 4cb..4cb+3.  X goes to M (rows r0..,
-// columns col0..) in FP32 and, in split-BF16 mode, to the panel copy
-// (columns pcol0.. of the panel).
-template <int MODE>
-__device__ __forceinline__ void trsm_block(const double* __restrict__ At, const double* __restrict__ LiT,
-                                           float* __restrict__ M, size_t ld, size_t r0, size_t col0,
-                                           unsigned short* __restrict__ phi, unsigned short* __restrict__ plo,
-                                           int pcol0) {
-    const int tid = threadIdx.x, rb = tid >> 4, cb = tid & 15;
-    double x[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) x[i][j] = 0.0;
-    const int tend = 4 * cb + 4;
-    for (int t = 0; t < tend; ++t) {
-        const double2 a01 = *reinterpret_cast<const double2*>(At + t * LDT + 4 * rb);
-        const double2 a23 = *reinterpret_cast<const double2*>(At + t * LDT + 4 * rb + 2);
-        const double2 b01 = *reinterpret_cast<const double2*>(LiT + t * LDT + 4 * cb);
-        const double2 b23 = *reinterpret_cast<const double2*>(LiT + t * LDT + 4 * cb + 2);
-        const double a[4] = {a01.x, a01.y, a23.x, a23.y}, b[4] = {b01.x, b01.y, b23.x, b23.y};
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) x[i][j] = fma(a[i], b[j], x[i][j]);
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const size_t row = r0 + 4 * rb + i;
-        const float4 v = make_float4((float)x[i][0], (float)x[i][1], (float)x[i][2], (float)x[i][3]);
-        *reinterpret_cast<float4*>(M + row * ld + col0 + 4 * cb) = v;
-        if constexpr (MODE == SOP_CHOL_BF16X3) {
-            const float vv[4] = {v.x, v.y, v.z, v.w};
-            ushort4 h, lo;
-            unsigned short hh[4], ll[4];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                hh[j] = f2bf(vv[j]);
-                ll[j] = f2bf(vv[j] - bf2f(hh[j]));
-            }
-            h = make_ushort4(hh[0], hh[1], hh[2], hh[3]);
-            lo = make_ushort4(ll[0], ll[1], ll[2], ll[3]);
-            *reinterpret_cast<ushort4*>(phi + row * NB2 + pcol0 + 4 * cb) = h;
-            *reinterpret_cast<ushort4*>(plo + row * NB2 + pcol0 + 4 * cb) = lo;
-        }
-    }
-}
-
-// Diagonal region of the outer panel at column c0 (kw columns, nb = kw/64
-// blocks), one workgroup, left-looking over its blocks k: A_kk minus the
-// panel's earlier columns, its FP64 factor and inverse (to linv, block k,
-// transposed), then the blocks I > k below it in the region as
-// (A_Ik - sum_{j<k} X_Ij L_kj^T) Li_kk^T.
-template <int MODE>
-__global__ __launch_bounds__(256) void k_panel_diag(float* __restrict__ M, size_t ld, int c0, int kw,
-                                                    double* __restrict__ linv, unsigned short* __restrict__ phi,
-                                                    unsigned short* __restrict__ plo, const int* __restrict__ flag) {
-    if (flag && !*flag) return;
-    // the diagonal block being factored (row stride LDS_ROW) and, once it
-    // is inverted, the transposed block being solved (row stride LDT)
-    __shared__ __attribute__((aligned(16))) double SA[NB * LDT];
-    __shared__ __attribute__((aligned(16))) double LiT[NB * LDT];
-    __shared__ double T[3][16][17];
-    __shared__ double dinv[NB];
-    __shared__ double col[NB];
-    double(*S)[LDS_ROW] = reinterpret_cast<double(*)[LDS_ROW]>(SA);
-    double* const At = SA;
-    const int nb = kw / NB, tid = threadIdx.x;
-#pragma unroll 1
-    for (int k = 0; k < nb; ++k) {
-        const size_t kr = (size_t)c0 + (size_t)NB * k;
-        block_update<MODE>(M, ld, kr, kr, c0, k, phi, plo, &S[0][0], LDS_ROW, false);
-        __syncthreads();
-        potrf64(S, dinv, col);
-        for (int e = tid; e < NB * NB; e += 256) {
-            const int r = e / NB, c = e % NB;
-            if (c <= r) M[(kr + r) * ld + kr + c] = (float)S[r][c];
-        }
-        tri_inverse64(S, LiT, T, dinv);
-        for (int e = tid; e < NB * NB; e += 256) linv[(size_t)k * NB * NB + e] = LiT[(e / NB) * LDT + e % NB];
-#pragma unroll 1
-        for (int I = k + 1; I < nb; ++I) {
-            const size_t r0 = (size_t)c0 + (size_t)NB * I;
-            block_update<MODE>(M, ld, r0, kr, c0, k, phi, plo, At, LDT, true);
-            __syncthreads();
-            trsm_block<MODE>(At, LiT, M, ld, r0, kr, phi, plo, NB * k);
-            __syncthreads();
-        }
-    }
-}
-
-// Rows of the outer panel below its diagonal region, 64 per workgroup, with
-// the diagonal blocks' inverses from linv: for k = 0..nb-1,
-// X_Rk = (A_Rk - sum_{j<k} X_Rj L_kj^T) Li_kk^T.
-template <int MODE>
-__global__ __launch_bounds__(256) void k_panel_rows(float* __restrict__ M, size_t ld, int c0, int kw, int mp,
-                                                    const double* __restrict__ linv, unsigned short* __restrict__ phi,
-                                                    unsigned short* __restrict__ plo, const int* __restrict__ flag) {
-    if (flag && !*flag) return;
-    __shared__ __attribute__((aligned(16))) double At[NB * LDT];
-    __shared__ __attribute__((aligned(16))) double LiT[NB * LDT];
-    const size_t r0 = (size_t)c0 + kw + (size_t)blockIdx.x * NB;
-    if (r0 >= (size_t)mp) return;
-    const int nb = kw / NB;
-#pragma unroll 1
-    for (int k = 0; k < nb; ++k) {
-        const size_t kr = (size_t)c0 + (size_t)NB * k;
-        for (int e = threadIdx.x; e < NB * NB; e += 256) LiT[(e / NB) * LDT + e % NB] = linv[(size_t)k * NB * NB + e];
-        block_update<MODE>(M, ld, r0, kr, c0, k, phi, plo, At, LDT, true);
-        __syncthreads();
-        trsm_block<MODE>(At, LiT, M, ld, r0, kr, phi, plo, NB * k);
-        __syncthreads();
-    }
+    inverse_level<1>(s, LiT, T);
+    inverse_level<2>(s, LiT, T);
+    inverse_level<3>(s, LiT, T);
 }
 
 // Trailing update after an outer panel: C_IJ -= P_I P_J^T over the panel's
@@ -856,11 +663,11 @@ static constexpr int KCF = 16;   // K-chunk of the FP32 update (floats)
 static constexpr int LPB = KCB + 8;  // padded LDS row (bf16 elements)
 static constexpr int LPF = KCF + 4;  // padded LDS row (floats)
 
-// Tile (blockIdx.x, blockIdx.y) of the update region: rows [rs, mp),
-// columns [cs, ce).  Tiles wholly above the diagonal have nothing to do.
-__device__ __forceinline__ bool big_tile(int rs, int cs, size_t& r0, size_t& c0) {
-    r0 = (size_t)rs + (size_t)blockIdx.x * GT;
-    c0 = (size_t)cs + (size_t)blockIdx.y * GT;
+// Tile (blockIdx.x, blockIdx.y) of the update region: rows and columns
+// [s, mp).  Tiles wholly above the diagonal have nothing to do.
+__device__ __forceinline__ bool big_tile(int s, size_t& r0, size_t& c0) {
+    r0 = (size_t)s + (size_t)blockIdx.x * GT;
+    c0 = (size_t)s + (size_t)blockIdx.y * GT;
     return c0 < r0 + GT;
 }
 
@@ -883,13 +690,14 @@ __device__ __forceinline__ void big_store(float* __restrict__ M, size_t ld, int 
 
 // Operands of the next K-chunk are loaded into registers while the MFMAs of
 // the current chunk run from LDS; they are stored to LDS after the chunk.
-__global__ __launch_bounds__(256) void k_gemm_bf16x3(float* __restrict__ M, size_t ld, int rs, int cs, int ce, int mp,
-                                                     int kw, const unsigned short* __restrict__ phi,
+__global__ __launch_bounds__(256) void k_gemm_bf16x3(float* __restrict__ M, size_t ld, int s, int mp, int kw,
+                                                     const unsigned short* __restrict__ phi,
                                                      const unsigned short* __restrict__ plo) {
     __shared__ __attribute__((aligned(16))) unsigned short sa[2][GT][LPB];  // hi, lo of the row tile
     __shared__ __attribute__((aligned(16))) unsigned short sb[2][GT][LPB];  // hi, lo of the column tile
+    const int ce = mp;
     size_t r0, c0;
-    if (!big_tile(rs, cs, r0, c0)) return;
+    if (!big_tile(s, r0, c0)) return;
     const int tid = threadIdx.x, l = tid & 63, w = wave_of();
     const int lr = l & 15, lk = (l >> 4) * 8;
     f32x4 acc[4][4];
@@ -974,12 +782,12 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(float* __restrict__ M, size
 // LDS rows hold a K-chunk of 16 permuted so that lane group g = l/16 reads
 // k = g, g+4, g+8, g+12 (its four MFMA steps) as one float4.  The next
 // chunk is loaded into registers during the current chunk's MFMAs.
-__global__ __launch_bounds__(256) void k_gemm_f32(float* __restrict__ M, size_t ld, int rs, int cs, int ce, int mp,
-                                                  int pc, int kw) {
+__global__ __launch_bounds__(256) void k_gemm_f32(float* __restrict__ M, size_t ld, int s, int mp, int pc, int kw) {
     __shared__ __attribute__((aligned(16))) float sa[GT][LPF];
     __shared__ __attribute__((aligned(16))) float sb[GT][LPF];
+    const int ce = mp;
     size_t r0, c0;
-    if (!big_tile(rs, cs, r0, c0)) return;
+    if (!big_tile(s, r0, c0)) return;
     const int tid = threadIdx.x, l = tid & 63, w = wave_of();
     const int lr = l & 15, lg = l >> 4;
     f32x4 acc[4][4];
@@ -1053,110 +861,329 @@ __global__ __launch_bounds__(256) void k_gemm_f32(float* __restrict__ M, size_t 
     big_store(M, ld, mp, ce, r0, c0, acc);
 }
 
-// Update of the next panel's columns (on the critical path, so built for
-// latency): C -= P_I P_J^T on 64x64 tiles, the four waves of a workgroup
-// splitting K (wave w takes the panel's columns 64w..64w+63, nothing when
-// beyond kw) and loading their MFMA operands straight from memory, all at
-// once; the four partial tiles are summed through LDS in wave order.
+// ---- the factorisation by fused 64-column steps -----------------------------
+// Within an outer panel (columns c0 .. c0+kw-1, nb = kw/64 column blocks),
+// step j (0 <= j <= nb) is one launch over the 64-row blocks g >= c0/64 + j
+// (workgroup x serves g = c0/64 + j + x; d = c0/64 + j is the step's
+// diagonal block):
+//  (1) for j >= 1, each row block's solve against column block j-1:
+//      X_g = B_g Li^T with B_g the block as the previous step left it and
+//      Li^T the inverse of L_{j-1,j-1} (FP64 accumulation), written to M and,
+//      split-BF16 mode, to the panel copy;
+//  (2) for j < nb, column block j of each row block minus the products of
+//      every column to its left that no trailing update has applied: the
+//      previous outer panel's columns (its lookahead update, folded in here)
+//      and this panel's columns 0 .. 64j-1, on MFMA:
+//      B_g = A_gj - sum_k X_gk X_dk^T.  Workgroup 0 (g = d) factors B in
+//      FP64 into L_jj and its inverse; the others store B for step j+1.
+// Every workgroup forms X_d for (2) itself (the diagonal row block's solve,
+// the same arithmetic as workgroup 0's), so no workgroup waits for another.
+// The trailing matrix beyond the next panel is updated by the wide MFMA
+// kernels on the second stream, one pass per outer panel.
+static constexpr int XS = NB + 4;  // row stride of the FP32 X blocks in LDS (16-byte rows)
+
+struct CholStep {
+    float* M;
+    size_t ld;
+    int mp;
+    int c0, kw;                      // this outer panel
+    int pc, pkw;                     // the previous outer panel (pkw = 0: none)
+    int j;                           // step 0 .. nb
+    const double* li_in;             // Li^T of column block j-1 (64 x 64, row a = column a of Li)
+    double* li_out;                  // Li^T of column block j
+    const float* b_in;               // B of column block j-1, mp x 64 by row
+    float* b_out;                    // B of column block j
+    unsigned short *phi, *plo;       // this panel's split-BF16 copy, mp x NB2
+    const unsigned short *pphi, *pplo;  // the previous panel's copy
+};
+
+// The 64 values of B a lane feeds the solve's MFMAs: B[16w + l%16][4s + l/16]
+// for s = 0..15 (B is 64 x 64 FP32, rows r0.. of a row-major mp x 64 buffer).
+__device__ __forceinline__ void load_solve_rows(const float* __restrict__ b, size_t r0, float (&bv)[16]) {
+    const int l = threadIdx.x & 63, w = wave_of();
+    const float* __restrict__ row = b + (r0 + 16 * w + (l & 15)) * NB + (l >> 4);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) bv[q] = row[4 * q];
+}
+
+// X = B Li^T for one 64x64 block on FP64 MFMA (B from load_solve_rows,
+// LiT[t*LDT + c] = Li[c][t], zero for t > c), to Xs (LDS, row stride XS)
+// and, when `out`, to M (rows r0.., columns col0..) and the panel copy
+// (columns pcol0..).  Wave w forms rows 16w..16w+15; output block jb
+// (columns 16jb..) sums t < 16(jb+1) only, where Li^T is not zero.
+// v_mfma_f64_16x16x4f64 lane layout: A[i=l%16][k=l/16], B[k=l/16][j=l%16],
+// D[l/16+4v][l%16] in register v (rows interleaved, unlike the FP32 form).
 template <int MODE>
-__global__ __launch_bounds__(256) void k_gemm_ks(float* __restrict__ M, size_t ld, int rs, int cs, int ce, int mp,
-                                                 int pc, int kw, const unsigned short* __restrict__ phi,
-                                                 const unsigned short* __restrict__ plo) {
-    __shared__ float red[4][NB][NB + 1];
-    const size_t r0 = (size_t)rs + (size_t)blockIdx.x * NB, c0 = (size_t)cs + (size_t)blockIdx.y * NB;
-    if (c0 > r0 + NB - 1) return;  // wholly above the diagonal
-    const int tid = threadIdx.x, l = tid & 63, w = wave_of(), lr = l & 15;
-    f32x4 acc[4][4];
+__device__ __forceinline__ void solve_block(const float (&bv)[16], const double* __restrict__ LiT,
+                                            float* __restrict__ Xs, bool out, float* __restrict__ M, size_t ld,
+                                            size_t r0, size_t col0, unsigned short* __restrict__ phi,
+                                            unsigned short* __restrict__ plo, int pcol0) {
+    const int l = threadIdx.x & 63, w = wave_of(), lr = l & 15, lk = l >> 4;
+    f64x4 acc[4];
 #pragma unroll
-    for (int bi = 0; bi < 4; ++bi)
+    for (int jb = 0; jb < 4; ++jb) acc[jb] = f64x4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const int k0 = NB * w;
-    if (k0 < kw) {
-        if constexpr (MODE == SOP_CHOL_BF16X3) {
-            const int lk = (l >> 4) * 8;
-            bf16x8 ah[2][4], al[2][4], bh[2][4], bl[2][4];
+    for (int q = 0; q < 16; ++q) {
+        const int t0 = 4 * q;
+        const double a = (double)bv[q];
 #pragma unroll
-            for (int h = 0; h < 2; ++h)
+        for (int jb = 0; jb < 4; ++jb)
+            if (t0 < 16 * (jb + 1))
+                acc[jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, LiT[(t0 + lk) * LDT + 16 * jb + lr], acc[jb], 0, 0,
+                                                                0);
+    }
 #pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    const size_t ao = (r0 + 16 * b + lr) * NB2 + k0 + 32 * h + lk;
-                    const size_t bo = (c0 + 16 * b + lr) * NB2 + k0 + 32 * h + lk;
-                    ah[h][b] = *reinterpret_cast<const bf16x8*>(phi + ao);
-                    al[h][b] = *reinterpret_cast<const bf16x8*>(plo + ao);
-                    bh[h][b] = *reinterpret_cast<const bf16x8*>(phi + bo);
-                    bl[h][b] = *reinterpret_cast<const bf16x8*>(plo + bo);
-                }
+    for (int jb = 0; jb < 4; ++jb)
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
-#pragma unroll
-                for (int bi = 0; bi < 4; ++bi)
-#pragma unroll
-                    for (int bj = 0; bj < 4; ++bj)
-                        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[h][bi], bh[h][bj], acc[bi][bj], 0, 0, 0);
-#pragma unroll
-                for (int bi = 0; bi < 4; ++bi)
-#pragma unroll
-                    for (int bj = 0; bj < 4; ++bj)
-                        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[h][bi], bl[h][bj], acc[bi][bj], 0, 0, 0);
-#pragma unroll
-                for (int bi = 0; bi < 4; ++bi)
-#pragma unroll
-                    for (int bj = 0; bj < 4; ++bj)
-                        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[h][bi], bh[h][bj], acc[bi][bj], 0, 0, 0);
-            }
-        } else {
-            // lane group g = l/16 holds k = kc+4g..kc+4g+3; MFMA step i uses
-            // component i, so step i covers k = kc+4g+i over the groups g
-            const int lg = l >> 4;
-            float4 av[4][4], bv[4][4];
-#pragma unroll
-            for (int h = 0; h < 4; ++h)
-#pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    av[h][b] = *reinterpret_cast<const float4*>(M + (r0 + 16 * b + lr) * ld + pc + k0 + 16 * h + 4 * lg);
-                    bv[h][b] = *reinterpret_cast<const float4*>(M + (c0 + 16 * b + lr) * ld + pc + k0 + 16 * h + 4 * lg);
-                }
-#pragma unroll
-            for (int h = 0; h < 4; ++h) {
-#pragma unroll
-                for (int bi = 0; bi < 4; ++bi)
-#pragma unroll
-                    for (int bj = 0; bj < 4; ++bj)
-                        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[h][bi].x, bv[h][bj].x, acc[bi][bj], 0, 0, 0);
-#pragma unroll
-                for (int bi = 0; bi < 4; ++bi)
-#pragma unroll
-                    for (int bj = 0; bj < 4; ++bj)
-                        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[h][bi].y, bv[h][bj].y, acc[bi][bj], 0, 0, 0);
-#pragma unroll
-                for (int bi = 0; bi < 4; ++bi)
-#pragma unroll
-                    for (int bj = 0; bj < 4; ++bj)
-                        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[h][bi].z, bv[h][bj].z, acc[bi][bj], 0, 0, 0);
-#pragma unroll
-                for (int bi = 0; bi < 4; ++bi)
-#pragma unroll
-                    for (int bj = 0; bj < 4; ++bj)
-                        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[h][bi].w, bv[h][bj].w, acc[bi][bj], 0, 0, 0);
+        for (int v = 0; v < 4; ++v) {
+            const int row = 16 * w + lk + 4 * v, c = 16 * jb + lr;
+            const float x = (float)acc[jb][v];
+            Xs[row * XS + c] = x;
+            if (!out) continue;
+            M[(r0 + row) * ld + col0 + c] = x;
+            if constexpr (MODE == SOP_CHOL_BF16X3) {
+                const unsigned short h = f2bf(x);
+                phi[(r0 + row) * NB2 + pcol0 + c] = h;
+                plo[(r0 + row) * NB2 + pcol0 + c] = f2bf(x - bf2f(h));
             }
         }
+}
+
+// acc += X_g X_d^T (the full 64x64 block, 4x4 MFMA tiles) over this
+// wavefront's share of the columns left of column block j-1 that no
+// trailing update has applied: the previous panel's pkw columns, then this
+// panel's first 64(j-1).  Those columns are cut into chunks (32 columns
+// split-BF16, 16 FP32) dealt to the four wavefronts in turn; each keeps one
+// chunk's operands in flight while the MFMAs of the previous one run.
+template <int MODE>
+__device__ __forceinline__ void mfma_split(f32x4 (&acc)[4][4], const CholStep& a, size_t rg, size_t rd) {
+    const int l = threadIdx.x & 63, w = wave_of(), lr = l & 15;
+    constexpr int CW = MODE == SOP_CHOL_BF16X3 ? 32 : 16;
+    const int n1 = a.pkw / CW, n = n1 + (a.j >= 2 ? NB * (a.j - 1) : 0) / CW;
+    if constexpr (MODE == SOP_CHOL_BF16X3) {
+        const int lk = (l >> 4) * 8;
+        struct Ops {
+            bf16x8 ah[4], al[4], bh[4], bl[4];
+        };
+        auto load = [&](Ops& o, int c) {
+            const unsigned short* __restrict__ ph = c < n1 ? a.pphi : a.phi;
+            const unsigned short* __restrict__ pl = c < n1 ? a.pplo : a.plo;
+            const int col = (c < n1 ? c : c - n1) * CW + lk;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const size_t ao = (rg + 16 * b + lr) * NB2 + col, bo = (rd + 16 * b + lr) * NB2 + col;
+                o.ah[b] = *reinterpret_cast<const bf16x8*>(ph + ao);
+                o.al[b] = *reinterpret_cast<const bf16x8*>(pl + ao);
+                o.bh[b] = *reinterpret_cast<const bf16x8*>(ph + bo);
+                o.bl[b] = *reinterpret_cast<const bf16x8*>(pl + bo);
+            }
+        };
+        auto mma = [&](const Ops& o) {
+#pragma unroll
+            for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < 4; ++bj)
+                    acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(o.ah[bi], o.bh[bj], acc[bi][bj], 0, 0, 0);
+#pragma unroll
+            for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < 4; ++bj)
+                    acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(o.ah[bi], o.bl[bj], acc[bi][bj], 0, 0, 0);
+#pragma unroll
+            for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < 4; ++bj)
+                    acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(o.al[bi], o.bh[bj], acc[bi][bj], 0, 0, 0);
+        };
+        Ops o0, o1;
+        if (w < n) load(o0, w);
+        for (int c = w; c < n; c += 8) {
+            if (c + 4 < n) load(o1, c + 4);
+            mma(o0);
+            if (c + 4 >= n) break;
+            if (c + 8 < n) load(o0, c + 8);
+            mma(o1);
+        }
+    } else {
+        // lane group g = l/16 holds k = 4g .. 4g+3 of a 16-column chunk (one
+        // 16-byte load); MFMA step i takes component i
+        const int lg = l >> 4;
+        struct Ops {
+            float4 a[4], b[4];
+        };
+        auto load = [&](Ops& o, int c) {
+            const size_t col = (c < n1 ? (size_t)a.pc + (size_t)c * CW : (size_t)a.c0 + (size_t)(c - n1) * CW) + 4 * lg;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                o.a[b] = *reinterpret_cast<const float4*>(a.M + (rg + 16 * b + lr) * a.ld + col);
+                o.b[b] = *reinterpret_cast<const float4*>(a.M + (rd + 16 * b + lr) * a.ld + col);
+            }
+        };
+        auto mma = [&](const Ops& o) {
+#pragma unroll
+            for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < 4; ++bj) {
+                    acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(o.a[bi].x, o.b[bj].x, acc[bi][bj], 0, 0, 0);
+                    acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(o.a[bi].y, o.b[bj].y, acc[bi][bj], 0, 0, 0);
+                    acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(o.a[bi].z, o.b[bj].z, acc[bi][bj], 0, 0, 0);
+                    acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(o.a[bi].w, o.b[bj].w, acc[bi][bj], 0, 0, 0);
+                }
+        };
+        Ops o0, o1;
+        if (w < n) load(o0, w);
+        for (int c = w; c < n; c += 8) {
+            if (c + 4 < n) load(o1, c + 4);
+            mma(o0);
+            if (c + 4 >= n) break;
+            if (c + 8 < n) load(o0, c + 8);
+            mma(o1);
+        }
     }
+}
+
+// The same over the 64 columns of the X blocks just solved, from LDS; in
+// split-BF16 mode each value is split as the panel copy holds it.
+template <int MODE>
+__device__ __forceinline__ void mfma_lds(f32x4 (&acc)[4], const float* __restrict__ Xg, const float* __restrict__ Xd) {
+    const int l = threadIdx.x & 63, w = wave_of(), lr = l & 15;
+    if constexpr (MODE == SOP_CHOL_BF16X3) {
+        const int lk = (l >> 4) * 8;
 #pragma unroll
-    for (int bi = 0; bi < 4; ++bi)
+        for (int kc = 0; kc < NB; kc += 32) {
+            bf16x8 ah, al, bh[4], bl[4];
+            const float* a = Xg + (16 * w + lr) * XS + kc + lk;
 #pragma unroll
-        for (int bj = 0; bj < 4; ++bj)
+            for (int e = 0; e < 8; ++e) {
+                const unsigned short h = f2bf(a[e]);
+                ah[e] = (short)h;
+                al[e] = (short)f2bf(a[e] - bf2f(h));
+            }
 #pragma unroll
-            for (int q = 0; q < 4; ++q) red[w][16 * bi + 4 * (l >> 4) + q][16 * bj + lr] = acc[bi][bj][q];
+            for (int t = 0; t < 4; ++t) {
+                const float* b = Xd + (16 * t + lr) * XS + kc + lk;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const unsigned short h = f2bf(b[e]);
+                    bh[t][e] = (short)h;
+                    bl[t][e] = (short)f2bf(b[e] - bf2f(h));
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[t], acc[t], 0, 0, 0);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[t], acc[t], 0, 0, 0);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[t], acc[t], 0, 0, 0);
+        }
+    } else {
+        const int lg = l >> 4;  // k = kc+4g .. kc+4g+3, as in mfma_rows
+#pragma unroll
+        for (int kc = 0; kc < NB; kc += 16) {
+            const float4 a = *reinterpret_cast<const float4*>(Xg + (16 * w + lr) * XS + kc + 4 * lg);
+            float4 b[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) b[t] = *reinterpret_cast<const float4*>(Xd + (16 * t + lr) * XS + kc + 4 * lg);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b[t].x, acc[t], 0, 0, 0);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b[t].y, acc[t], 0, 0, 0);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.z, b[t].z, acc[t], 0, 0, 0);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.w, b[t].w, acc[t], 0, 0, 0);
+        }
+    }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_chol_step(CholStep a) {
+    __shared__ __attribute__((aligned(16))) double SA[NB * LDT];   // B_g^T for a solve, then the block factored
+    __shared__ __attribute__((aligned(16))) double LiT[NB * LDT];  // Li^T in, then the new inverse
+    __shared__ __attribute__((aligned(16))) float Xg[NB * XS];
+    __shared__ __attribute__((aligned(16))) float Xd[NB * XS];
+    __shared__ double T[3][16][17];
+    __shared__ double dinv[NB];
+    const int tid = threadIdx.x;
+    const int nb = a.kw / NB, j = a.j;
+    const int x = blockIdx.x;
+    const size_t rd = (size_t)a.c0 + (size_t)NB * j;  // first row of the diagonal block d
+    const size_t rg = rd + (size_t)NB * x;             // first row of this workgroup's block g
+    const bool update = j < nb;
+    const int l = tid & 63, w = wave_of();
+    const size_t cj = rd;  // column block j starts at column c0 + 64j = rd
+    // every global read first, so their latencies overlap: this row block of
+    // column block j, the B rows of the solves, the inverse Li^T
+    float mblk[4][4], bg[16], bd[16];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            mblk[t][q] = update ? a.M[(rg + 16 * w + 4 * (l >> 4) + q) * a.ld + cj + 16 * t + (l & 15)] : 0.f;
+    if (j >= 1) {
+        load_solve_rows(a.b_in, rg, bg);
+        if (update && x != 0) load_solve_rows(a.b_in, rd, bd);
+        for (int e = tid; e < NB * NB; e += 256) LiT[(e >> 6) * LDT + (e & 63)] = a.li_in[e];
+        __syncthreads();
+        const int pcol = NB * (j - 1);  // column block j-1 within the panel
+        if (update && x != 0)  // X_d, for the update only
+            solve_block<MODE>(bd, LiT, Xd, false, a.M, a.ld, rd, a.c0 + pcol, a.phi, a.plo, pcol);
+        solve_block<MODE>(bg, LiT, Xg, true, a.M, a.ld, rg, a.c0 + pcol, a.phi, a.plo, pcol);
+    }
+    if (!update) return;
+    __syncthreads();  // X blocks complete; SA and LiT are free
+    // the columns left of j-1, K split over the wavefronts; the four partial
+    // blocks are summed through LDS in wavefront order
+    f32x4 acc[4];
+    {
+        f32x4 part[4][4];
+#pragma unroll
+        for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 4; ++bj) part[bi][bj] = f32x4{0.f, 0.f, 0.f, 0.f};
+        mfma_split<MODE>(part, a, rg, rd);
+        constexpr int PS = NB + 1;
+        float* const P = (w < 2 ? reinterpret_cast<float*>(SA) : reinterpret_cast<float*>(LiT)) + (w & 1) * NB * PS;
+#pragma unroll
+        for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 4; ++bj)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) P[(16 * bi + 4 * (l >> 4) + q) * PS + 16 * bj + (l & 15)] = part[bi][bj][q];
+        __syncthreads();
+        const float* const P0 = reinterpret_cast<const float*>(SA);
+        const float* const P2 = reinterpret_cast<const float*>(LiT);
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int e = (16 * w + 4 * (l >> 4) + q) * PS + 16 * t + (l & 15);
+                acc[t][q] = ((P0[e] + P0[NB * PS + e]) + P2[e]) + P2[NB * PS + e];
+            }
+        __syncthreads();  // SA and LiT are reused below
+    }
+    if (j >= 1) mfma_lds<MODE>(acc, Xg, x == 0 ? Xg : Xd);
+    double(*S)[LDS_ROW] = reinterpret_cast<double(*)[LDS_ROW]>(SA);
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int row = 16 * w + 4 * (l >> 4) + q, c = 16 * t + (l & 15);
+            const float b = mblk[t][q] - acc[t][q];
+            if (x == 0)
+                S[row][c] = (double)b;
+            else
+                a.b_out[(rg + row) * NB + c] = b;
+        }
+    if (x != 0) return;
     __syncthreads();
-#pragma unroll 4
+    potrf64(S, dinv);
     for (int e = tid; e < NB * NB; e += 256) {
         const int r = e / NB, c = e % NB;
-        const size_t row = r0 + r, col = c0 + c;
-        if (row >= (size_t)mp || col >= (size_t)ce || col > row) continue;
-        const float s = ((red[0][r][c] + red[1][r][c]) + red[2][r][c]) + red[3][r][c];
-        M[row * ld + col] -= s;
+        if (c <= r) a.M[(rd + r) * a.ld + cj + c] = (float)S[r][c];
     }
+    tri_inverse64(S, LiT, T, dinv);
+    for (int e = tid; e < NB * NB; e += 256) a.li_out[e] = LiT[(e >> 6) * LDT + (e & 63)];
 }
 
 // ---------------------------------------------------------------------------
@@ -1180,80 +1207,75 @@ hipError_t launch_hi_tea_step(const DevJob* jobs, const DevJobState* st, const i
 
 // Rows [rs, mp) x columns [cs, ce) of the trailing matrix minus the outer
 // panel (columns [pc, pc+kw)) times its transpose.
-static void trailing_update(float* M, size_t ld, int rs, int cs, int ce, int mp, int pc, int kw, int mode,
-                            const unsigned short* phi, const unsigned short* plo, hipStream_t stream,
-                            bool latency = false) {
-    if (rs >= mp || cs >= ce) return;
-    if (latency) {
-        const dim3 g((mp - rs) / NB, (ce - cs) / NB);
-        if (mode == SOP_CHOL_F32)
-            hipLaunchKernelGGL(k_gemm_ks<SOP_CHOL_F32>, g, dim3(256), 0, stream, M, ld, rs, cs, ce, mp, pc, kw, phi,
-                               plo);
-        else
-            hipLaunchKernelGGL(k_gemm_ks<SOP_CHOL_BF16X3>, g, dim3(256), 0, stream, M, ld, rs, cs, ce, mp, pc, kw,
-                               phi, plo);
-        return;
-    }
-    const dim3 g((mp - rs + GT - 1) / GT, (ce - cs + GT - 1) / GT);
+static void trailing_update(float* M, size_t ld, int s, int mp, int pc, int kw, int mode, const unsigned short* phi,
+                            const unsigned short* plo, hipStream_t stream) {
+    if (s >= mp) return;
+    const int nr = (mp - s + GT - 1) / GT;
+    const dim3 g(nr, nr);
     if (mode == SOP_CHOL_F32)
-        hipLaunchKernelGGL(k_gemm_f32, g, dim3(256), 0, stream, M, ld, rs, cs, ce, mp, pc, kw);
+        hipLaunchKernelGGL(k_gemm_f32, g, dim3(256), 0, stream, M, ld, s, mp, pc, kw);
     else
-        hipLaunchKernelGGL(k_gemm_bf16x3, g, dim3(256), 0, stream, M, ld, rs, cs, ce, mp, kw, phi, plo);
+        hipLaunchKernelGGL(k_gemm_bf16x3, g, dim3(256), 0, stream, M, ld, s, mp, kw, phi, plo);
 }
 
-// Right-looking blocked factorisation with a lookahead of one outer panel.
-// After panel p is factored, the columns of panel p+1 are updated on the
-// main stream (T1) and the rest of the trailing matrix on `aux` (T2), so
-// panel p+1 is factored while T2(p) runs.  T1(p+1) and T2(p) both write
-// panel p+2's columns, so the main stream waits for T2(p) before T1(p+1).
-// With aux == nullptr the same updates run in order on `stream`.
-hipError_t launch_chol_factor(float* M, size_t ld, int mp, int mode, double* linv, unsigned short* phi,
-                              unsigned short* plo, const int* flag, hipStream_t stream, hipStream_t aux,
-                              hipEvent_t ev_f, hipEvent_t ev_t2) {
-    const bool two = aux != nullptr && ev_f != nullptr && ev_t2 != nullptr;
-    bool t2_pending = false;
-    for (int c0 = 0; c0 < mp; c0 += NB2) {
-        const int kw = mp - c0 < NB2 ? mp - c0 : NB2;
-        // split-BF16 copies of consecutive panels alternate between two
-        // buffers: T2 of panel p still reads its copy while panel p+1 is factored
-        const size_t pbuf = (size_t)((c0 / NB2) & 1) * mp * NB2;
-        unsigned short* const ph = phi + pbuf;
-        unsigned short* const pl = plo + pbuf;
-        const int below = mp - c0 - kw;
-        if (mode == SOP_CHOL_F32) {
-            hipLaunchKernelGGL(k_panel_diag<SOP_CHOL_F32>, dim3(1), dim3(256), 0, stream, M, ld, c0, kw, linv, ph, pl,
-                               flag);
-            if (below > 0)
-                hipLaunchKernelGGL(k_panel_rows<SOP_CHOL_F32>, dim3(below / NB), dim3(256), 0, stream, M, ld, c0, kw,
-                                   mp, linv, ph, pl, flag);
-        } else {
-            hipLaunchKernelGGL(k_panel_diag<SOP_CHOL_BF16X3>, dim3(1), dim3(256), 0, stream, M, ld, c0, kw, linv, ph,
-                               pl, flag);
-            if (below > 0)
-                hipLaunchKernelGGL(k_panel_rows<SOP_CHOL_BF16X3>, dim3(below / NB), dim3(256), 0, stream, M, ld, c0,
-                                   kw, mp, linv, ph, pl, flag);
+// Blocked factorisation over outer panels of NB2 columns, each factored by
+// fused 64-column steps (k_chol_step).  A panel's columns receive the
+// previous panel's update inside its own steps; the rest of the trailing
+// matrix (from the panel after the next one on) is updated by the wide MFMA
+// kernels on `aux`, overlapped with the next panel.  Panel p waits for the
+// trailing update of panel p-2, which wrote its columns and read the copy
+// buffer panel p writes.  With aux == nullptr everything runs on `stream`.
+hipError_t launch_chol_factor(float* M, size_t ld, int mp, int mode, const CholWork& w, hipStream_t stream,
+                              hipStream_t aux, hipEvent_t ev_f, hipEvent_t ev_t2a, hipEvent_t ev_t2b) {
+    // FP32 overlaps its trailing updates with the next panel's steps; the
+    // split-BF16 update is short enough that overlapping it slows the
+    // latency-bound steps by more than it hides, so it runs in stream order
+    const bool two = mode == SOP_CHOL_F32 && aux != nullptr && ev_f != nullptr && ev_t2a != nullptr &&
+                     ev_t2b != nullptr;
+    hipEvent_t ev_t2[2] = {ev_t2a, ev_t2b};
+    bool pending[2] = {false, false};
+    hipError_t e = hipSuccess;
+    const int nblk = mp / NB;
+    int pc = 0, pkw = 0;
+    for (int c0 = 0, p = 0; c0 < mp; c0 += NB2, ++p) {
+        const int kw = mp - c0 < NB2 ? mp - c0 : NB2, nb = kw / NB;
+        unsigned short* const ph = w.phi + (size_t)(p & 1) * mp * NB2;
+        unsigned short* const pl = w.plo + (size_t)(p & 1) * mp * NB2;
+        const unsigned short* const qh = w.phi + (size_t)((p + 1) & 1) * mp * NB2;
+        const unsigned short* const ql = w.plo + (size_t)((p + 1) & 1) * mp * NB2;
+        if (two && pending[p & 1]) {
+            if ((e = hipStreamWaitEvent(stream, ev_t2[p & 1], 0)) != hipSuccess) return e;
+            pending[p & 1] = false;
         }
-        const int s0 = c0 + kw;
-        if (s0 >= mp) break;
-        const int s1 = s0 + NB2 < mp ? s0 + NB2 : mp;
-        if (two) {
-            hipError_t e = hipEventRecord(ev_f, stream);  // panel p factored
-            if (e != hipSuccess) return e;
-            if (t2_pending && (e = hipStreamWaitEvent(stream, ev_t2, 0)) != hipSuccess) return e;
-            trailing_update(M, ld, s0, s0, s1, mp, c0, kw, mode, ph, pl, stream, true);
-            if ((e = hipStreamWaitEvent(aux, ev_f, 0)) != hipSuccess) return e;
-            trailing_update(M, ld, s1, s1, mp, mp, c0, kw, mode, ph, pl, aux);
-            if ((e = hipEventRecord(ev_t2, aux)) != hipSuccess) return e;
-            t2_pending = true;
-        } else {  // the same two updates in stream order
-            trailing_update(M, ld, s0, s0, s1, mp, c0, kw, mode, ph, pl, stream, true);
-            trailing_update(M, ld, s1, s1, mp, mp, c0, kw, mode, ph, pl, stream);
+        for (int j = 0; j <= nb; ++j) {
+            const int nwg = nblk - (c0 / NB + j);
+            if (nwg <= 0) continue;
+            const CholStep a{M, ld, mp, c0, kw, pc, pkw, j,
+                             w.linv + (size_t)((j + 1) & 1) * NB * NB, w.linv + (size_t)(j & 1) * NB * NB,
+                             w.bbuf + (size_t)((j + 1) & 1) * mp * NB, w.bbuf + (size_t)(j & 1) * mp * NB,
+                             ph, pl, qh, ql};
+            if (mode == SOP_CHOL_F32)
+                hipLaunchKernelGGL(k_chol_step<SOP_CHOL_F32>, dim3(nwg), dim3(256), 0, stream, a);
+            else
+                hipLaunchKernelGGL(k_chol_step<SOP_CHOL_BF16X3>, dim3(nwg), dim3(256), 0, stream, a);
         }
+        const int s1 = c0 + kw + NB2;  // the first column the next panel does not fold in
+        if (s1 < mp) {
+            if (two) {
+                if ((e = hipEventRecord(ev_f, stream)) != hipSuccess) return e;
+                if ((e = hipStreamWaitEvent(aux, ev_f, 0)) != hipSuccess) return e;
+                trailing_update(M, ld, s1, mp, c0, kw, mode, ph, pl, aux);
+                if ((e = hipEventRecord(ev_t2[p & 1], aux)) != hipSuccess) return e;
+                pending[p & 1] = true;
+            } else {
+                trailing_update(M, ld, s1, mp, c0, kw, mode, ph, pl, stream);
+            }
+        }
+        pc = c0;
+        pkw = kw;
     }
-    if (t2_pending) {
-        hipError_t e = hipStreamWaitEvent(stream, ev_t2, 0);
-        if (e != hipSuccess) return e;
-    }
+    for (int q = 0; q < 2; ++q)
+        if (pending[q] && (e = hipStreamWaitEvent(stream, ev_t2[q], 0)) != hipSuccess) return e;
     return hipGetLastError();
 }
 
@@ -1264,8 +1286,8 @@ hipError_t launch_chol_step(const DevJob* jobs, const DevJobState* st, const int
         hipLaunchKernelGGL(k_chol_build, dim3(ntiles, ntiles), dim3(256), 0, stream, jobs, st, tiles, kstep);
         // the factorisation's error covers the event record/wait pairs that
         // order its two streams: report it before the noise reads the factor
-        const hipError_t e = launch_chol_factor(w->M, w->ld, w->mp, w->mode, w->linv, w->phi, w->plo, nullptr,
-                                                stream, w->aux, w->ev_f, w->ev_t2);
+        const hipError_t e =
+            launch_chol_factor(w->M, w->ld, w->mp, w->mode, w->work, stream, w->aux, w->ev_f, w->ev_t2[0], w->ev_t2[1]);
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(k_chol_L, dim3(ntiles * 16), dim3(256), 0, stream, jobs, st, tiles, ntiles, kstep);
@@ -1287,9 +1309,11 @@ hipError_t launch_chol_step(const DevJob* jobs, const DevJobState* st, const int
 namespace sop {
 
 constexpr int kNoAllocMaxOrder = 12288;  // 3N for N = 4,096 beads
+constexpr size_t kNoAllocBBytes = (size_t)2 * kNoAllocMaxOrder * 64 * sizeof(float);  // chol_bbuf_bytes
 constexpr size_t kNoAllocPanelBytes = (size_t)2 * kNoAllocMaxOrder * kCholPanel * sizeof(unsigned short);
-constexpr size_t kNoAllocScratchBytes = kCholLinvBytes + 2 * kNoAllocPanelBytes;  // linv | phi | plo
-static_assert(kCholLinvBytes % 256 == 0 && kNoAllocPanelBytes % 256 == 0, "scratch sections stay 256-byte aligned");
+constexpr size_t kNoAllocScratchBytes = kCholLinvBytes + kNoAllocBBytes + 2 * kNoAllocPanelBytes;  // linv|bbuf|phi|plo
+static_assert(kCholLinvBytes % 256 == 0 && kNoAllocBBytes % 256 == 0 && kNoAllocPanelBytes % 256 == 0,
+              "scratch sections stay 256-byte aligned");
 
 static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
@@ -1337,6 +1361,9 @@ extern "C" int sop_chol_factor_noalloc(int device, int n, float* M, unsigned lon
     if (ld < (unsigned long long)mp)
         return fail(SOP_E_ARG, "cholesky (no allocation): leading dimension " + std::to_string(ld) +
                                    " is below the padded order " + std::to_string(mp));
+    if (ld % 4 != 0 || reinterpret_cast<uintptr_t>(M) % 16 != 0)
+        return fail(SOP_E_ARG, "cholesky (no allocation): rows must start on 16-byte boundaries "
+                               "(M 16-byte aligned, leading dimension a multiple of 4)");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(SOP_E_HIP, "no HIP device available");
     if (device < 0 || device >= ndev || device >= kMaxDevices) return fail(SOP_E_ARG, "device index out of range");
@@ -1345,13 +1372,16 @@ extern "C" int sop_chol_factor_noalloc(int device, int n, float* M, unsigned lon
     hipError_t e = scratch_on(device, &base);
     if (e != hipSuccess)
         return fail(SOP_E_HIP, std::string("cholesky (no allocation): scratch address: ") + hipGetErrorString(e));
-    double* linv = reinterpret_cast<double*>(base);
-    unsigned short* phi = reinterpret_cast<unsigned short*>(base + kCholLinvBytes);
-    unsigned short* plo = reinterpret_cast<unsigned short*>(base + kCholLinvBytes + kNoAllocPanelBytes);
+    // the workspace of an order-mp factorisation, carved from the front of each section
+    CholWork w{};
+    w.linv = reinterpret_cast<double*>(base);
+    w.bbuf = reinterpret_cast<float*>(base + kCholLinvBytes);
+    w.phi = reinterpret_cast<unsigned short*>(base + kCholLinvBytes + kNoAllocBBytes);
+    w.plo = reinterpret_cast<unsigned short*>(base + kCholLinvBytes + kNoAllocBBytes + kNoAllocPanelBytes);
     const hipStream_t s = static_cast<hipStream_t>(stream);
     if (mp > n) hipLaunchKernelGGL(k_chol_pad, dim3((mp + 255) / 256, mp - n), dim3(256), 0, s, M, (size_t)ld, n, mp);
     e = hipGetLastError();
-    if (e == hipSuccess) e = launch_chol_factor(M, (size_t)ld, mp, mode, linv, phi, plo, nullptr, s);
+    if (e == hipSuccess) e = launch_chol_factor(M, (size_t)ld, mp, mode, w, s);
     if (e != hipSuccess) return fail(SOP_E_HIP, std::string("cholesky (no allocation): ") + hipGetErrorString(e));
     return 0;
 }

@@ -62,6 +62,7 @@ def test_autotune_leaves_the_jobs_as_they_were():
 @pytest.mark.parametrize("shape", [
     dict(unit_rows=512, shared_units=0, pair_batch=8, verlet_chunk=4, sort_every=10, max_blocks=256),
     dict(unit_rows=768, shared_units=1, pair_batch=4, verlet_chunk=8, sort_every=40, max_blocks=0),
+    dict(unit_rows=128, shared_units=1, pair_batch=6, verlet_chunk=8, sort_every=20, max_blocks=0),
 ])
 def test_every_shape_gives_the_same_bits(shape):
     jobs = _jobs()

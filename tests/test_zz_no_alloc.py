@@ -58,5 +58,7 @@ def test_noalloc_factor_rejects_bad_sizes():
     assert b"exceeds the maximum" in S.lib.sop_last_error()
     assert f(0, 100, buf.data_ptr(), 64, S.CHOL_F32, None) == -1  # ld below the padded order 128
     assert b"leading dimension" in S.lib.sop_last_error()
+    assert f(0, 60, buf.data_ptr(), 66, S.CHOL_F32, None) == -1  # rows off 16-byte boundaries
+    assert b"16-byte" in S.lib.sop_last_error()
     assert f(0, 64, buf.data_ptr(), 64, 7, None) == -1
     assert f(0, 0, buf.data_ptr(), 64, S.CHOL_F32, None) == -1
