@@ -253,7 +253,8 @@ def hi_phase(S, n=4096, reps=5, steps=50, cov=True):
     """Exact hydrodynamics at N = 4096 (12,288 degrees of freedom): the
     blocked Cholesky of the RPY matrix, split-BF16 vs FP32 MFMA trailing
     updates, timed and checked against the FP64 matrix; then the exact-HI
-    integrator (factor every 10 steps, 100 pN force clamp) in both modes."""
+    integrator (factor every 10 steps, 100 pN force clamp) in both modes,
+    and the TEA integrator on the same chain."""
     sysm = S.synth_chain(n, seed=4096)
     sysm.fixed = np.array([0], np.int32)
     sysm.pulled = np.array([n - 1], np.int32)
@@ -284,6 +285,24 @@ def hi_phase(S, n=4096, reps=5, steps=50, cov=True):
         out[name]["integrator_steps_per_s"] = done / dt
         eng.close()
     out["speedup_bf16x3_over_f32"] = out["f32"]["ms_per_factor"] / out["bf16x3"]["ms_per_factor"]
+    # TEA on the same chain and force clamp: epsilon and beta' every 10 steps
+    p = S.make_params(kspring_cov=14.0, R_limit=2.0, pairs_cutoff=20.0, pairlist_cutoff=20.0,
+                      pairs_threshold=32.0, pairs_freq=10, possiblepairs_freq=100, timestep=H_TAU,
+                      temperature=0.6, seed=4096, nav=1000, pulling=1, fixedEnd=0, pulledEnd=n - 1,
+                      fconst=100.0 / PN_PER_KCAL_A, hi_on=1, hi_exact=0, hi_a=3.8, hi_epsilon_freq=10,
+                      hi_capricious=0)
+    eng = S.Engine(0, 16 << 30)
+    jid = eng.add_job(p, sysm, 10 * steps + 10)
+    eng.run_blocks(1)
+    eng.synchronize()
+    t0 = time.perf_counter()
+    eng.run_blocks(10 * steps // eng.block_steps())
+    eng.synchronize()
+    dt = time.perf_counter() - t0
+    eps, beta = eng.hi(jid)
+    out["tea"] = {"integrator_steps_per_s": (10 * steps // eng.block_steps()) * eng.block_steps() / dt,
+                  "epsilon": eps, "beta": beta}
+    eng.close()
     return out
 
 

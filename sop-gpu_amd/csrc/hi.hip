@@ -1123,7 +1123,17 @@ __global__ __launch_bounds__(256) void k_chol_step(CholStep a) {
     if (j >= 1) {
         load_solve_rows(a.b_in, rg, bg);
         if (update && x != 0) load_solve_rows(a.b_in, rd, bd);
-        for (int e = tid; e < NB * NB; e += 256) LiT[(e >> 6) * LDT + (e & 63)] = a.li_in[e];
+        {  // Li^T: eight 16-byte loads per thread, all in flight before the LDS stores
+            const double2* __restrict__ src = reinterpret_cast<const double2*>(a.li_in);
+            double2 lv[NB * NB / 512];
+#pragma unroll
+            for (int k = 0; k < NB * NB / 512; ++k) lv[k] = src[tid + 256 * k];
+#pragma unroll
+            for (int k = 0; k < NB * NB / 512; ++k) {
+                const int e = 2 * (tid + 256 * k);  // e, e+1 in the same row
+                *reinterpret_cast<double2*>(LiT + (e >> 6) * LDT + (e & 63)) = lv[k];
+            }
+        }
         __syncthreads();
         const int pcol = NB * (j - 1);  // column block j-1 within the panel
         if (update && x != 0)  // X_d, for the update only
@@ -1178,12 +1188,23 @@ __global__ __launch_bounds__(256) void k_chol_step(CholStep a) {
     if (x != 0) return;
     __syncthreads();
     potrf64(S, dinv);
-    for (int e = tid; e < NB * NB; e += 256) {
-        const int r = e / NB, c = e % NB;
+#pragma unroll
+    for (int k = 0; k < NB * NB / 256; ++k) {
+        const int e = tid + 256 * k, r = e / NB, c = e % NB;
         if (c <= r) a.M[(rd + r) * a.ld + cj + c] = (float)S[r][c];
     }
     tri_inverse64(S, LiT, T, dinv);
-    for (int e = tid; e < NB * NB; e += 256) a.li_out[e] = LiT[(e >> 6) * LDT + (e & 63)];
+    {  // eight 16-byte LDS reads per thread, then the stores
+        double2 lv[NB * NB / 512];
+#pragma unroll
+        for (int k = 0; k < NB * NB / 512; ++k) {
+            const int e = 2 * (tid + 256 * k);
+            lv[k] = *reinterpret_cast<const double2*>(LiT + (e >> 6) * LDT + (e & 63));
+        }
+        double2* __restrict__ dst = reinterpret_cast<double2*>(a.li_out);
+#pragma unroll
+        for (int k = 0; k < NB * NB / 512; ++k) dst[tid + 256 * k] = lv[k];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1227,11 +1248,7 @@ static void trailing_update(float* M, size_t ld, int s, int mp, int pc, int kw, 
 // buffer panel p writes.  With aux == nullptr everything runs on `stream`.
 hipError_t launch_chol_factor(float* M, size_t ld, int mp, int mode, const CholWork& w, hipStream_t stream,
                               hipStream_t aux, hipEvent_t ev_f, hipEvent_t ev_t2a, hipEvent_t ev_t2b) {
-    // FP32 overlaps its trailing updates with the next panel's steps; the
-    // split-BF16 update is short enough that overlapping it slows the
-    // latency-bound steps by more than it hides, so it runs in stream order
-    const bool two = mode == SOP_CHOL_F32 && aux != nullptr && ev_f != nullptr && ev_t2a != nullptr &&
-                     ev_t2b != nullptr;
+    const bool two = aux != nullptr && ev_f != nullptr && ev_t2a != nullptr && ev_t2b != nullptr;
     hipEvent_t ev_t2[2] = {ev_t2a, ev_t2b};
     bool pending[2] = {false, false};
     hipError_t e = hipSuccess;
