@@ -937,17 +937,13 @@ int Engine::plan_lds_units(std::vector<int2>& units, std::vector<int4>& segs) co
 }
 
 void Engine::drop_graph() {
-    bool any = false;
-    for (auto g : graphs_) any = any || g != nullptr;
-    if (!any) return;
+    if (graphs_.empty()) return;
     // a replay of these graphs may still run on the stream: let it finish
     // before the executable graphs are destroyed
     (void)hipStreamSynchronize(stream_);
-    for (auto& g : graphs_)
-        if (g) {
-            (void)hipGraphExecDestroy(g);
-            g = nullptr;
-        }
+    for (auto& kv : graphs_)
+        if (kv.second) (void)hipGraphExecDestroy(kv.second);
+    graphs_.clear();
 }
 
 bool Engine::uses_cells(size_t n) const {
@@ -1023,6 +1019,28 @@ int Engine::lists_due(int slot_only) const {
         if (hj.dj.sort_rows && s % ((long long)hj.dj.pairs_freq * shape_.sort_every) == 0) m |= 4;
     }
     return m;
+}
+
+// Whether exact-HI job hj refactorises at step k of the block it is in.
+static bool refactors(const HostJob& hj, int k) {
+    const long long sk = hj.step + k;
+    return sk < hj.end_step && sk % hj.hw.chol_freq == 0;
+}
+
+// What the next block of the batch launches beyond its fixed kernels: the
+// list kernels due, then for each exact-HI job (slot order) the steps of the
+// block that refactorise, each job's list closed by -1.  Blocks with equal
+// keys launch the same kernels with the same arguments.
+std::vector<long long> Engine::block_key() const {
+    std::vector<long long> key{lists_due(-1)};
+    if (any_exact_)
+        for (const HostJob& hj : jobs_) {
+            if (!(hj.dj.flags & JF_HI_EXACT)) continue;
+            for (int k = 0; k < L_; ++k)
+                if (refactors(hj, k)) key.push_back(k);
+            key.push_back(-1);
+        }
+    return key;
 }
 
 // One block: per-job updaters, list rebuilds, L steps, step advance.
@@ -1120,8 +1138,7 @@ int Engine::launch_block(int slot_only) {
                 const int jn = slot_only >= 0 ? nt : hj.ntile;
                 const DevJob* tj = slot_only >= 0 ? jobs : d_jobs_;
                 const DevJobState* ts = slot_only >= 0 ? st : d_state_;
-                const long long sk = hj.step + k;
-                const bool refactor = sk < hj.end_step && sk % hj.hw.chol_freq == 0;
+                const bool refactor = refactors(hj, k);
                 if (int rc = check(launch_chol_step(tj, ts, jt, jn, k, refactor ? &hj.hw : nullptr, stream_),
                                    "Cholesky step"))
                     return rc;
@@ -1214,8 +1231,7 @@ int Engine::stage_step(int k) {
     if (any_exact_)
         for (auto& hj : jobs_) {
             if (!(hj.dj.flags & JF_HI_EXACT)) continue;
-            const long long sk = hj.step + k;
-            const bool refactor = sk < hj.end_step && sk % hj.hw.chol_freq == 0;
+            const bool refactor = refactors(hj, k);
             if (int rc = check(launch_chol_step(d_jobs_, d_state_, d_tiles_ + hj.tile0, hj.ntile, k,
                                                 refactor ? &hj.hw : nullptr, stream_),
                                "Cholesky step"))
@@ -1235,7 +1251,7 @@ int Engine::read_state_of(int id, DevJobState* out) {
     return read_state(slot, out);
 }
 
-int Engine::capture_graph(int due) {
+int Engine::capture_graph(const std::vector<long long>& key) {
     hipGraph_t g = nullptr;
     if (int rc = check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture")) return rc;
     int rc = launch_block(-1);
@@ -1245,12 +1261,11 @@ int Engine::capture_graph(int due) {
         return rc;
     }
     if (int r2 = check(e, "end capture")) return r2;
-    e = hipGraphInstantiate(&graphs_[due], g, nullptr, nullptr, 0);
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
-    if (e != hipSuccess) {
-        graphs_[due] = nullptr;
-        return check(e, "graph instantiate");
-    }
+    if (e != hipSuccess) return check(e, "graph instantiate");
+    graphs_[key] = x;
     return 0;
 }
 
@@ -1258,15 +1273,19 @@ int Engine::run_blocks(int nblocks) {
     if (hipSetDevice(device_) != hipSuccess) return fail(SOP_E_HIP, "hipSetDevice failed");
     if (int rc = sync_tables()) return rc;
     if (jobs_.empty()) return 0;
-    // exact-HI factorisations are launched on the steps the host knows are
-    // refactorisation steps, so those blocks are not replayed from a graph
-    const bool graph = use_graph_ && !profiling_ && !any_exact_;
+    // a block is replayed from the graph captured for its launch sequence;
+    // past kMaxGraphs sequences (exact-HI jobs whose refactorisation steps
+    // keep shifting against the block) the new ones are launched directly
+    const bool graph = use_graph_ && !profiling_;
     for (int b = 0; b < nblocks; ++b) {
-        if (graph) {
-            const int due = lists_due(-1);
-            if (!graphs_[due])
-                if (int rc = capture_graph(due)) return rc;
-            if (int rc = check(hipGraphLaunch(graphs_[due], stream_), "graph launch")) return rc;
+        const std::vector<long long> key = graph ? block_key() : std::vector<long long>();
+        auto it = graph ? graphs_.find(key) : graphs_.end();
+        if (graph && it == graphs_.end() && graphs_.size() < kMaxGraphs) {
+            if (int rc = capture_graph(key)) return rc;
+            it = graphs_.find(key);
+        }
+        if (it != graphs_.end()) {
+            if (int rc = check(hipGraphLaunch(it->second, stream_), "graph launch")) return rc;
         } else {
             if (int rc = launch_block(-1)) return rc;
         }

@@ -153,7 +153,8 @@ class Engine {
     int sync_tables();
     int launch_block(int slot_only);  // slot_only < 0: all jobs
     int lists_due(int slot_only) const;
-    int capture_graph(int due);
+    std::vector<long long> block_key() const;
+    int capture_graph(const std::vector<long long>& key);
     void drop_graph();
     int check(hipError_t e, const char* what);
     int read_state(int slot, DevJobState* out);
@@ -213,8 +214,10 @@ class Engine {
     int exact_mode_ = SOP_CHOL_BF16X3;
     bool any_tea_ = false, any_exact_ = false;
     bool use_graph_ = true;
-    // one captured block per set of list kernels it launches (lists_due())
-    hipGraphExec_t graphs_[8] = {};
+    // captured blocks by launch sequence (block_key()): the list kernels due
+    // and each exact-HI job's refactorisation steps; at most kMaxGraphs
+    static constexpr size_t kMaxGraphs = 32;
+    std::map<std::vector<long long>, hipGraphExec_t> graphs_;
     // profiling
     bool profiling_ = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool_;

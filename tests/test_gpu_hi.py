@@ -154,3 +154,35 @@ def test_exact_finite_temperature_step_vs_oracle(faithful, mode):
     for got in (dx_gpu, dx_ref):
         np.testing.assert_allclose(got, expect, rtol=5e-5, atol=4 * ulp)
     np.testing.assert_allclose(dx_gpu, dx_ref, rtol=5e-5, atol=4 * ulp)
+
+
+@pytest.mark.parametrize("mode", [S.CHOL_BF16X3, S.CHOL_F32])
+def test_graph_replay_with_exact_hi_job_equals_direct_launches(mode):
+    """A batch holding an exact-HI job is replayed from captured graphs keyed
+    by the block's launch sequence: with refactorisation every 15 steps and
+    10-step blocks the sequence cycles through three keys (refactor at step
+    0, at step 5, none).  Every job's coordinates after 150 steps at T = 300
+    K equal those of the same batch launched kernel by kernel."""
+    n_exact, n_ramp, n_tea = 300, 500, 200
+    jobs = []
+    s = numpy_chain(n_exact, seed=81)
+    jobs.append((S.make_params(**hi_kw(hi_exact=1, hi_chol_freq=15, temperature=0.6, seed=5)), s))
+    s = numpy_chain(n_ramp, seed=82)
+    s.fixed = np.array([0], np.int32)
+    s.pulled = np.array([n_ramp - 1], np.int32)
+    jobs.append((S.make_params(**c0_params(pulling=1, fixedEnd=0, pulledEnd=n_ramp - 1, deltax=0.02,
+                                           pullFreq=100, seed=6)), s))
+    jobs.append((S.make_params(**hi_kw(temperature=0.6, seed=7)), numpy_chain(n_tea, seed=83)))
+    out = []
+    for graph in (True, False):
+        with S.Engine(0, 1 << 30) as e:
+            e.set_chol_mode(mode)
+            e.set_tuning(0, graph)
+            ids = [e.add_job(p, sysm, 150) for p, sysm in jobs]
+            assert e.block_steps() == 10
+            e.run_blocks(15)
+            e.synchronize()
+            out.append([e.coords(j) for j in ids])
+    for a, b in zip(*out):
+        assert np.isfinite(a).all()
+        np.testing.assert_array_equal(a, b)
