@@ -955,7 +955,10 @@ __device__ __forceinline__ void solve_block(const float (&bv)[16], const double*
 // panel's first 64(j-1).  Those columns are cut into chunks (32 columns
 // split-BF16, 16 FP32) dealt to the four wavefronts in turn; each keeps one
 // chunk's operands in flight while the MFMAs of the previous one run.
-template <int MODE>
+// SYM (the diagonal block, g = d): the rows are loaded once and only the
+// tiles on and below the diagonal are formed (the factorisation reads the
+// lower triangle); each of them sums the same products in the same order.
+template <int MODE, bool SYM>
 __device__ __forceinline__ void mfma_split(f32x4 (&acc)[4][4], const CholStep& a, size_t rg, size_t rd) {
     const int l = threadIdx.x & 63, w = wave_of(), lr = l & 15;
     constexpr int CW = MODE == SOP_CHOL_BF16X3 ? 32 : 16;
@@ -974,26 +977,33 @@ __device__ __forceinline__ void mfma_split(f32x4 (&acc)[4][4], const CholStep& a
                 const size_t ao = (rg + 16 * b + lr) * NB2 + col, bo = (rd + 16 * b + lr) * NB2 + col;
                 o.ah[b] = *reinterpret_cast<const bf16x8*>(ph + ao);
                 o.al[b] = *reinterpret_cast<const bf16x8*>(pl + ao);
-                o.bh[b] = *reinterpret_cast<const bf16x8*>(ph + bo);
-                o.bl[b] = *reinterpret_cast<const bf16x8*>(pl + bo);
+                if constexpr (!SYM) {
+                    o.bh[b] = *reinterpret_cast<const bf16x8*>(ph + bo);
+                    o.bl[b] = *reinterpret_cast<const bf16x8*>(pl + bo);
+                }
             }
         };
         auto mma = [&](const Ops& o) {
+            const bf16x8(&bh)[4] = SYM ? o.ah : o.bh;
+            const bf16x8(&bl)[4] = SYM ? o.al : o.bl;
 #pragma unroll
             for (int bi = 0; bi < 4; ++bi)
 #pragma unroll
                 for (int bj = 0; bj < 4; ++bj)
-                    acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(o.ah[bi], o.bh[bj], acc[bi][bj], 0, 0, 0);
+                    if (!SYM || bj <= bi)
+                        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(o.ah[bi], bh[bj], acc[bi][bj], 0, 0, 0);
 #pragma unroll
             for (int bi = 0; bi < 4; ++bi)
 #pragma unroll
                 for (int bj = 0; bj < 4; ++bj)
-                    acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(o.ah[bi], o.bl[bj], acc[bi][bj], 0, 0, 0);
+                    if (!SYM || bj <= bi)
+                        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(o.ah[bi], bl[bj], acc[bi][bj], 0, 0, 0);
 #pragma unroll
             for (int bi = 0; bi < 4; ++bi)
 #pragma unroll
                 for (int bj = 0; bj < 4; ++bj)
-                    acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(o.al[bi], o.bh[bj], acc[bi][bj], 0, 0, 0);
+                    if (!SYM || bj <= bi)
+                        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(o.al[bi], bh[bj], acc[bi][bj], 0, 0, 0);
         };
         Ops o0, o1;
         if (w < n) load(o0, w);
@@ -1016,19 +1026,21 @@ __device__ __forceinline__ void mfma_split(f32x4 (&acc)[4][4], const CholStep& a
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
                 o.a[b] = *reinterpret_cast<const float4*>(a.M + (rg + 16 * b + lr) * a.ld + col);
-                o.b[b] = *reinterpret_cast<const float4*>(a.M + (rd + 16 * b + lr) * a.ld + col);
+                if constexpr (!SYM) o.b[b] = *reinterpret_cast<const float4*>(a.M + (rd + 16 * b + lr) * a.ld + col);
             }
         };
         auto mma = [&](const Ops& o) {
+            const float4(&ob)[4] = SYM ? o.a : o.b;
 #pragma unroll
             for (int bi = 0; bi < 4; ++bi)
 #pragma unroll
-                for (int bj = 0; bj < 4; ++bj) {
-                    acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(o.a[bi].x, o.b[bj].x, acc[bi][bj], 0, 0, 0);
-                    acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(o.a[bi].y, o.b[bj].y, acc[bi][bj], 0, 0, 0);
-                    acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(o.a[bi].z, o.b[bj].z, acc[bi][bj], 0, 0, 0);
-                    acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(o.a[bi].w, o.b[bj].w, acc[bi][bj], 0, 0, 0);
-                }
+                for (int bj = 0; bj < 4; ++bj)
+                    if (!SYM || bj <= bi) {
+                        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(o.a[bi].x, ob[bj].x, acc[bi][bj], 0, 0, 0);
+                        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(o.a[bi].y, ob[bj].y, acc[bi][bj], 0, 0, 0);
+                        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(o.a[bi].z, ob[bj].z, acc[bi][bj], 0, 0, 0);
+                        acc[bi][bj] = __builtin_amdgcn_mfma_f32_16x16x4f32(o.a[bi].w, ob[bj].w, acc[bi][bj], 0, 0, 0);
+                    }
         };
         Ops o0, o1;
         if (w < n) load(o0, w);
@@ -1151,7 +1163,10 @@ __global__ __launch_bounds__(256) void k_chol_step(CholStep a) {
         for (int bi = 0; bi < 4; ++bi)
 #pragma unroll
             for (int bj = 0; bj < 4; ++bj) part[bi][bj] = f32x4{0.f, 0.f, 0.f, 0.f};
-        mfma_split<MODE>(part, a, rg, rd);
+        if (x == 0)
+            mfma_split<MODE, true>(part, a, rg, rd);
+        else
+            mfma_split<MODE, false>(part, a, rg, rd);
         constexpr int PS = NB + 1;
         float* const P = (w < 2 ? reinterpret_cast<float*>(SA) : reinterpret_cast<float*>(LiT)) + (w & 1) * NB * PS;
 #pragma unroll
