@@ -540,116 +540,147 @@ __device__ __forceinline__ void chol16_step(double (&v)[16], double (&y)[16]) {
 static constexpr int LDS_ROW = NB + 1;  // row stride of the FP64 block being factored
 static constexpr int LDT = NB + 2;      // row stride of the transposed FP64 blocks (16-byte rows)
 
-// Factor the 64x64 FP64 block s (lower triangle, row-major) in place with
-// 256 threads: four 16-column sub-blocks, each factored together with the
-// rows below it by wavefront 0 in registers (no barrier inside), then the
-// rank-16 update of the 16x16 blocks of the rest of the lower triangle on
-// FP64 MFMA, one block per wavefront at a time.  dinv receives the
-// reciprocals of the diagonal.
-__device__ __forceinline__ void potrf64(double (*s)[LDS_ROW], double* __restrict__ dinv) {
-    const int tid = threadIdx.x, l = tid & 63, w = wave_of(), lr = l & 15, lk = l >> 4;
+// Column block K of the 64x64 FP64 block s (rows 16K..63) factored by
+// wavefront 0 in registers: lane r holds row 16K + r (lanes 0..15 the
+// diagonal sub-block, the others the rows below, forward-substituted in
+// the same pass).  Lanes past row 63 read a clamped row and keep zeros;
+// lanes 0..15 write their scratch above the diagonal back, where nothing
+// reads it.  dinv receives the reciprocals of the sub-block's diagonal.
+__device__ __forceinline__ void chol_pass(double (*s)[LDS_ROW], double* __restrict__ dinv, int K) {
+    const int r = threadIdx.x & 63, q0 = 16 * K, row = q0 + r;
+    const bool live = row < NB;
+    const int rr = live ? row : NB - 1;
+    double v[16], y[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+        const double e = s[rr][q0 + c];
+        v[c] = live && (r >= 16 || c <= r) ? e : 0.0;
+    }
+    chol16_step<0>(v, y);
+    if (live) {
+#pragma unroll
+        for (int c = 0; c < 16; ++c) s[row][q0 + c] = v[c];
+    }
+    if (r == 0) {
+#pragma unroll
+        for (int c = 0; c < 16; ++c) dinv[q0 + c] = y[c];
+    }
+}
+
+// C_IJ -= L_IK L_JK^T for the 16x16 tile (I, J) of s on FP64 MFMA, one
+// wavefront.  v_mfma_f64_16x16x4f64 lane layout: A[i=l%16][k=l/16],
+// B[k=l/16][j=l%16], D[l/16+4v][l%16] in register v.
+__device__ __forceinline__ void rank16_tile(double (*s)[LDS_ROW], int I, int J, int K) {
+    const int l = threadIdx.x & 63, lr = l & 15, lk = l >> 4;
+    f64x4 acc;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[v] = s[16 * I + lk + 4 * v][16 * J + lr];
+#pragma unroll
+    for (int k0 = 0; k0 < 16; k0 += 4)
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-s[16 * I + lr][16 * K + k0 + lk], s[16 * J + lr][16 * K + k0 + lk],
+                                                   acc, 0, 0, 0);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) s[16 * I + lk + 4 * v][16 * J + lr] = acc[v];
+}
+
+// Column c (lane c < 16) of inv(L_KK) by forward substitution, y_t =
+// (e_c - sum_{u<t} L_tu y_u) / L_tt with the sums accumulated right-looking
+// so each y_t waits for one product; written transposed to LiT (LiT[a*LDT
+// + b] = Li[b][a]).
+__device__ __forceinline__ void inv_diag16(const double (*s)[LDS_ROW], double* __restrict__ LiT,
+                                           const double* __restrict__ dinv, int K) {
+    const int c = threadIdx.x & 63, q0 = 16 * K;
+    if (c < 16) {
+        double acc[16], y[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[r] = 0.0;
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            y[t] = t < c ? 0.0 : (t == c ? dinv[q0 + t] : -acc[t] * dinv[q0 + t]);
+#pragma unroll
+            for (int r = t + 1; r < 16; ++r) acc[r] = fma(s[q0 + r][q0 + t], y[t], acc[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < 16; ++r) LiT[(q0 + c) * LDT + q0 + r] = y[r];
+    }
+}
+
+// Off-diagonal 16x16 block (I, J) of the inverse, J < I, on FP64 MFMA by one
+// wavefront, in two halves: T = sum_{J<=K<I} L_IK Li_KJ into the
+// wavefront's LDS tile (needs the rows J..I-1 of the inverse), then
+// Li_IJ = -Li_II T (needs Li_II).
+__device__ __forceinline__ void inv_sum16(const double (*s)[LDS_ROW], const double* __restrict__ LiT,
+                                          double (*T)[17], int I, int J) {
+    const int l = threadIdx.x & 63, lr = l & 15, lk = l >> 4;
+    f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+    for (int k0 = 0; k0 < 16 * (I - J); k0 += 4)
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(s[16 * I + lr][16 * J + k0 + lk],
+                                                   LiT[(16 * J + lr) * LDT + 16 * J + k0 + lk], acc, 0, 0, 0);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) T[lk + 4 * v][lr] = acc[v];
+}
+__device__ __forceinline__ void inv_apply16(double* __restrict__ LiT, const double (*T)[17], int I, int J) {
+    const int l = threadIdx.x & 63, lr = l & 15, lk = l >> 4;
+    f64x4 li = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k0 = 0; k0 < 16; k0 += 4)
+        li = __builtin_amdgcn_mfma_f64_16x16x4f64(-LiT[(16 * I + k0 + lk) * LDT + 16 * I + lr], T[k0 + lk][lr], li, 0,
+                                                  0, 0);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) LiT[(16 * J + lr) * LDT + 16 * I + lk + 4 * v] = li[v];
+}
+
+// Factor the 64x64 FP64 block s (lower triangle, row-major) in place and
+// form the inverse of the factor, transposed, in LiT (zero above its
+// diagonal), with 256 threads.  The factor runs in four rounds, one per
+// 16-column block K: wavefront 0 applies the previous block's rank-16
+// update to block K's tiles, then factors block K in registers; meanwhile
+// wavefronts 2 and 3 apply that update to the tiles right of block K, and
+// wavefront 1 forms row K-1 of the inverse (its diagonal block by
+// substitution, the blocks left of it on FP64 MFMA) -- so the inverse trails
+// the factor by one round instead of following it.  The inverse's last row
+// is formed after the last round.  dinv receives the reciprocals of the
+// factor's diagonal.
+__device__ __forceinline__ void potrf_inv64(double (*s)[LDS_ROW], double* __restrict__ LiT, double (*T)[16][17],
+                                            double* __restrict__ dinv) {
+    const int tid = threadIdx.x, w = wave_of();
+    if (w == 0)
+        chol_pass(s, dinv, 0);
+    else
+        for (int e = tid - 64; e < NB * LDT; e += 192) LiT[e] = 0.0;
+    __syncthreads();
 #pragma unroll 1
-    for (int sb = 0; sb < NB / 16; ++sb) {
-        const int q0 = 16 * sb;
-        if (tid < 64) {
-            const int r = tid, row = q0 + r;  // lanes 0..15: sub-block rows; 16..63: rows below
-            const bool live = row < NB;
-            double v[16], y[16];
-#pragma unroll
-            for (int c = 0; c < 16; ++c) v[c] = live && (r >= 16 || c <= r) ? s[row][q0 + c] : 0.0;
-            chol16_step<0>(v, y);
-            if (live) {
-#pragma unroll
-                for (int c = 0; c < 16; ++c)
-                    if (r >= 16 || c <= r) s[row][q0 + c] = v[c];
+    for (int K = 1; K < NB / 16; ++K) {
+        const int k = K - 1;  // the block factored in the previous round
+        if (w == 0) {
+            for (int I = K; I < NB / 16; ++I) rank16_tile(s, I, K, k);
+            chol_pass(s, dinv, K);
+        } else if (w == 1) {
+            inv_diag16(s, LiT, dinv, k);
+            for (int J = 0; J < k; ++J) {
+                inv_sum16(s, LiT, T[0], k, J);
+                inv_apply16(LiT, T[0], k, J);
             }
-            if (r == 0) {
-#pragma unroll
-                for (int c = 0; c < 16; ++c) dinv[q0 + c] = y[c];
-            }
-        }
-        __syncthreads();
-        // C_IJ -= L_I L_J^T for the blocks sb < J <= I <= 3 (tiles numbered
-        // row by row), with L_I the rows 16I.. of column block sb
-        const int nbt = 3 - sb, ntile = nbt * (nbt + 1) / 2;
-        for (int tt = w; tt < ntile; tt += 4) {
-            int bi = 0;
-            while ((bi + 1) * (bi + 2) / 2 <= tt) ++bi;
-            const int I = sb + 1 + bi, Jb = sb + 1 + (tt - bi * (bi + 1) / 2);
-            f64x4 acc;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) acc[v] = s[16 * I + lk + 4 * v][16 * Jb + lr];
-#pragma unroll
-            for (int k0 = 0; k0 < 16; k0 += 4)
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-s[16 * I + lr][q0 + k0 + lk], s[16 * Jb + lr][q0 + k0 + lk],
-                                                           acc, 0, 0, 0);
-#pragma unroll
-            for (int v = 0; v < 4; ++v) s[16 * I + lk + 4 * v][16 * Jb + lr] = acc[v];
+        } else {
+            // tiles (I, J), K < J <= I, dealt to wavefronts 2 and 3 in turn
+            int t = 0;
+            for (int J = K + 1; J < NB / 16; ++J)
+                for (int I = J; I < NB / 16; ++I, ++t)
+                    if ((t & 1) == w - 2) rank16_tile(s, I, J, k);
         }
         __syncthreads();
     }
-}
-
-// Off-diagonal 16x16 blocks of the inverse at distance D = I - J, on FP64
-// MFMA, one block per wavefront: T = sum_{J<=K<I} L_IK Li_KJ (K = 16D
-// columns of L against rows of Li^T), then Li_IJ = -Li_II T, with T passed
-// through the wavefront's own LDS tile.
-template <int D>
-__device__ __forceinline__ void inverse_level(const double (*s)[LDS_ROW], double* __restrict__ LiT,
-                                              double (*T)[16][17]) {
-    const int l = threadIdx.x & 63, w = wave_of(), lr = l & 15, lk = l >> 4;
-    if (w + D < 4) {
-        const int J = w, I = w + D;
-        f64x4 acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int k0 = 0; k0 < 16 * D; k0 += 4)
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(s[16 * I + lr][16 * J + k0 + lk],
-                                                       LiT[(16 * J + lr) * LDT + 16 * J + k0 + lk], acc, 0, 0, 0);
-#pragma unroll
-        for (int v = 0; v < 4; ++v) T[w][lk + 4 * v][lr] = acc[v];
-        f64x4 li = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int k0 = 0; k0 < 16; k0 += 4)
-            li = __builtin_amdgcn_mfma_f64_16x16x4f64(-LiT[(16 * I + k0 + lk) * LDT + 16 * I + lr], T[w][k0 + lk][lr],
-                                                      li, 0, 0, 0);
-#pragma unroll
-        for (int v = 0; v < 4; ++v) LiT[(16 * J + lr) * LDT + 16 * I + lk + 4 * v] = li[v];
-    }
+    // row 3 of the inverse: its diagonal block (wavefront 0) beside the sums
+    // of the blocks left of it (wavefronts 1..3), then the products
+    constexpr int R = NB / 16 - 1;
+    if (w == 0)
+        inv_diag16(s, LiT, dinv, R);
+    else
+        inv_sum16(s, LiT, T[w - 1], R, w - 1);
     __syncthreads();
-}
-
-// Inverse of the lower-triangular 64x64 factor L (FP64, in s; dinv holds
-// the reciprocals of its diagonal) by 16x16 blocks, 256 threads: the four
-// diagonal blocks by forward substitution (wavefront I, lane c: column c of
-// inv(L_II)), then the blocks at distance d = I - J = 1, 2, 3.  Output
-// transposed: LiT[a*LDT + b] = Li[b][a], zero for a > b.
-__device__ __forceinline__ void tri_inverse64(const double (*s)[LDS_ROW], double* __restrict__ LiT,
-                                              double (*T)[16][17], const double* __restrict__ dinv) {
-    const int tid = threadIdx.x;
-    for (int e = tid; e < NB * LDT; e += 256) LiT[e] = 0.0;
+    if (w != 0) inv_apply16(LiT, T[w - 1], R, w - 1);
     __syncthreads();
-    {
-        // column c of inv(L_II): y_t = (e_c - sum_{u<t} L_tu y_u) / L_tt, the
-        // sums accumulated right-looking so each y_t waits for one product
-        const int I = tid >> 6, c = tid & 63, q0 = 16 * I;
-        if (c < 16) {
-            double acc[16], y[16];
-#pragma unroll
-            for (int r = 0; r < 16; ++r) acc[r] = 0.0;
-#pragma unroll
-            for (int t = 0; t < 16; ++t) {
-                y[t] = t < c ? 0.0 : (t == c ? dinv[q0 + t] : -acc[t] * dinv[q0 + t]);
-#pragma unroll
-                for (int r = t + 1; r < 16; ++r) acc[r] = fma(s[q0 + r][q0 + t], y[t], acc[r]);
-            }
-#pragma unroll
-            for (int r = 0; r < 16; ++r) LiT[(q0 + c) * LDT + q0 + r] = y[r];
-        }
-    }
-    __syncthreads();
-    inverse_level<1>(s, LiT, T);
-    inverse_level<2>(s, LiT, T);
-    inverse_level<3>(s, LiT, T);
 }
 
 // Trailing update after an outer panel: C_IJ -= P_I P_J^T over the panel's
@@ -878,6 +909,14 @@ __global__ __launch_bounds__(256) void k_gemm_f32(float* __restrict__ M, size_t 
 //      FP64 into L_jj and its inverse; the others store B for step j+1.
 // Every workgroup forms X_d for (2) itself (the diagonal row block's solve,
 // the same arithmetic as workgroup 0's), so no workgroup waits for another.
+// Lookahead: the diagonal workgroup's chain (solve, update, factor, inverse)
+// is the step's critical path, so the long part of its update -- every
+// column left of j-1 -- is formed one step early: while step j-1 runs, one
+// extra workgroup solves row block d for column block j-2 itself and forms
+// P = A_dd - sum_{k < j-1} X_dk X_dk^T into a step buffer; step j's
+// diagonal workgroup then only subtracts X_d,j-1 X_d,j-1^T (64 columns)
+// before it factors.  The panel's first step has no such block and forms
+// the whole sum.
 // The trailing matrix beyond the next panel is updated by the wide MFMA
 // kernels on the second stream, one pass per outer panel.
 static constexpr int XS = NB + 4;  // row stride of the FP32 X blocks in LDS (16-byte rows)
@@ -893,6 +932,9 @@ struct CholStep {
     double* li_out;                  // Li^T of column block j
     const float* b_in;               // B of column block j-1, mp x 64 by row
     float* b_out;                    // B of column block j
+    const float* p_in;               // the diagonal block's P formed by the previous step (j >= 1)
+    float* p_out;                    // the next diagonal block's P (when look)
+    int look;                        // 1: an extra workgroup forms P for step j+1
     unsigned short *phi, *plo;       // this panel's split-BF16 copy, mp x NB2
     const unsigned short *pphi, *pplo;  // the previous panel's copy
 };
@@ -1118,23 +1160,31 @@ __global__ __launch_bounds__(256) void k_chol_step(CholStep a) {
     __shared__ double dinv[NB];
     const int tid = threadIdx.x;
     const int nb = a.kw / NB, j = a.j;
-    const int x = blockIdx.x;
+    // workgroup 0: the diagonal block; 1 (when a.look): the lookahead, which
+    // serves row block d+1; then the row blocks d+1, d+2, ...
+    const bool look = a.look && blockIdx.x == 1;
+    const int x = look ? 1 : (int)blockIdx.x - (a.look && blockIdx.x > 1 ? 1 : 0);
     const size_t rd = (size_t)a.c0 + (size_t)NB * j;  // first row of the diagonal block d
     const size_t rg = rd + (size_t)NB * x;             // first row of this workgroup's block g
     const bool update = j < nb;
+    const bool diag_p = x == 0 && j >= 1;              // the diagonal block arrives as P
     const int l = tid & 63, w = wave_of();
-    const size_t cj = rd;  // column block j starts at column c0 + 64j = rd
+    // the block this workgroup updates: column block j (rows rg), or for the
+    // lookahead the next diagonal block (rows and columns rg)
+    const size_t cj = look ? rg : rd;
     // every global read first, so their latencies overlap: this row block of
-    // column block j, the B rows of the solves, the inverse Li^T
+    // column block j (or P), the B rows of the solves, the inverse Li^T
     float mblk[4][4], bg[16], bd[16];
 #pragma unroll
     for (int t = 0; t < 4; ++t)
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-            mblk[t][q] = update ? a.M[(rg + 16 * w + 4 * (l >> 4) + q) * a.ld + cj + 16 * t + (l & 15)] : 0.f;
+        for (int q = 0; q < 4; ++q) {
+            const int row = 16 * w + 4 * (l >> 4) + q, c = 16 * t + (l & 15);
+            mblk[t][q] = !update ? 0.f : diag_p ? a.p_in[row * NB + c] : a.M[(rg + row) * a.ld + cj + c];
+        }
     if (j >= 1) {
         load_solve_rows(a.b_in, rg, bg);
-        if (update && x != 0) load_solve_rows(a.b_in, rd, bd);
+        if (update && x != 0 && !look) load_solve_rows(a.b_in, rd, bd);
         {  // Li^T: eight 16-byte loads per thread, all in flight before the LDS stores
             const double2* __restrict__ src = reinterpret_cast<const double2*>(a.li_in);
             double2 lv[NB * NB / 512];
@@ -1148,23 +1198,27 @@ __global__ __launch_bounds__(256) void k_chol_step(CholStep a) {
         }
         __syncthreads();
         const int pcol = NB * (j - 1);  // column block j-1 within the panel
-        if (update && x != 0)  // X_d, for the update only
+        if (update && x != 0 && !look)  // X_d, for the update only
             solve_block<MODE>(bd, LiT, Xd, false, a.M, a.ld, rd, a.c0 + pcol, a.phi, a.plo, pcol);
-        solve_block<MODE>(bg, LiT, Xg, true, a.M, a.ld, rg, a.c0 + pcol, a.phi, a.plo, pcol);
+        // the lookahead's rows are written by the workgroup that serves them
+        solve_block<MODE>(bg, LiT, Xg, !look, a.M, a.ld, rg, a.c0 + pcol, a.phi, a.plo, pcol);
     }
     if (!update) return;
     __syncthreads();  // X blocks complete; SA and LiT are free
-    // the columns left of j-1, K split over the wavefronts; the four partial
-    // blocks are summed through LDS in wavefront order
     f32x4 acc[4];
-    {
+    if (diag_p) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    } else {
+        // the columns left of j-1, K split over the wavefronts; the four
+        // partial blocks are summed through LDS in wavefront order
         f32x4 part[4][4];
 #pragma unroll
         for (int bi = 0; bi < 4; ++bi)
 #pragma unroll
             for (int bj = 0; bj < 4; ++bj) part[bi][bj] = f32x4{0.f, 0.f, 0.f, 0.f};
-        if (x == 0)
-            mfma_split<MODE, true>(part, a, rg, rd);
+        if (x == 0 || look)
+            mfma_split<MODE, true>(part, a, rg, rg);
         else
             mfma_split<MODE, false>(part, a, rg, rd);
         constexpr int PS = NB + 1;
@@ -1187,7 +1241,7 @@ __global__ __launch_bounds__(256) void k_chol_step(CholStep a) {
             }
         __syncthreads();  // SA and LiT are reused below
     }
-    if (j >= 1) mfma_lds<MODE>(acc, Xg, x == 0 ? Xg : Xd);
+    if (j >= 1) mfma_lds<MODE>(acc, Xg, x == 0 || look ? Xg : Xd);
     double(*S)[LDS_ROW] = reinterpret_cast<double(*)[LDS_ROW]>(SA);
 #pragma unroll
     for (int t = 0; t < 4; ++t)
@@ -1195,20 +1249,21 @@ __global__ __launch_bounds__(256) void k_chol_step(CholStep a) {
         for (int q = 0; q < 4; ++q) {
             const int row = 16 * w + 4 * (l >> 4) + q, c = 16 * t + (l & 15);
             const float b = mblk[t][q] - acc[t][q];
-            if (x == 0)
+            if (look)
+                a.p_out[row * NB + c] = b;
+            else if (x == 0)
                 S[row][c] = (double)b;
             else
                 a.b_out[(rg + row) * NB + c] = b;
         }
-    if (x != 0) return;
+    if (x != 0 || look) return;
     __syncthreads();
-    potrf64(S, dinv);
+    potrf_inv64(S, LiT, T, dinv);
 #pragma unroll
     for (int k = 0; k < NB * NB / 256; ++k) {
         const int e = tid + 256 * k, r = e / NB, c = e % NB;
-        if (c <= r) a.M[(rd + r) * a.ld + cj + c] = (float)S[r][c];
+        if (c <= r) a.M[(rd + r) * a.ld + rd + c] = (float)S[r][c];
     }
-    tri_inverse64(S, LiT, T, dinv);
     {  // eight 16-byte LDS reads per thread, then the stores
         double2 lv[NB * NB / 512];
 #pragma unroll
@@ -1268,6 +1323,7 @@ hipError_t launch_chol_factor(float* M, size_t ld, int mp, int mode, const CholW
     bool pending[2] = {false, false};
     hipError_t e = hipSuccess;
     const int nblk = mp / NB;
+    float* const pbuf = reinterpret_cast<float*>(w.linv + (size_t)2 * NB * NB);  // after the two inverses
     int pc = 0, pkw = 0;
     for (int c0 = 0, p = 0; c0 < mp; c0 += NB2, ++p) {
         const int kw = mp - c0 < NB2 ? mp - c0 : NB2, nb = kw / NB;
@@ -1282,14 +1338,16 @@ hipError_t launch_chol_factor(float* M, size_t ld, int mp, int mode, const CholW
         for (int j = 0; j <= nb; ++j) {
             const int nwg = nblk - (c0 / NB + j);
             if (nwg <= 0) continue;
+            const int look = j + 1 < nb ? 1 : 0;  // a next diagonal block in this panel
             const CholStep a{M, ld, mp, c0, kw, pc, pkw, j,
                              w.linv + (size_t)((j + 1) & 1) * NB * NB, w.linv + (size_t)(j & 1) * NB * NB,
                              w.bbuf + (size_t)((j + 1) & 1) * mp * NB, w.bbuf + (size_t)(j & 1) * mp * NB,
+                             pbuf + (size_t)(j & 1) * NB * NB, pbuf + (size_t)((j + 1) & 1) * NB * NB, look,
                              ph, pl, qh, ql};
             if (mode == SOP_CHOL_F32)
-                hipLaunchKernelGGL(k_chol_step<SOP_CHOL_F32>, dim3(nwg), dim3(256), 0, stream, a);
+                hipLaunchKernelGGL(k_chol_step<SOP_CHOL_F32>, dim3(nwg + look), dim3(256), 0, stream, a);
             else
-                hipLaunchKernelGGL(k_chol_step<SOP_CHOL_BF16X3>, dim3(nwg), dim3(256), 0, stream, a);
+                hipLaunchKernelGGL(k_chol_step<SOP_CHOL_BF16X3>, dim3(nwg + look), dim3(256), 0, stream, a);
         }
         const int s1 = c0 + kw + NB2;  // the first column the next panel does not fold in
         if (s1 < mp) {
