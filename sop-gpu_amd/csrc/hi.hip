@@ -46,6 +46,7 @@ namespace sop {
 typedef short bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 struct F6 {
     float xx, xy, xz, yy, yz, zz;
@@ -809,6 +810,138 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(float* __restrict__ M, size
     big_store(M, ld, mp, ce, r0, c0, acc);
 }
 
+// Split-BF16 trailing update on 256x256 tiles for the large trailing
+// regions: four times the MFMA work of a 128x128 tile per operand byte
+// staged, half the operand traffic per flop.  One workgroup of eight
+// wavefronts per CU (two per SIMD); wavefront w forms the 64x128 block
+// (w>>1, w&1) as 2x4 tiles of 32x32 on v_mfma_f32_32x32x16_bf16 (lane l:
+// A[i=l%32][k=8(l/32)..+7], B[k=8(l/32)..+7][j=l%32], D row
+// 8(q/4)+4(l/32)+q%4, column l%32 in accumulator register q).  Per
+// 32-column chunk the hi and lo halves of the tile's 256 panel rows and 256
+// panel columns (64 KB) are staged in LDS in 16-byte units, unit u of row r
+// stored at u ^ ((r >> 1) & 3) so that the eight lanes of each read phase
+// hit distinct banks; the next chunk is loaded into registers while the
+// current one is multiplied.  Tiles are numbered over the lower triangle
+// row by row and dealt to the XCDs in contiguous ranges (workgroup i runs
+// on XCD i % 8), so the tiles an XCD holds at once share their row panels
+// in its L2.
+static constexpr int BT = 256;  // tile edge
+static constexpr int BK = 32;   // K chunk (bf16 elements): four 16-byte units per row
+constexpr int kXcds = 8;        // MI355X: workgroup i is dispatched to XCD i % 8
+
+__device__ __forceinline__ int swz(int r, int u) { return u ^ ((r >> 1) & 3); }
+
+// The loader of k_trail_bf16x3: thread t moves unit t%4 of rows t/4 and
+// t/4 + 128, hi and lo, of the row panel (rows from r0) and of the column
+// panel (rows from c0) -- eight 16-byte loads, zeros past row mp -- into v,
+// and v into LDS.  Written as macros: the same code as helper functions or
+// lambdas makes this compiler (ROCm 7.2 clang) crash in its machine copy
+// propagation pass.
+#define TRAIL_FETCH(KC)                                                                       \
+    for (int pnl = 0; pnl < 2; ++pnl)                                                         \
+        for (int q = 0; q < 2; ++q) {                                                         \
+            const size_t g = (pnl ? c0 : r0) + (size_t)(lrow + 128 * q);                      \
+            if (g < (size_t)mp) {                                                             \
+                const size_t off = g * NB2 + (size_t)(KC) + 8 * lu;                           \
+                v[pnl][0][q] = *reinterpret_cast<const uint4*>(phi + off);                    \
+                v[pnl][1][q] = *reinterpret_cast<const uint4*>(plo + off);                    \
+            } else {                                                                          \
+                v[pnl][0][q] = v[pnl][1][q] = make_uint4(0u, 0u, 0u, 0u);                     \
+            }                                                                                 \
+        }
+#define TRAIL_STASH()                                                                         \
+    for (int pnl = 0; pnl < 2; ++pnl)                                                         \
+        for (int h = 0; h < 2; ++h)                                                           \
+            for (int q = 0; q < 2; ++q) {                                                     \
+                const int r = lrow + 128 * q;                                                 \
+                sm[pnl][h][r][swz(r, lu)] = v[pnl][h][q];                                     \
+            }
+
+__global__ __launch_bounds__(512) void k_trail_bf16x3(float* __restrict__ M, size_t ld, int s, int mp, int kw,
+                                                      int ntr, const unsigned short* __restrict__ phi,
+                                                      const unsigned short* __restrict__ plo) {
+    __shared__ uint4 sm[2][2][BT][BK / 8];  // [row panel, column panel][hi, lo][row][unit]: 64 KB
+    const int ntile = ntr * (ntr + 1) / 2, per = (ntile + kXcds - 1) / kXcds;
+    const int t = (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
+    if (t >= ntile) return;
+    int bi = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+    while (bi * (bi + 1) / 2 > t) --bi;
+    while ((bi + 1) * (bi + 2) / 2 <= t) ++bi;
+    const int bj = t - bi * (bi + 1) / 2;
+    const size_t r0 = (size_t)s + (size_t)BT * bi, c0 = (size_t)s + (size_t)BT * bj;
+    const int tid = threadIdx.x, l = tid & 63, w = wave_of();
+    // loader: thread t moves unit t%4 of rows t/4 and t/4 + 128, hi and lo,
+    // of the row panel and of the column panel: eight 16-byte loads
+    const int lu = tid & 3, lrow = tid >> 2;
+    uint4 v[2][2][2];
+    f32x16 acc[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f32x16{};
+    TRAIL_FETCH(0);
+    TRAIL_STASH();
+    __syncthreads();
+    const int wr = 64 * (w >> 1), wc = 128 * (w & 1), li = l & 31, lk = l >> 5;
+    for (int kc = 0; kc < kw; kc += BK) {
+        const bool more = kc + BK < kw;
+        if (more) { TRAIL_FETCH(kc + BK); }
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {  // two K = 16 substeps per chunk
+            const int u = 2 * ks + lk;
+            bf16x8 ah[2], al[2], bh[4], bl[4];
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const int ra = wr + 32 * b + li;
+                const uint4 xa = sm[0][0][ra][swz(ra, u)], ya = sm[0][1][ra][swz(ra, u)];
+                ah[b] = *reinterpret_cast<const bf16x8*>(&xa);
+                al[b] = *reinterpret_cast<const bf16x8*>(&ya);
+            }
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int rb = wc + 32 * b + li;
+                const uint4 xb = sm[1][0][rb][swz(rb, u)], yb = sm[1][1][rb][swz(rb, u)];
+                bh[b] = *reinterpret_cast<const bf16x8*>(&xb);
+                bl[b] = *reinterpret_cast<const bf16x8*>(&yb);
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+        }
+        if (more) {
+            __syncthreads();  // every wave has read this chunk
+            TRAIL_STASH();
+            __syncthreads();
+        }
+    }
+    // C -= acc over the lower triangle of the tile, rows below mp
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const size_t row = r0 + wr + 32 * i + 8 * (q >> 2) + 4 * lk + (q & 3);
+                const size_t col = c0 + wc + 32 * j + li;
+                if (row >= (size_t)mp || col > row) continue;
+                M[row * ld + col] -= acc[i][j][q];
+            }
+}
+#undef TRAIL_FETCH
+#undef TRAIL_STASH
+
 // FP32 trailing update: operands are the panel columns [pc, pc+kw) of M.
 // LDS rows hold a K-chunk of 16 permuted so that lane group g = l/16 reads
 // k = g, g+4, g+8, g+12 (its four MFMA steps) as one float4.  The next
@@ -1298,13 +1431,20 @@ hipError_t launch_hi_tea_step(const DevJob* jobs, const DevJobState* st, const i
 
 // Rows [rs, mp) x columns [cs, ce) of the trailing matrix minus the outer
 // panel (columns [pc, pc+kw)) times its transpose.
+// Regions whose 256x256 tiles would leave most CUs idle take 128x128 tiles.
+static constexpr int kTrailBigMinTiles = 256;
+
 static void trailing_update(float* M, size_t ld, int s, int mp, int pc, int kw, int mode, const unsigned short* phi,
                             const unsigned short* plo, hipStream_t stream) {
     if (s >= mp) return;
     const int nr = (mp - s + GT - 1) / GT;
     const dim3 g(nr, nr);
+    const int ntr = (mp - s + BT - 1) / BT, ntile = ntr * (ntr + 1) / 2;
     if (mode == SOP_CHOL_F32)
         hipLaunchKernelGGL(k_gemm_f32, g, dim3(256), 0, stream, M, ld, s, mp, pc, kw);
+    else if (ntile >= kTrailBigMinTiles)
+        hipLaunchKernelGGL(k_trail_bf16x3, dim3((ntile + kXcds - 1) / kXcds * kXcds), dim3(512), 0, stream, M, ld, s,
+                           mp, kw, ntr, phi, plo);
     else
         hipLaunchKernelGGL(k_gemm_bf16x3, g, dim3(256), 0, stream, M, ld, s, mp, kw, phi, plo);
 }
