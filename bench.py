@@ -472,6 +472,10 @@ def main():
     ap.add_argument("--c0-steps", type=int, default=100000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--shape", default="",
+                    help="fixed launch shape for the batched mix instead of the autotuner's, e.g. "
+                         "unit_rows=1024,shared_units=1,pair_batch=6,verlet_chunk=8,sort_every=10 "
+                         "(profiler runs: the tuner's device snapshots are many small copies)")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-hi", action="store_true")
     ap.add_argument("--no-capsid", action="store_true")
@@ -509,7 +513,11 @@ def main():
     ids = [eng.add_job(p, sysm, 1000000) for sysm, p in jobs]
     L = eng.block_steps()
     assert PERIOD % L == 0
-    tune = eng.autotune()  # the jobs do not advance
+    if args.shape:  # a fixed launch shape instead of the autotuner's search
+        eng.set_shape(**{k: int(v) for k, v in (kv.split("=") for kv in args.shape.split(","))})
+        tune = {"shape": eng.shape(), "fixed": True}
+    else:
+        tune = eng.autotune()  # the jobs do not advance
     tuned = tune["shape"]["max_blocks"]
     eng.set_tuning(tuned, not args.no_graph)
     print(f"# autotune: {tune}", file=sys.stderr, flush=True)
