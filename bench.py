@@ -275,6 +275,9 @@ def hi_phase(S, n=4096, reps=5, steps=50, cov=True):
         eng = S.Engine(0, 16 << 30)
         eng.set_chol_mode(mode)
         jid = eng.add_job(p, sysm, steps + 10)
+        tune = eng.autotune()  # launch shapes for this job; it does not advance
+        eng.set_tuning(tune["shape"]["max_blocks"], True)
+        out[name]["integrator_shape"] = tune["shape"]
         eng.run_blocks(1)  # warm-up block
         eng.synchronize()
         t0 = time.perf_counter()
@@ -293,6 +296,8 @@ def hi_phase(S, n=4096, reps=5, steps=50, cov=True):
                       hi_capricious=0)
     eng = S.Engine(0, 16 << 30)
     jid = eng.add_job(p, sysm, 10 * steps + 10)
+    tune = eng.autotune()
+    eng.set_tuning(tune["shape"]["max_blocks"], True)
     eng.run_blocks(1)
     eng.synchronize()
     t0 = time.perf_counter()
@@ -301,7 +306,7 @@ def hi_phase(S, n=4096, reps=5, steps=50, cov=True):
     dt = time.perf_counter() - t0
     eps, beta = eng.hi(jid)
     out["tea"] = {"integrator_steps_per_s": (10 * steps // eng.block_steps()) * eng.block_steps() / dt,
-                  "epsilon": eps, "beta": beta}
+                  "epsilon": eps, "beta": beta, "shape": tune["shape"]}
     eng.close()
     return out
 

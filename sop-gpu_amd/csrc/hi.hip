@@ -131,14 +131,55 @@ __device__ __forceinline__ void eps_add(const float4 c1, const float4 c2, float 
 // of a few thousand beads fills the chip.
 constexpr int kHiSplit = 4;
 
+// Ordered all-pairs sums of a workgroup's 16 beads.  Wavefront w owns bead
+// dbase + w: for each chunk of kHiChunk partners its lanes form the NC
+// components of the (bead, partner) terms -- partner j = base + lane + 64u
+// -- into an LDS buffer, and after the chunk's barrier lanes 16c + b of
+// wavefront 0 add component c of bead dbase + b in partner order, four
+// terms per LDS read, while the other wavefronts already form the next
+// chunk into the second buffer.  Terms that the sequential loop skips (the
+// bead itself, partners past n) are -0.0f, which leaves every sum unchanged
+// (x + -0 = x for every x), so each sum is the same sequence of additions
+// as the reference's bead-by-bead loop.  `term(j, v)` fills the wavefront's
+// term for partner j; `acc` enters as the summing lane's starting value and
+// leaves as its sum.
+constexpr int kHiChunk = 256, kHiRow = kHiChunk + 4;
+template <int NC, typename Term>
+__device__ __forceinline__ void hi_ordered_sums(float (*buf)[NC][16][kHiRow], int n, Term term, float& acc) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int sc = lane >> 4, sb = lane & 15;
+    int c = 0;
+    for (int base = 0; base < n; base += kHiChunk, ++c) {
+        float(*tb)[16][kHiRow] = buf[c & 1];
+#pragma unroll
+        for (int u = 0; u < kHiChunk / 64; ++u) {
+            float v[NC];
+            term(base + lane + 64 * u, v);
+#pragma unroll
+            for (int k = 0; k < NC; ++k) tb[k][w][lane + 64 * u] = v[k];
+        }
+        __syncthreads();
+        if (w == 0 && sc < NC) {
+            const float4* __restrict__ row = reinterpret_cast<const float4*>(&tb[sc][sb][0]);
+#pragma unroll 8
+            for (int k = 0; k < kHiChunk / 4; ++k) {
+                const float4 q = row[k];
+                acc += q.x;
+                acc += q.y;
+                acc += q.z;
+                acc += q.w;
+            }
+        }
+    }
+}
+
 // Per-bead sums of the TEA epsilon (bdhitea_kernel.cu:212-249): without
-// lists, for each chunk of 64 partners j every thread of the workgroup
-// forms one (bead, partner) term into LDS and lanes 0-15 add them to their
-// bead's sums in j order (the same bits as the bead-by-bead loop); with
-// lists, wavefront 0 of the tile's first workgroup walks bead i's lists.
+// lists, the ordered chunk sums of hi_ordered_sums (the same bits as the
+// bead-by-bead loop); with lists, wavefront 0 of the tile's first
+// workgroup walks bead i's lists.
 __global__ __launch_bounds__(1024) void k_hi_eps(const DevJob* __restrict__ jobs, const DevJobState* __restrict__ st,
                                                  const int2* __restrict__ tiles, int ntiles) {
-    __shared__ float term[4][16][65];
+    __shared__ __attribute__((aligned(16))) float eterm[2][4][16][kHiRow];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int2 td = tiles[blockIdx.x / kHiSplit];
     const int q = blockIdx.x % kHiSplit;
@@ -166,41 +207,36 @@ __global__ __launch_bounds__(1024) void k_hi_eps(const DevJob* __restrict__ jobs
         J.eps[d] = sum.w;
         return;
     }
-    const int b16 = t & 15, jl = t >> 4;
-    for (int base = 0; base < n; base += 64) {
-        const int i = base + jl;
-        float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (i < n && i != d && d < n) {
-            float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-            eps_add(c, cur[i], a, z);
-            p = z;
-        }
-        term[0][b16][jl] = p.x;
-        term[1][b16][jl] = p.y;
-        term[2][b16][jl] = p.z;
-        term[3][b16][jl] = p.w;
-        __syncthreads();
-        if (t < 16 && d < n) {
-            const int m = min(64, n - base);
-            for (int k = 0; k < m; ++k) {
-                if (base + k == d) continue;
-                sum.x += term[0][b16][k];
-                sum.y += term[1][b16][k];
-                sum.z += term[2][b16][k];
-                sum.w += term[3][b16][k];
-            }
-        }
-        __syncthreads();
-    }
-    if (t < 16 && d < n) {
-        J.ci[d] = sum;
-        J.eps[d] = sum.w;
+    // unlisted: wavefront w serves bead dbase + w (c and d above were the
+    // listed path's); lanes 16c + b of wavefront 0 sum component c of bead dbase + b
+    const int dbase = td.y + 16 * q, dw = dbase + w;
+    const float4 cw = dw < n ? cur[dw] : make_float4(0.f, 0.f, 0.f, 0.f);
+    float acc = 0.f;
+    hi_ordered_sums<4>(eterm, n,
+                       [&](int i, float(&v)[4]) {
+                           float4 z = make_float4(-0.f, -0.f, -0.f, -0.f);
+                           if (i < n && i != dw && dw < n) {
+                               z = make_float4(0.f, 0.f, 0.f, 0.f);
+                               eps_add(cw, cur[i], a, z);
+                           }
+                           v[0] = z.x, v[1] = z.y, v[2] = z.z, v[3] = z.w;
+                       },
+                       acc);
+    // component c of bead dbase + b is in lane 16c + b of wavefront 0
+    const float sy = __shfl(acc, lane + 16), sz = __shfl(acc, lane + 32), sw = __shfl(acc, lane + 48);
+    if (w == 0 && lane < 16 && dbase + lane < n) {
+        J.ci[dbase + lane] = make_float4(acc, sy, sz, sw);
+        J.eps[dbase + lane] = sw;
     }
 }
 
-// TeaUpdater::update (bdhitea.cu:133-176): epsilon and beta' in FP64.
-__global__ void k_hi_beta(const DevJob* __restrict__ jobs, DevJobState* __restrict__ st, int njobs) {
-    const int jb = blockIdx.x * blockDim.x + threadIdx.x;
+// TeaUpdater::update (bdhitea.cu:133-176): epsilon and beta' in FP64.  One
+// wavefront per job: the per-bead values are staged in LDS 256 at a time and
+// added in bead order by every lane (the reference's sequential double sum,
+// the same bits; the lanes' sums are equal).
+__global__ __launch_bounds__(64) void k_hi_beta(const DevJob* __restrict__ jobs, DevJobState* __restrict__ st,
+                                                int njobs) {
+    const int jb = blockIdx.x, lane = threadIdx.x;
     if (jb >= njobs) return;
     const DevJob& J = jobs[jb];
     if (!(J.flags & JF_HI) || (J.flags & JF_HI_EXACT)) return;
@@ -208,8 +244,29 @@ __global__ void k_hi_beta(const DevJob* __restrict__ jobs, DevJobState* __restri
     const long long s = S.step;
     if (!(s < J.end_step && s % J.stride == 0 && s % J.hi_eps_freq == 0)) return;
     const int N = J.n;
+    __shared__ __attribute__((aligned(16))) float chunk[256];
     double epsilon = 0.0;
-    for (int i = 0; i < N; ++i) epsilon += J.eps[i];
+    for (int base = 0; base < N; base += 256) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = base + 64 * u + lane;
+            chunk[64 * u + lane] = i < N ? J.eps[i] : 0.f;
+        }
+        __syncthreads();
+        const int m = min(256, N - base);
+        const float4* __restrict__ c4 = reinterpret_cast<const float4*>(chunk);
+        int k = 0;
+        for (; k + 4 <= m; k += 4) {
+            const float4 q = c4[k >> 2];
+            epsilon += (double)q.x;
+            epsilon += (double)q.y;
+            epsilon += (double)q.z;
+            epsilon += (double)q.w;
+        }
+        for (; k < m; ++k) epsilon += (double)chunk[k];
+        __syncthreads();
+    }
+    if (lane != 0) return;  // the sum is uniform; one lane updates the state
     if (N > 1) epsilon /= 3. * N * (3. * N - 3.);
     if (epsilon > 1.0) {
         if (J.hi_capricious) S.overflow |= 4;
@@ -242,14 +299,13 @@ __device__ __forceinline__ void tea_force(const float4 c1, const float4 c2, cons
 
 // The TEA step (bdhitea_kernel.cu:364-515).  Unlisted: kHiSplit 1024-thread
 // workgroups per 64-bead tile of a TEA job, each serving 16 beads, so a
-// single job of a few thousand beads still fills the chip; for each chunk of
-// 64 partners every thread forms one (bead, partner) term into LDS, then 16
-// lanes of wavefront 0 add them to their beads' sums in partner order, the
-// order of the reference's sequential loop.  Listed: wavefront 0 of the
-// tile's first workgroup walks the lists.
+// single job of a few thousand beads still fills the chip; the terms are
+// added in partner order, the order of the reference's sequential loop, by
+// hi_ordered_sums.  Listed: wavefront 0 of the tile's first workgroup walks
+// the lists.
 __global__ __launch_bounds__(1024) void k_hi_tea(const DevJob* __restrict__ jobs, const DevJobState* __restrict__ st,
                                                  const int2* __restrict__ tiles, int ntiles, int kstep) {
-    __shared__ float term[3][16][65];
+    __shared__ __attribute__((aligned(16))) float tterm[2][3][16][kHiRow];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int2 td = tiles[blockIdx.x / kHiSplit];
     const int q = blockIdx.x % kHiSplit;
@@ -311,27 +367,43 @@ __global__ __launch_bounds__(1024) void k_hi_tea(const DevJob* __restrict__ jobs
         hi_finish(J, d, s, fx, fy, fz, coord);
         return;
     }
-    const int b16 = t & 15, jl = t >> 4;
-    for (int base = 0; base < n; base += 64) {
-        const int i = base + jl;
-        tx = 0.f; ty = 0.f; tz = 0.f;
-        if (i < n && i != d && own) tea_force(coord, cur[i], J.mforce[i], J.rforce[i], ci0, ci1, ci2, a, tx, ty, tz);
-        term[0][b16][jl] = tx;
-        term[1][b16][jl] = ty;
-        term[2][b16][jl] = tz;
-        __syncthreads();
-        if (t < 16 && own) {
-            const int m = min(64, n - base);
-            for (int k = 0; k < m; ++k) {
-                if (base + k == d) continue;
-                fx += term[0][b16][k];
-                fy += term[1][b16][k];
-                fz += term[2][b16][k];
-            }
-        }
-        __syncthreads();
+    // unlisted: wavefront w serves bead dbase + w with that bead's ci
+    // (uniform over the wavefront); lanes 16c + b of wavefront 0 sum
+    // component c of bead dbase + b from its starting value
+    const int dbase = td.y + 16 * q, dw = dbase + w, ds = dbase + (lane & 15);
+    const bool ownw = dw < n;
+    float wi0 = 0.f, wi1 = 0.f, wi2 = 0.f;
+    float4 cw = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (ownw) {
+        const float4 cs = J.ci[dw];
+        const float bb = S.beta * S.beta;
+        wi0 = 1.f / sqrtf(1.f + bb * cs.x) * S.beta;
+        wi1 = 1.f / sqrtf(1.f + bb * cs.y) * S.beta;
+        wi2 = 1.f / sqrtf(1.f + bb * cs.z) * S.beta;
+        cw = cur[dw];
     }
-    if (t < 16 && own) hi_finish(J, d, s, fx, fy, fz, coord);
+    float acc = 0.f;
+    if (w == 0 && lane < 48 && ds < n) {  // the bead's starting value, component lane / 16
+        const float4 cs = J.ci[ds], f0 = J.mforce[ds], df = J.rforce[ds];
+        const float bb = S.beta * S.beta;
+        const int k = lane >> 4;
+        const float cc = k == 0 ? cs.x : (k == 1 ? cs.y : cs.z);
+        const float f = k == 0 ? f0.x : (k == 1 ? f0.y : f0.z), g = k == 0 ? df.x : (k == 1 ? df.y : df.z);
+        acc = f;
+        acc += g * (1.f / sqrtf(1.f + bb * cc));
+    }
+    hi_ordered_sums<3>(tterm, n,
+                       [&](int i, float(&v)[3]) {
+                           float x = -0.f, y = -0.f, z = -0.f;
+                           if (i < n && i != dw && ownw) {
+                               x = 0.f, y = 0.f, z = 0.f;
+                               tea_force(cw, cur[i], J.mforce[i], J.rforce[i], wi0, wi1, wi2, a, x, y, z);
+                           }
+                           v[0] = x, v[1] = y, v[2] = z;
+                       },
+                       acc);
+    const float sy = __shfl(acc, lane + 16), sz = __shfl(acc, lane + 32);
+    if (w == 0 && lane < 16 && ds < n) hi_finish(J, ds, s, acc, sy, sz, cur[ds]);
 }
 
 // ---------------------------------------------------------------------------
@@ -340,58 +412,45 @@ __global__ __launch_bounds__(1024) void k_hi_tea(const DevJob* __restrict__ jobs
 
 // F_i + sum_j D_ij F_j into fdf, summed in bead order.  kHiSplit 1024-thread
 // workgroups per 64-bead tile of the (single) exact-HI job of the launch,
-// 16 beads each: for each chunk of 64 beads j every thread forms one term
-// D_ij F_j (the three row sums, written exactly as the sequential loop
-// writes them) into LDS, then lanes 0-15 add them to their bead's sums in j
-// order, so every sum is the same bits as the bead-by-bead loop.
+// 16 beads each, wavefront w forming bead dbase + w's terms D_ij F_j (the
+// three row sums, written exactly as the sequential loop writes them) and
+// wavefront 0 adding them in j order (hi_ordered_sums), so every sum is the
+// same bits as the bead-by-bead loop.
 __global__ __launch_bounds__(1024) void k_chol_D(const DevJob* __restrict__ jobs, const DevJobState* __restrict__ st,
                                                  const int2* __restrict__ tiles, int ntiles, int kstep) {
-    __shared__ float term[3][16][65];
-    const int t = threadIdx.x, b16 = t & 15, jl = t >> 4;
+    __shared__ __attribute__((aligned(16))) float dterm[2][3][16][kHiRow];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int2 td = tiles[blockIdx.x / kHiSplit];
     const DevJob& J = jobs[td.x];
     if (!(J.flags & JF_HI_EXACT)) return;  // one job per launch: uniform
     const long long s = st[td.x].step + kstep;
     if (s >= J.end_step) return;
     const int n = J.n;
-    // this thread's bead i: one of the workgroup's 16 (the sums' owners are
-    // lanes 0-15 of wavefront 0); jl: its partner in each chunk of 64
-    const int d = td.y + 16 * (blockIdx.x % kHiSplit) + b16;
+    const int dbase = td.y + 16 * (blockIdx.x % kHiSplit), dw = dbase + w, ds = dbase + (lane & 15);
     const float4* __restrict__ cur = J.coord[s & 1];
-    const float4 c = d < n ? cur[d] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 c = dw < n ? cur[dw] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float a = J.hi_a;
-    float fx = 0.f, fy = 0.f, fz = 0.f;
-    if (t < 16 && d < n) {
-        const float4 f0 = J.mforce[d];
-        fx = f0.x, fy = f0.y, fz = f0.z;
+    float acc = 0.f;
+    if (w == 0 && lane < 48 && ds < n) {
+        const float4 f0 = J.mforce[ds];
+        acc = lane < 16 ? f0.x : (lane < 32 ? f0.y : f0.z);
     }
-    for (int base = 0; base < n; base += 64) {
-        const int j = base + jl;
-        float tx = 0.f, ty = 0.f, tz = 0.f;
-        if (j < n && j != d && d < n) {
-            const float4 dr = unit_sep(c, cur[j]);
-            const F6 D = rpy(dr, a);
-            const float4 g = J.mforce[j];
-            tx = D.xx * g.x + D.xy * g.y + D.xz * g.z;
-            ty = D.xy * g.x + D.yy * g.y + D.yz * g.z;
-            tz = D.xz * g.x + D.yz * g.y + D.zz * g.z;
-        }
-        term[0][b16][jl] = tx;
-        term[1][b16][jl] = ty;
-        term[2][b16][jl] = tz;
-        __syncthreads();
-        if (t < 16 && d < n) {
-            const int m = min(64, n - base);
-            for (int k = 0; k < m; ++k) {
-                if (base + k == d) continue;
-                fx += term[0][b16][k];
-                fy += term[1][b16][k];
-                fz += term[2][b16][k];
-            }
-        }
-        __syncthreads();
-    }
-    if (t < 16 && d < n) J.fdf[d] = make_float4(fx, fy, fz, 0.0f);
+    hi_ordered_sums<3>(dterm, n,
+                       [&](int j, float(&v)[3]) {
+                           float tx = -0.f, ty = -0.f, tz = -0.f;
+                           if (j < n && j != dw && dw < n) {
+                               const float4 dr = unit_sep(c, cur[j]);
+                               const F6 D = rpy(dr, a);
+                               const float4 g = J.mforce[j];
+                               tx = D.xx * g.x + D.xy * g.y + D.xz * g.z;
+                               ty = D.xy * g.x + D.yy * g.y + D.yz * g.z;
+                               tz = D.xz * g.x + D.yz * g.y + D.zz * g.z;
+                           }
+                           v[0] = tx, v[1] = ty, v[2] = tz;
+                       },
+                       acc);
+    const float sy = __shfl(acc, lane + 16), sz = __shfl(acc, lane + 32);
+    if (w == 0 && lane < 16 && ds < n) J.fdf[ds] = make_float4(acc, sy, sz, 0.0f);
 }
 
 // D/D_ii lower block triangle into M on refactorisation steps: block
@@ -455,6 +514,9 @@ __global__ __launch_bounds__(256) void k_chol_L(const DevJob* __restrict__ jobs,
         const bool faithful = (J.flags & JF_HI_FAITHFUL) != 0;
         const int r = 3 * d;
         float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+        // unrolled so that several columns' loads are in flight per lane; each
+        // lane still adds its columns in order
+#pragma unroll 4
         for (int c = lane; c < r + 3; c += 64) {
             const int blk = c / 3, b = c - 3 * blk;
             const float gc = g[4 * blk + b];
@@ -614,7 +676,6 @@ __device__ __forceinline__ void inv_sum16(const double (*s)[LDS_ROW], const doub
                                           double (*T)[17], int I, int J) {
     const int l = threadIdx.x & 63, lr = l & 15, lk = l >> 4;
     f64x4 acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 4
     for (int k0 = 0; k0 < 16 * (I - J); k0 += 4)
         acc = __builtin_amdgcn_mfma_f64_16x16x4f64(s[16 * I + lr][16 * J + k0 + lk],
                                                    LiT[(16 * J + lr) * LDT + 16 * J + k0 + lk], acc, 0, 0, 0);
@@ -1418,7 +1479,7 @@ hipError_t launch_hi_tea_update(const DevJob* jobs, DevJobState* st, const int2*
                                 hipStream_t stream) {
     if (ntiles > 0)
         hipLaunchKernelGGL(k_hi_eps, dim3(ntiles * kHiSplit), dim3(1024), 0, stream, jobs, st, tiles, ntiles);
-    hipLaunchKernelGGL(k_hi_beta, dim3((njobs + 63) / 64), dim3(64), 0, stream, jobs, st, njobs);
+    hipLaunchKernelGGL(k_hi_beta, dim3(njobs), dim3(64), 0, stream, jobs, st, njobs);
     return hipGetLastError();
 }
 
