@@ -485,6 +485,8 @@ def main():
     ap.add_argument("--no-hi", action="store_true")
     ap.add_argument("--no-capsid", action="store_true")
     ap.add_argument("--no-c0", action="store_true")
+    ap.add_argument("--c0-only", action="store_true",
+                    help="run only the N = 1,000 single-chain phase (profiling)")
     ap.add_argument("--hi-only", action="store_true",
                     help="run only the exact-HI phase (Cholesky timing and integrator, no covariance check) "
                          "and print its JSON; for profiling the factorisation")
@@ -504,6 +506,9 @@ def main():
 
     if args.hi_only:
         print(json.dumps({"hi_cholesky": hi_phase(S, cov=False)}))
+        return
+    if args.c0_only:
+        print(json.dumps({"c0_single_chain": c0_phase(S, c0_system(S), args.c0_steps)}))
         return
 
     t_build = time.time()

@@ -202,26 +202,20 @@ __device__ __forceinline__ void bonded_terms(const DevJob& J, int nc, int nn, in
     }
 }
 
-// One bead's step: FENE, native and repulsive terms gathered from `cin`
-// (global memory, or the job's coordinates staged in LDS), indentation
-// walls, then noise, the fixed/pulled rules and the Euler update, written to
-// `cout` in global memory.  Hydrodynamic jobs stop after the forces.
-// FO (forces only): the same force terms, written to `cout` (indexed by
-// bead) before the noise, with no state changed -- the parity hook of the
-// force gather runs the kernel the step runs.
-template <int PB, bool FO = false>
-__device__ __forceinline__ void step_bead(const DevJob& J, const DevJobState& S, long long s, int i, int r, int n,
-                                          const float4* __restrict__ cin, float4* __restrict__ cout) {
-    const float4 c = cin[i];
-    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+// The first part of a bead's step, before the repulsive pairs: the extra
+// force record, then FENE and native LJ 12-6 (for sorted rows from the
+// row-ordered copies -- a wavefront's 64 rows read 64 consecutive words per
+// slot -- else in bead order, partner indices 16-bit where the job has a
+// 16-bit copy).  i: bead, r: its Verlet row; c = cin[i].
+template <bool FO>
+__device__ __forceinline__ void step_head(const DevJob& J, int i, int r, int n, const float4* __restrict__ cin,
+                                          const float4 c, float& fx, float& fy, float& fz) {
+    fx = 0.0f, fy = 0.0f, fz = 0.0f;
     if (J.flags & JF_EXTRA_FORCE) {
         const float4 e = J.fextra[i];
         fx = e.x; fy = e.y; fz = e.z;
         if constexpr (!FO) J.fextra[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
-    // FENE and native LJ 12-6: for sorted rows from the row-ordered copies
-    // (a wavefront's 64 rows read 64 consecutive words per slot), else in
-    // bead order, partner indices 16-bit where the job has a 16-bit copy
     if (J.sort_rows) {
         const int cnt = gmem(J.rb_cnt)[r];
         bonded_terms(J, cnt & 0xffff, cnt >> 16, J.pair_ld, gmem(J.rb_cov_j) + r, gmem(J.rb_cov_r0) + r,
@@ -233,14 +227,15 @@ __device__ __forceinline__ void step_bead(const DevJob& J, const DevJobState& S,
         bonded_terms(J, J.cov_cnt[i], J.nat_cnt[i], n, gmem(J.cov_j) + i, gmem(J.cov_r0) + i, gmem(J.nat_j) + i,
                      J.nat_par + i, cin, c, fx, fy, fz);
     }
-    // repulsive LJ over the Verlet list (pairs_kernel.cu:41-56)
-    const int np = gmem(J.pair_cnt)[r];
-    if (J.idx16)
-        pair_loop<PB>(gmem(static_cast<const unsigned short*>(J.pair_j)) + r, np, J.pair_ld, cin, c, J.rc2, J.a2, J.m6,
-                  fx, fy, fz);
-    else
-        pair_loop<PB>(gmem(static_cast<const int*>(J.pair_j)) + r, np, J.pair_ld, cin, c, J.rc2, J.a2, J.m6, fx, fy,
-                  fz);
+}
+
+// The rest of a bead's step, after the repulsive pairs: indentation walls,
+// then (FO) the forces written to cout, or the noise, the fixed/pulled rules
+// and the Euler update written to `cout` in global memory.  Hydrodynamic
+// jobs stop after the forces.
+template <bool FO>
+__device__ __forceinline__ void step_tail(const DevJob& J, const DevJobState& S, long long s, int i, const float4 c,
+                                          float fx, float fy, float fz, float4* __restrict__ cout) {
     // indentation tip and substrate plane (indentation_kernel.cu:71-128)
     if (J.flags & JF_INDENTATION) {
         float4 fin = J.tipF[i];
@@ -310,6 +305,30 @@ __device__ __forceinline__ void step_bead(const DevJob& J, const DevJobState& S,
     const float T0 = reset ? 0.0f : J.T[i];
     J.T[i] = T0 + J.tempNorm * (dx * dx + dy * dy + dz * dz);
     cout[i] = cn;
+}
+
+// One bead's step: FENE, native and repulsive terms gathered from `cin`
+// (global memory, or the job's coordinates staged in LDS), indentation
+// walls, then noise, the fixed/pulled rules and the Euler update, written to
+// `cout` in global memory.  Hydrodynamic jobs stop after the forces.
+// FO (forces only): the same force terms, written to `cout` (indexed by
+// bead) before the noise, with no state changed -- the parity hook of the
+// force gather runs the kernel the step runs.
+template <int PB, bool FO = false>
+__device__ __forceinline__ void step_bead(const DevJob& J, const DevJobState& S, long long s, int i, int r, int n,
+                                          const float4* __restrict__ cin, float4* __restrict__ cout) {
+    const float4 c = cin[i];
+    float fx, fy, fz;
+    step_head<FO>(J, i, r, n, cin, c, fx, fy, fz);
+    // repulsive LJ over the Verlet list (pairs_kernel.cu:41-56)
+    const int np = gmem(J.pair_cnt)[r];
+    if (J.idx16)
+        pair_loop<PB>(gmem(static_cast<const unsigned short*>(J.pair_j)) + r, np, J.pair_ld, cin, c, J.rc2, J.a2, J.m6,
+                      fx, fy, fz);
+    else
+        pair_loop<PB>(gmem(static_cast<const int*>(J.pair_j)) + r, np, J.pair_ld, cin, c, J.rc2, J.a2, J.m6, fx, fy,
+                      fz);
+    step_tail<FO>(J, S, s, i, c, fx, fy, fz, cout);
 }
 
 // FO: forces only, into fout (bead-indexed), for every job of the tiles
