@@ -174,6 +174,9 @@ typedef struct sop_tuning {
     int pair_batch;   /* Verlet entries fetched ahead per lane in the LDS step: 4, 6 or 8 */
     int verlet_chunk; /* 64-entry chunks of a possible-list row in flight in the LDS Verlet build: 4 or 8 */
     int sort_every;   /* Verlet rows re-sorted by length at every k-th Verlet rebuild */
+    int split_rows;   /* 4, 8 or 16: the jobs of at most 5120 beads are stepped with each bead's force terms
+                         spread over a 1024-thread workgroup of that many beads (for a few small jobs; the
+                         same results); 0: one bead per lane */
 } sop_tuning;
 int sop_engine_get_shape(sop_engine* e, sop_tuning* out);
 int sop_engine_set_shape(sop_engine* e, const sop_tuning* shape);

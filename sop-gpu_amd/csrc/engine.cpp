@@ -1504,7 +1504,7 @@ int Engine::compute_forces(int id, float* out4) {
     if (int rc = tiles_for_slot(slot, &nt)) return rc;
     const bool lds = hj.dj.lds_step != 0;
     if (int rc = check(launch_forces(d_jobs_ + slot, d_state_ + slot, d_tiles1_, nt, !lds, d_sunits1_, d_segs1_,
-                                     lds ? nsunits1_ : 0, hj.n, d_scratch4_, stream_),
+                                     lds ? nsunits1_ : 0, hj.n, d_scratch4_, stream_, shape_),
                        "forces"))
         return rc;
     if (int rc = synchronize()) return rc;
@@ -1580,20 +1580,22 @@ int Engine::list_stats(double* avg4, long long* beads) {
 }
 
 Tuning Engine::tuning() const {
-    return Tuning{max_blocks_, shape_.unit_rows, shared_units_ ? 1 : 0, shape_.pair_batch, shape_.verlet_chunk,
-                  shape_.sort_every};
+    return Tuning{max_blocks_,          shape_.unit_rows, shared_units_ ? 1 : 0, shape_.pair_batch,
+                  shape_.verlet_chunk, shape_.sort_every, shape_.split_rows};
 }
 
 int Engine::set_shape(const Tuning& t) {
     const bool ok = t.max_blocks >= 0 && t.unit_rows >= 64 && t.unit_rows <= 1024 && t.unit_rows % 64 == 0 &&
                     (t.pair_batch == 4 || t.pair_batch == 6 || t.pair_batch == 8) &&
-                    (t.verlet_chunk == 4 || t.verlet_chunk == 8) && t.sort_every >= 1 && t.sort_every <= 1000;
+                    (t.verlet_chunk == 4 || t.verlet_chunk == 8) && t.sort_every >= 1 && t.sort_every <= 1000 &&
+                    (t.split_rows == 0 || t.split_rows == 4 || t.split_rows == 8 || t.split_rows == 16);
     if (!ok) return fail(SOP_E_ARG, "launch shape out of range");
     max_blocks_ = t.max_blocks;
     shape_.unit_rows = t.unit_rows;
     shape_.pair_batch = t.pair_batch;
     shape_.verlet_chunk = t.verlet_chunk;
     shape_.sort_every = t.sort_every;
+    shape_.split_rows = t.split_rows;
     shared_units_ = t.shared_units != 0;
     tables_dirty_ = true;  // the unit plan depends on unit_rows and shared_units
     tiles1_n_ = -1;
@@ -1738,6 +1740,10 @@ int Engine::autotune(Tuning* best_out, double* ms_default, double* ms_best, int*
         { Tuning t = best; t.verlet_chunk = best.verlet_chunk == 8 ? 4 : 8; consider(t); }
         for (int v : {10, 40})
             if (v != best.sort_every) { Tuning t = best; t.sort_every = v; consider(t); }
+        // a few small jobs: each bead's terms spread over a workgroup
+        if (ntiles_ <= kSplitMaxTiles)
+            for (int v : {16, 8, 4})
+                if (v != best.split_rows) { Tuning t = best; t.split_rows = v; consider(t); }
     }
     if (any_global_) {
         const int full = (ntiles_ + 3) / 4;

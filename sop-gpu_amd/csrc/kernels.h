@@ -10,20 +10,27 @@ namespace sop {
 // (none of them changes any result): threads of a step unit (its Verlet
 // rows), Verlet entries fetched ahead per lane in the step, 64-entry chunks
 // in flight per row in the Verlet build, and the row-sort cadence (every
-// k-th Verlet rebuild).
+// k-th Verlet rebuild).  split_rows (4, 8 or 16; 0: off) steps the LDS-step
+// jobs by k_step_split instead, one workgroup per split_rows rows with each
+// row's force terms spread over the workgroup (for a few small jobs).
 struct StepShape {
     int unit_rows = kLdsUnit;
     int pair_batch = 6;
     int verlet_chunk = 8;
     int sort_every = kSortEvery;
+    int split_rows = 0;
 };
+// The autotuner tries split_rows only where the launch has at most this
+// many 64-row tiles (the spread pays where one bead per lane leaves CUs idle).
+constexpr int kSplitMaxTiles = 128;
 
 // One Langevin step (local step = state.step + kstep) of every job that owns
 // a tile.  max_blocks caps the persistent grid (0 = one block per 4 tiles).
 // Jobs flagged lds_step are stepped by the LDS kernel over `units` (first
 // segment, segment count) of `segs` (job slot, first row, rows, LDS offset
 // in float4; lds_beads = the largest LDS use of a unit), the others by the
-// tile kernel (launched when any_global is set).
+// tile kernel (launched when any_global is set).  With shape.split_rows set
+// the LDS-step jobs are stepped by k_step_split over the tiles instead.
 hipError_t launch_step(const DevJob* jobs, const DevJobState* st, const int2* tiles, int ntiles, int any_global,
                        const int2* units, const int4* segs, int nunits, int lds_beads, int kstep, int max_blocks,
                        hipStream_t stream, const StepShape& shape);
@@ -56,7 +63,7 @@ hipError_t launch_energies(const DevJob* jobs, const DevJobState* st, int job, i
 // of the given tiles / LDS plan, into out[bead] (one job per launch).
 hipError_t launch_forces(const DevJob* jobs, const DevJobState* st, const int2* tiles, int ntiles, int any_global,
                          const int2* units, const int4* segs, int nunits, int lds_beads, float4* out,
-                         hipStream_t stream);
+                         hipStream_t stream, const StepShape& shape);
 hipError_t launch_rng_probe(uint4* state, int n, int count, int gaussian, float* out, hipStream_t stream);
 // Counts (into *bad, device) the b bit patterns in [lo, hi) times numerators
 // a[0..na) (device) whose short division differs from the IEEE quotient.

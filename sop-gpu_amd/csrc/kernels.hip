@@ -80,6 +80,31 @@ __device__ __forceinline__ __attribute__((address_space(1))) T* gmem(T* p) {
 // fused step
 // ---------------------------------------------------------------------------
 
+// The scalar factors of the three force terms, shared by every step
+// kernel so that each evaluates the same float expressions.
+// Repulsive LJ, for d2 < rc2 (pairs_kernel.cu:41-56): the force on the bead
+// is (x, y, z) * q; `fast` (the job's a2 and rc2 admit the short division)
+// takes div_rn_fast for d2 >= kDivFastMinB.
+__device__ __forceinline__ float pair_q(float d2, float a2, float m6, bool fast) {
+    float q = div_rn_fast(a2, d2);
+    if (!(fast && d2 >= kDivFastMinB)) q = div_ieee(a2, d2);
+    q = q * q;
+    return m6 * q * q;
+}
+// FENE (covalent_kernel.cu:36-48): the force is w * (x, y, z).
+__device__ __forceinline__ float fene_w(float ks, float R2, float x, float y, float z, float r0) {
+    float w = sqrtf(x * x + y * y + z * z);
+    w = w - r0;
+    return ks * w / ((w + r0) * (1.0f - w * w / R2));
+}
+// Native LJ 12-6 (native_kernel.cu:36-49), par = (r0^2, -12 eh / r0^2): the
+// force is (x, y, z) * q.
+__device__ __forceinline__ float native_q(float x, float y, float z, float2 par) {
+    float q = par.x / (x * x + y * y + z * z);
+    const float p6 = q * q * q;
+    return q * (p6 * p6 - p6) * par.y;
+}
+
 // Repulsive LJ over a bead's slot-major Verlet entries, in list order.  The
 // list words are streamed with non-temporal loads (each is used once per
 // step) and fetched one batch of PB ahead: while batch b is computed the
@@ -97,10 +122,7 @@ __device__ __forceinline__ void pair_loop(const __attribute__((address_space(1))
         const float x = (o).x - c.x, y = (o).y - c.y, z = (o).z - c.z; \
         const float d2 = x * x + y * y + z * z;                      \
         if (d2 < rc2) {                                              \
-            float q = div_rn_fast(a2, d2);                           \
-            if (!(fast && d2 >= kDivFastMinB)) q = div_ieee(a2, d2); \
-            q = q * q;                                               \
-            q = m6 * q * q;                                          \
+            const float q = pair_q(d2, a2, m6, fast);                \
             fx += x * q;                                             \
             fy += y * q;                                             \
             fz += z * q;                                             \
@@ -158,9 +180,7 @@ __device__ __forceinline__ void bonded_terms(const DevJob& J, int nc, int nn, in
             const float r0 = cr[(size_t)k * n];
             const float4 o = cin[j];
             const float x = o.x - c.x, y = o.y - c.y, z = o.z - c.z;
-            float w = sqrtf(x * x + y * y + z * z);
-            w = w - r0;
-            w = J.ks * w / ((w + r0) * (1.0f - w * w / J.R2));
+            const float w = fene_w(J.ks, J.R2, x, y, z, r0);
             fx += w * x;
             fy += w * y;
             fz += w * z;
@@ -171,9 +191,7 @@ __device__ __forceinline__ void bonded_terms(const DevJob& J, int nc, int nn, in
 #define SOP_NAT(o, par)                                                 \
 {                                                                   \
     const float x = (o).x - c.x, y = (o).y - c.y, z = (o).z - c.z;     \
-    float q = (par).x / (x * x + y * y + z * z);                    \
-    const float p6 = q * q * q;                                     \
-    q = q * (p6 * p6 - p6) * (par).y;                               \
+    const float q = native_q(x, y, z, par);                         \
     fx += x * q;                                                    \
     fy += y * q;                                                    \
     fz += z * q;                                                    \
@@ -398,6 +416,185 @@ __global__ __launch_bounds__(1024, PB <= 6 ? 8 : 4) void k_step_lds(const DevJob
             t -= span;
         }
     }
+}
+
+// The step with each bead's force terms spread over a workgroup, for jobs
+// whose rows are too few to fill the GPU one row per lane (a single chain of
+// a thousand beads fills 16 wavefronts).  One 1024-thread workgroup steps R
+// Verlet rows of an LDS-step job, with the job's coordinates staged in LDS.
+// A row's terms form one stream in the order the one-lane step adds them:
+// FENE (nc), native (nn), then the Verlet entries (np).  Thirteen
+// wavefronts evaluate the stream in chunks of kSplitK(R) positions (a lane
+// per row and position, 64 / R positions of R rows per wavefront), each
+// term written to LDS as its three components (-0.0 where a pair lies
+// beyond the cut-off or the row has no term at that position: x + (-0.0)
+// is x for every float x).  Three owner wavefronts, one per component, add
+// a chunk's terms in stream order while the next chunk is evaluated
+// (double buffer), starting from the extra force record.  The sums are the
+// one-lane step's sums, rounded the same way term by term, so the step is
+// bit-identical to k_step_lds.  Wavefront 0 then runs the walls, noise and
+// update for the R rows.
+constexpr int kSplitOwners = 3;
+constexpr int kSplitWorkers = 16 - kSplitOwners;
+template <int R>
+constexpr int split_k() { return kSplitWorkers * (64 / R); }
+// row stride of the term buffers: K, or K + 4 where K / 4 is even, so that
+// the owners' 16-byte reads of R rows fall in distinct bank groups
+template <int R>
+constexpr int split_kp() { return (split_k<R>() / 4) % 2 ? split_k<R>() : split_k<R>() + 4; }
+
+// Term p of the stream of Verlet row r (bead i, nc + nn + np terms) as
+// (tx, ty, tz); -0.0 each where nothing is added.
+__device__ __forceinline__ void split_term(const DevJob& J, int p, int r, int i, int nc, int nn, int np,
+                                           const float4* __restrict__ cin, const float4 c, bool fast, float& tx,
+                                           float& ty, float& tz) {
+    tx = -0.0f, ty = -0.0f, tz = -0.0f;
+    if (p < nc) {
+        int j;
+        float r0;
+        if (J.sort_rows) {
+            j = gmem(J.rb_cov_j)[(size_t)p * J.pair_ld + r];
+            r0 = gmem(J.rb_cov_r0)[(size_t)p * J.pair_ld + r];
+        } else {
+            j = J.nat_j16 ? (int)gmem(J.cov_j16)[(size_t)p * J.n + i] : gmem(J.cov_j)[(size_t)p * J.n + i];
+            r0 = gmem(J.cov_r0)[(size_t)p * J.n + i];
+        }
+        const float4 o = cin[j];
+        const float x = o.x - c.x, y = o.y - c.y, z = o.z - c.z;
+        const float w = fene_w(J.ks, J.R2, x, y, z, r0);
+        tx = w * x;
+        ty = w * y;
+        tz = w * z;
+    } else if (p < nc + nn) {
+        const int k = p - nc;
+        int j;
+        float2 par;
+        if (J.sort_rows) {
+            j = gmem(J.rb_nat_j)[(size_t)k * J.pair_ld + r];
+            par = J.rb_nat_par[(size_t)k * J.pair_ld + r];
+        } else {
+            j = J.nat_j16 ? (int)gmem(J.nat_j16)[(size_t)k * J.n + i] : gmem(J.nat_j)[(size_t)k * J.n + i];
+            par = J.nat_par[(size_t)k * J.n + i];
+        }
+        const float4 o = cin[j];
+        const float x = o.x - c.x, y = o.y - c.y, z = o.z - c.z;
+        const float q = native_q(x, y, z, par);
+        tx = x * q;
+        ty = y * q;
+        tz = z * q;
+    } else if (p < nc + nn + np) {
+        const size_t e = (size_t)(p - nc - nn) * J.pair_ld + r;
+        const int j = J.idx16 ? (int)__builtin_nontemporal_load(gmem(static_cast<const unsigned short*>(J.pair_j)) + e)
+                              : __builtin_nontemporal_load(gmem(static_cast<const int*>(J.pair_j)) + e);
+        const float4 o = cin[j];
+        const float x = o.x - c.x, y = o.y - c.y, z = o.z - c.z;
+        const float d2 = x * x + y * y + z * z;
+        if (d2 < J.rc2) {
+            const float q = pair_q(d2, J.a2, J.m6, fast);
+            tx = x * q;
+            ty = y * q;
+            tz = z * q;
+        }
+    }
+}
+
+// Grid: ntiles * (64 / R) workgroups; workgroup b steps rows
+// 64 * tile + R * (b % (64 / R)) .. + R - 1 of tile b / (64 / R) when the
+// tile's job is an LDS-step job (the others are stepped by k_step).
+template <bool FO, int R>
+__global__ __launch_bounds__(1024) void k_step_split(const DevJob* __restrict__ jobs, const DevJobState* __restrict__ st,
+                                                     const int2* __restrict__ tiles, int ntiles, int kstep,
+                                                     float4* __restrict__ fout) {
+    constexpr int K = split_k<R>(), KP = split_kp<R>(), S = 64 / R;
+    extern __shared__ float4 lds_coord[];
+    __shared__ __align__(16) float term[2][3][R][KP];
+    __shared__ int4 rinfo[R];  // bead, nc, nn, np of each row (np = -1: no row)
+    __shared__ float fsum[3][R];
+    const int tile = blockIdx.x / S;
+    if (tile >= ntiles) return;
+    const int2 td = tiles[tile];
+    const DevJob& J = jobs[td.x];
+    if (!J.lds_step) return;
+    const long long s = st[td.x].step + kstep;
+    if (!FO && s >= J.end_step) return;
+    const int n = J.n;
+    const int row0 = td.y + R * (blockIdx.x % S);
+    if (row0 >= n) return;  // every exit above is uniform over the workgroup
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    {
+        const float4* __restrict__ g = J.coord[s & 1];
+        for (int k = tid; k < n; k += 1024) lds_coord[k] = g[k];
+    }
+    if (tid < R) {
+        const int r = row0 + tid;
+        int4 ri = make_int4(0, 0, 0, -1);
+        float e[3] = {0.0f, 0.0f, 0.0f};
+        if (r < n) {
+            const int i = J.perm[r];
+            int nc, nn;
+            if (J.sort_rows) {
+                const int cnt = gmem(J.rb_cnt)[r];
+                nc = cnt & 0xffff;
+                nn = cnt >> 16;
+            } else {
+                nc = J.cov_cnt[i];
+                nn = J.nat_cnt[i];
+            }
+            ri = make_int4(i, nc, nn, gmem(J.pair_cnt)[r]);
+            if (J.flags & JF_EXTRA_FORCE) {
+                const float4 x = J.fextra[i];
+                e[0] = x.x, e[1] = x.y, e[2] = x.z;
+                if constexpr (!FO) J.fextra[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
+        }
+        rinfo[tid] = ri;
+        fsum[0][tid] = e[0], fsum[1][tid] = e[1], fsum[2][tid] = e[2];
+    }
+    __syncthreads();
+    int lmax = 0;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int4 ri = rinfo[q];
+        lmax = max(lmax, ri.y + ri.z + ri.w);
+    }
+    const int nch = (lmax + K - 1) / K;
+    const bool fast = div_fast_numerator(J.a2) && J.rc2 <= kDivFastMaxB;
+    const bool owner = w < kSplitOwners;
+    // a worker evaluates row q, position pos of every chunk
+    const int q = lane % R, pos = (w - kSplitOwners) * S + lane / R;
+    const int4 ri = rinfo[q];
+    const int len = ri.w < 0 ? 0 : ri.y + ri.z + ri.w;
+    const float4 c = lds_coord[ri.x];
+    float acc = (owner && lane < R) ? fsum[w][lane] : 0.0f;
+    for (int ch = 0; ch <= nch; ++ch) {
+        if (!owner && ch < nch) {
+            const int p = ch * K + pos;
+            float tx = -0.0f, ty = -0.0f, tz = -0.0f;
+            if (p < len) split_term(J, p, row0 + q, ri.x, ri.y, ri.z, ri.w, lds_coord, c, fast, tx, ty, tz);
+            term[ch & 1][0][q][pos] = tx;
+            term[ch & 1][1][q][pos] = ty;
+            term[ch & 1][2][q][pos] = tz;
+        }
+        if (owner && ch > 0 && lane < R) {
+            // chunk ch - 1 in stream order; positions up to the next multiple
+            // of 4 past the longest row hold -0.0
+            const float* __restrict__ t = &term[(ch - 1) & 1][w][lane][0];
+            const int m = min(K, lmax - (ch - 1) * K);
+            for (int k = 0; k < m; k += 4) {
+                const float4 v = *reinterpret_cast<const float4*>(t + k);
+                acc += v.x;
+                acc += v.y;
+                acc += v.z;
+                acc += v.w;
+            }
+        }
+        __syncthreads();
+    }
+    if (owner && lane < R) fsum[w][lane] = acc;
+    __syncthreads();
+    if (w == 0 && lane < R && ri.w >= 0)
+        step_tail<FO>(J, st[td.x], s, ri.x, c, fsum[0][lane], fsum[1][lane], fsum[2][lane],
+                      FO ? fout : J.coord[(s + 1) & 1]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1392,6 +1589,22 @@ static int grid_for(int ntiles, int max_blocks) {
     return b < 1 ? 1 : b;
 }
 
+// k_step_split over every tile (its workgroups of other jobs exit at once).
+template <bool FO>
+static void launch_split(int R, const DevJob* jobs, const DevJobState* st, const int2* tiles, int ntiles,
+                         int lds_beads, int kstep, float4* out, hipStream_t stream) {
+    const size_t lds = (size_t)lds_beads * sizeof(float4);
+    if (R == 4)
+        hipLaunchKernelGGL((k_step_split<FO, 4>), dim3(ntiles * 16), dim3(1024), lds, stream, jobs, st, tiles, ntiles,
+                           kstep, out);
+    else if (R == 8)
+        hipLaunchKernelGGL((k_step_split<FO, 8>), dim3(ntiles * 8), dim3(1024), lds, stream, jobs, st, tiles, ntiles,
+                           kstep, out);
+    else
+        hipLaunchKernelGGL((k_step_split<FO, 16>), dim3(ntiles * 4), dim3(1024), lds, stream, jobs, st, tiles, ntiles,
+                           kstep, out);
+}
+
 hipError_t launch_step(const DevJob* jobs, const DevJobState* st, const int2* tiles, int ntiles, int any_global,
                        const int2* units, const int4* segs, int nunits, int lds_beads, int kstep, int max_blocks,
                        hipStream_t stream, const StepShape& shape) {
@@ -1399,7 +1612,9 @@ hipError_t launch_step(const DevJob* jobs, const DevJobState* st, const int2* ti
     if (any_global)
         hipLaunchKernelGGL(k_step<false>, dim3(grid_for(ntiles, max_blocks)), dim3(256), 0, stream, jobs, st, tiles,
                            ntiles, kstep, nullptr);
-    if (nunits > 0) {
+    if (nunits > 0 && shape.split_rows) {
+        launch_split<false>(shape.split_rows, jobs, st, tiles, ntiles, lds_beads, kstep, nullptr, stream);
+    } else if (nunits > 0) {
         const dim3 g(nunits), b(shape.unit_rows);
         const size_t lds = (size_t)lds_beads * sizeof(float4);
         if (shape.pair_batch == 4)
@@ -1414,12 +1629,14 @@ hipError_t launch_step(const DevJob* jobs, const DevJobState* st, const int2* ti
 
 hipError_t launch_forces(const DevJob* jobs, const DevJobState* st, const int2* tiles, int ntiles, int any_global,
                          const int2* units, const int4* segs, int nunits, int lds_beads, float4* out,
-                         hipStream_t stream) {
+                         hipStream_t stream, const StepShape& shape) {
     if (ntiles <= 0) return hipSuccess;
     if (any_global)
         hipLaunchKernelGGL(k_step<true>, dim3(grid_for(ntiles, 0)), dim3(256), 0, stream, jobs, st, tiles, ntiles, 0,
                            out);
-    if (nunits > 0)
+    if (nunits > 0 && shape.split_rows)
+        launch_split<true>(shape.split_rows, jobs, st, tiles, ntiles, lds_beads, 0, out, stream);
+    else if (nunits > 0)
         hipLaunchKernelGGL((k_step_lds<true, 6>), dim3(nunits), dim3(kLdsUnit), (size_t)lds_beads * sizeof(float4),
                            stream, jobs, st, units, segs, nunits, 0, out);
     return hipGetLastError();
