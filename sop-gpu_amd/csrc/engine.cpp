@@ -1065,9 +1065,10 @@ int Engine::launch_block(int slot_only) {
             slot_only >= 0
                 ? launch_possible(jobs, st, tiles, nt, nj, cell_words_of(jobs_[slot_only].dj),
                                   !jobs_[slot_only].dj.lds_step, d_sunits1_, d_segs1_,
-                                  jobs_[slot_only].dj.lds_step ? nsunits1_ : 0, jobs_[slot_only].n, 0, stream_)
+                                  jobs_[slot_only].dj.lds_step ? nsunits1_ : 0, jobs_[slot_only].n, 0, stream_,
+                                  shape_)
                 : launch_possible(jobs, st, tiles, nt, nj, cell_words_, any_global_, d_sunits_, d_segs_, nsunits_,
-                                  lds_beads_, 0, stream_);
+                                  lds_beads_, 0, stream_, shape_);
         if (int rc = check(e, "possible pairs")) return rc;
     }
     if (due & 2) {
@@ -1204,7 +1205,7 @@ int Engine::stage_possible() {
     if (int rc = sync_tables()) return rc;
     return check(launch_possible(d_jobs_, d_state_, d_tiles_, ntiles_, (int)jobs_.size(), cell_words_, any_global_,
                                  d_sunits_,
-                                 d_segs_, nsunits_, lds_beads_, 0, stream_),
+                                 d_segs_, nsunits_, lds_beads_, 0, stream_, shape_),
                  "possible pairs");
 }
 int Engine::stage_verlet() {
@@ -1546,7 +1547,8 @@ int Engine::rebuild_lists(int id, int mask) {
         if (int rc = check(launch_possible(d_jobs_ + slot, d_state_ + slot, d_tiles1_, nt, 1,
                                            cell_words_of(jobs_[slot].dj), !jobs_[slot].dj.lds_step, d_sunits1_,
                                            d_segs1_,
-                                           jobs_[slot].dj.lds_step ? nsunits1_ : 0, jobs_[slot].n, 1, stream_),
+                                           jobs_[slot].dj.lds_step ? nsunits1_ : 0, jobs_[slot].n, 1, stream_,
+                                           shape_),
                            "possible pairs"))
             return rc;
     if (mask & 2)

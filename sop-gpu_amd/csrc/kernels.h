@@ -43,7 +43,7 @@ hipError_t launch_step(const DevJob* jobs, const DevJobState* st, const int2* ti
 // outside that plan (the tile kernel is launched only then).
 hipError_t launch_possible(const DevJob* jobs, DevJobState* st, const int2* tiles, int ntiles, int njobs,
                            int cell_words, int any_global, const int2* sunits, const int4* segs, int nsunits,
-                           int lds_beads, int force, hipStream_t stream);
+                           int lds_beads, int force, hipStream_t stream, const StepShape& shape);
 // Largest cell_words launch_possible accepts (the LDS bitmap of one wavefront).
 constexpr int kMaxCellWords = 8192;
 // any_idx32: some job of the launch stores its lists as int32 (n >= 65536).

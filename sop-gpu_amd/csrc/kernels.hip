@@ -155,14 +155,16 @@ __device__ __forceinline__ void pair_loop(const __attribute__((address_space(1))
 // langevin_kernel.cu:43-50): a fixed bead's force is zeroed, a pulled bead's
 // gets the cantilever force.  The one-byte flag is read for every bead, the
 // 16-byte force record only for pulled beads.
-__device__ __forceinline__ void ext_rules(const DevJob& J, int i, float& fx, float& fy, float& fz) {
-    const int f = J.bflag[i];
+__device__ __forceinline__ void ext_rules_flag(const DevJob& J, int i, int f, float& fx, float& fy, float& fz) {
     if (f == 1) {
         fx = 0.0f; fy = 0.0f; fz = 0.0f;
     } else if (f == 2) {
         const float4 e = J.ext[i];
         fx += e.x; fy += e.y; fz += e.z;
     }
+}
+__device__ __forceinline__ void ext_rules(const DevJob& J, int i, float& fx, float& fy, float& fz) {
+    ext_rules_flag(J, i, J.bflag[i], fx, fy, fz);
 }
 
 // FENE (covalent_kernel.cu:36-48) and native LJ 12-6 (native_kernel.cu:36-49)
@@ -247,13 +249,30 @@ __device__ __forceinline__ void step_head(const DevJob& J, int i, int r, int n, 
     }
 }
 
+// The per-bead state the tail reads (RNG state, temperature accumulator,
+// fixed/pulled flag), loaded ahead by a caller with latency to hide.
+struct TailPre {
+    uint4 rng;
+    float T;
+    int flag;
+};
+__device__ __forceinline__ TailPre tail_prefetch(const DevJob& J, int i) {
+    TailPre t;
+    t.rng = J.rng[i];
+    t.T = J.T[i];
+    t.flag = (J.flags & JF_PULLING) ? (int)J.bflag[i] : 0;
+    return t;
+}
+
 // The rest of a bead's step, after the repulsive pairs: indentation walls,
 // then (FO) the forces written to cout, or the noise, the fixed/pulled rules
 // and the Euler update written to `cout` in global memory.  Hydrodynamic
-// jobs stop after the forces.
+// jobs stop after the forces.  pre: the state loaded ahead (null: loaded
+// here).
 template <bool FO>
 __device__ __forceinline__ void step_tail(const DevJob& J, const DevJobState& S, long long s, int i, const float4 c,
-                                          float fx, float fy, float fz, float4* __restrict__ cout) {
+                                          float fx, float fy, float fz, float4* __restrict__ cout,
+                                          const TailPre* pre = nullptr) {
     // indentation tip and substrate plane (indentation_kernel.cu:71-128)
     if (J.flags & JF_INDENTATION) {
         float4 fin = J.tipF[i];
@@ -291,18 +310,18 @@ __device__ __forceinline__ void step_tail(const DevJob& J, const DevJobState& S,
     // (integrateTea_prepare, bdhitea_kernel.cu:33-65); the update itself
     // runs in hi.hip once every bead's forces are known
     if (J.flags & JF_HI) {
-        uint4 rs = J.rng[i];
+        uint4 rs = pre ? pre->rng : J.rng[i];
         const float var = S.var;
         const float3 g = gauss3(rs);
         J.rng[i] = rs;
         J.rforce[i] = make_float4(g.x * var, g.y * var, g.z * var, 0.0f);
-        if (J.flags & JF_PULLING) ext_rules(J, i, fx, fy, fz);
+        if (J.flags & JF_PULLING) ext_rules_flag(J, i, pre ? pre->flag : (int)J.bflag[i], fx, fy, fz);
         J.mforce[i] = make_float4(fx, fy, fz, 0.0f);
         return;
     }
     // Langevin step (langevin_kernel.cu:24-56)
     if (!(J.flags & JF_MINIMIZATION)) {
-        uint4 rs = J.rng[i];
+        uint4 rs = pre ? pre->rng : J.rng[i];
         const float var = S.var;
         const float3 g = gauss3(rs);
         J.rng[i] = rs;
@@ -310,7 +329,7 @@ __device__ __forceinline__ void step_tail(const DevJob& J, const DevJobState& S,
         fy += g.y * var;
         fz += g.z * var;
     }
-    if (J.flags & JF_PULLING) ext_rules(J, i, fx, fy, fz);
+    if (J.flags & JF_PULLING) ext_rules_flag(J, i, pre ? pre->flag : (int)J.bflag[i], fx, fy, fz);
     const float m = J.hOverZeta;
     const float dx = m * fx, dy = m * fy, dz = m * fz;
     float4 cn = c;
@@ -320,7 +339,7 @@ __device__ __forceinline__ void step_tail(const DevJob& J, const DevJobState& S,
     // the temperature accumulator is cleared at updater passes that fall
     // on a multiple of nav (output_manager.cu:124-126)
     const bool reset = (s % J.stride == 0) && (s % J.nav == 0);
-    const float T0 = reset ? 0.0f : J.T[i];
+    const float T0 = reset ? 0.0f : (pre ? pre->T : J.T[i]);
     J.T[i] = T0 + J.tempNorm * (dx * dx + dy * dy + dz * dz);
     cout[i] = cn;
 }
@@ -443,51 +462,65 @@ constexpr int split_k() { return kSplitWorkers * (64 / R); }
 template <int R>
 constexpr int split_kp() { return (split_k<R>() / 4) % 2 ? split_k<R>() : split_k<R>() + 4; }
 
-// Term p of the stream of Verlet row r (bead i, nc + nn + np terms) as
-// (tx, ty, tz); -0.0 each where nothing is added.
-__device__ __forceinline__ void split_term(const DevJob& J, int p, int r, int i, int nc, int nn, int np,
-                                           const float4* __restrict__ cin, const float4 c, bool fast, float& tx,
-                                           float& ty, float& tz) {
-    tx = -0.0f, ty = -0.0f, tz = -0.0f;
+// Term p of the stream of Verlet row r (bead i, nc + nn + np terms), in two
+// parts so that the list reads of a chunk go out ahead of its evaluation:
+// split_fetch reads the partner and the pair's parameters (type 0 FENE with
+// par.x = r0, 1 native, 2 repulsive, 3 no term), split_eval gathers the
+// partner from the staged coordinates and forms (tx, ty, tz), -0.0 each where
+// nothing is added.
+struct SplitSrc {
+    int j, type;
+    float2 par;
+};
+__device__ __forceinline__ SplitSrc split_fetch(const DevJob& J, int p, int r, int i, int nc, int nn, int np) {
+    SplitSrc f;
+    f.j = 0;
+    f.type = 3;
+    f.par = make_float2(0.0f, 0.0f);
     if (p < nc) {
-        int j;
-        float r0;
+        f.type = 0;
         if (J.sort_rows) {
-            j = gmem(J.rb_cov_j)[(size_t)p * J.pair_ld + r];
-            r0 = gmem(J.rb_cov_r0)[(size_t)p * J.pair_ld + r];
+            f.j = gmem(J.rb_cov_j)[(size_t)p * J.pair_ld + r];
+            f.par.x = gmem(J.rb_cov_r0)[(size_t)p * J.pair_ld + r];
         } else {
-            j = J.nat_j16 ? (int)gmem(J.cov_j16)[(size_t)p * J.n + i] : gmem(J.cov_j)[(size_t)p * J.n + i];
-            r0 = gmem(J.cov_r0)[(size_t)p * J.n + i];
+            f.j = J.nat_j16 ? (int)gmem(J.cov_j16)[(size_t)p * J.n + i] : gmem(J.cov_j)[(size_t)p * J.n + i];
+            f.par.x = gmem(J.cov_r0)[(size_t)p * J.n + i];
         }
-        const float4 o = cin[j];
-        const float x = o.x - c.x, y = o.y - c.y, z = o.z - c.z;
-        const float w = fene_w(J.ks, J.R2, x, y, z, r0);
+    } else if (p < nc + nn) {
+        const int k = p - nc;
+        f.type = 1;
+        if (J.sort_rows) {
+            f.j = gmem(J.rb_nat_j)[(size_t)k * J.pair_ld + r];
+            f.par = J.rb_nat_par[(size_t)k * J.pair_ld + r];
+        } else {
+            f.j = J.nat_j16 ? (int)gmem(J.nat_j16)[(size_t)k * J.n + i] : gmem(J.nat_j)[(size_t)k * J.n + i];
+            f.par = J.nat_par[(size_t)k * J.n + i];
+        }
+    } else if (p < nc + nn + np) {
+        const size_t e = (size_t)(p - nc - nn) * J.pair_ld + r;
+        f.type = 2;
+        f.j = J.idx16 ? (int)__builtin_nontemporal_load(gmem(static_cast<const unsigned short*>(J.pair_j)) + e)
+                      : __builtin_nontemporal_load(gmem(static_cast<const int*>(J.pair_j)) + e);
+    }
+    return f;
+}
+__device__ __forceinline__ void split_eval(const DevJob& J, const SplitSrc& f, const float4* __restrict__ cin,
+                                           const float4 c, bool fast, float& tx, float& ty, float& tz) {
+    tx = -0.0f, ty = -0.0f, tz = -0.0f;
+    if (f.type == 3) return;
+    const float4 o = cin[f.j];
+    const float x = o.x - c.x, y = o.y - c.y, z = o.z - c.z;
+    if (f.type == 0) {
+        const float w = fene_w(J.ks, J.R2, x, y, z, f.par.x);
         tx = w * x;
         ty = w * y;
         tz = w * z;
-    } else if (p < nc + nn) {
-        const int k = p - nc;
-        int j;
-        float2 par;
-        if (J.sort_rows) {
-            j = gmem(J.rb_nat_j)[(size_t)k * J.pair_ld + r];
-            par = J.rb_nat_par[(size_t)k * J.pair_ld + r];
-        } else {
-            j = J.nat_j16 ? (int)gmem(J.nat_j16)[(size_t)k * J.n + i] : gmem(J.nat_j)[(size_t)k * J.n + i];
-            par = J.nat_par[(size_t)k * J.n + i];
-        }
-        const float4 o = cin[j];
-        const float x = o.x - c.x, y = o.y - c.y, z = o.z - c.z;
-        const float q = native_q(x, y, z, par);
+    } else if (f.type == 1) {
+        const float q = native_q(x, y, z, f.par);
         tx = x * q;
         ty = y * q;
         tz = z * q;
-    } else if (p < nc + nn + np) {
-        const size_t e = (size_t)(p - nc - nn) * J.pair_ld + r;
-        const int j = J.idx16 ? (int)__builtin_nontemporal_load(gmem(static_cast<const unsigned short*>(J.pair_j)) + e)
-                              : __builtin_nontemporal_load(gmem(static_cast<const int*>(J.pair_j)) + e);
-        const float4 o = cin[j];
-        const float x = o.x - c.x, y = o.y - c.y, z = o.z - c.z;
+    } else {
         const float d2 = x * x + y * y + z * z;
         if (d2 < J.rc2) {
             const float q = pair_q(d2, J.a2, J.m6, fast);
@@ -500,7 +533,11 @@ __device__ __forceinline__ void split_term(const DevJob& J, int p, int r, int i,
 
 // Grid: ntiles * (64 / R) workgroups; workgroup b steps rows
 // 64 * tile + R * (b % (64 / R)) .. + R - 1 of tile b / (64 / R) when the
-// tile's job is an LDS-step job (the others are stepped by k_step).
+// tile's job is an LDS-step job (the others are stepped by k_step).  Every
+// lane works for row lane % R: it reads that row's bead and counts itself,
+// and a worker reads its first two chunks' list entries, while the
+// coordinates are staged; wavefront 0 also reads the extra force record and
+// the state its tail needs.
 template <bool FO, int R>
 __global__ __launch_bounds__(1024) void k_step_split(const DevJob* __restrict__ jobs, const DevJobState* __restrict__ st,
                                                      const int2* __restrict__ tiles, int ntiles, int kstep,
@@ -508,7 +545,6 @@ __global__ __launch_bounds__(1024) void k_step_split(const DevJob* __restrict__ 
     constexpr int K = split_k<R>(), KP = split_kp<R>(), S = 64 / R;
     extern __shared__ float4 lds_coord[];
     __shared__ __align__(16) float term[2][3][R][KP];
-    __shared__ int4 rinfo[R];  // bead, nc, nn, np of each row (np = -1: no row)
     __shared__ float fsum[3][R];
     const int tile = blockIdx.x / S;
     if (tile >= ntiles) return;
@@ -525,52 +561,57 @@ __global__ __launch_bounds__(1024) void k_step_split(const DevJob* __restrict__ 
         const float4* __restrict__ g = J.coord[s & 1];
         for (int k = tid; k < n; k += 1024) lds_coord[k] = g[k];
     }
-    if (tid < R) {
-        const int r = row0 + tid;
-        int4 ri = make_int4(0, 0, 0, -1);
-        float e[3] = {0.0f, 0.0f, 0.0f};
-        if (r < n) {
-            const int i = J.perm[r];
-            int nc, nn;
-            if (J.sort_rows) {
-                const int cnt = gmem(J.rb_cnt)[r];
-                nc = cnt & 0xffff;
-                nn = cnt >> 16;
-            } else {
-                nc = J.cov_cnt[i];
-                nn = J.nat_cnt[i];
-            }
-            ri = make_int4(i, nc, nn, gmem(J.pair_cnt)[r]);
-            if (J.flags & JF_EXTRA_FORCE) {
-                const float4 x = J.fextra[i];
-                e[0] = x.x, e[1] = x.y, e[2] = x.z;
-                if constexpr (!FO) J.fextra[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            }
-        }
-        rinfo[tid] = ri;
-        fsum[0][tid] = e[0], fsum[1][tid] = e[1], fsum[2][tid] = e[2];
-    }
-    __syncthreads();
-    int lmax = 0;
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-        const int4 ri = rinfo[q];
-        lmax = max(lmax, ri.y + ri.z + ri.w);
-    }
-    const int nch = (lmax + K - 1) / K;
-    const bool fast = div_fast_numerator(J.a2) && J.rc2 <= kDivFastMaxB;
     const bool owner = w < kSplitOwners;
-    // a worker evaluates row q, position pos of every chunk
-    const int q = lane % R, pos = (w - kSplitOwners) * S + lane / R;
-    const int4 ri = rinfo[q];
-    const int len = ri.w < 0 ? 0 : ri.y + ri.z + ri.w;
-    const float4 c = lds_coord[ri.x];
-    float acc = (owner && lane < R) ? fsum[w][lane] : 0.0f;
+    // this lane's row (loads from a clamped row where the tile ends early)
+    const int q = lane % R, r = row0 + q, rc = min(r, n - 1);
+    const int i = gmem(J.perm)[rc];
+    int nc, nn;
+    if (J.sort_rows) {
+        const int cnt = gmem(J.rb_cnt)[rc];
+        nc = cnt & 0xffff;
+        nn = cnt >> 16;
+    } else {
+        nc = gmem(J.cov_cnt)[i];
+        nn = gmem(J.nat_cnt)[i];
+    }
+    const int np = gmem(J.pair_cnt)[rc];
+    const int len = r < n ? nc + nn + np : 0;
+    // the longest row: lanes 0 .. R-1 of each wavefront hold every row
+    int lmax = len;
+#pragma unroll
+    for (int o = R / 2; o > 0; o >>= 1) lmax = max(lmax, __shfl_xor(lmax, o));
+    lmax = __shfl(lmax, 0);
+    const int nch = (lmax + K - 1) / K;
+    // a worker evaluates position pos of every chunk
+    const int pos = (w - kSplitOwners) * S + lane / R;
+    SplitSrc f0, f1;
+    if (!owner) {
+        f0 = split_fetch(J, pos < len ? pos : INT_MAX, r, i, nc, nn, np);
+        f1 = split_fetch(J, K + pos < len ? K + pos : INT_MAX, r, i, nc, nn, np);
+    }
+    float acc = 0.0f;
+    TailPre pre;
+    if (owner && lane < R && r < n) {
+        if (J.flags & JF_EXTRA_FORCE) {
+            const float4 e = J.fextra[i];
+            acc = w == 0 ? e.x : w == 1 ? e.y : e.z;
+        }
+        if (w == 0 && !FO) pre = tail_prefetch(J, i);
+    }
+    __syncthreads();  // coordinates staged; extra forces read by all three owners
+    if constexpr (!FO) {
+        if (w == 0 && lane < R && r < n && (J.flags & JF_EXTRA_FORCE))
+            J.fextra[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    const bool fast = div_fast_numerator(J.a2) && J.rc2 <= kDivFastMaxB;
+    const float4 c = lds_coord[i];
     for (int ch = 0; ch <= nch; ++ch) {
         if (!owner && ch < nch) {
-            const int p = ch * K + pos;
-            float tx = -0.0f, ty = -0.0f, tz = -0.0f;
-            if (p < len) split_term(J, p, row0 + q, ri.x, ri.y, ri.z, ri.w, lds_coord, c, fast, tx, ty, tz);
+            float tx, ty, tz;
+            split_eval(J, f0, lds_coord, c, fast, tx, ty, tz);
+            f0 = f1;
+            const int p = (ch + 2) * K + pos;
+            if (ch + 2 < nch) f1 = split_fetch(J, p < len ? p : INT_MAX, r, i, nc, nn, np);
             term[ch & 1][0][q][pos] = tx;
             term[ch & 1][1][q][pos] = ty;
             term[ch & 1][2][q][pos] = tz;
@@ -592,9 +633,9 @@ __global__ __launch_bounds__(1024) void k_step_split(const DevJob* __restrict__ 
     }
     if (owner && lane < R) fsum[w][lane] = acc;
     __syncthreads();
-    if (w == 0 && lane < R && ri.w >= 0)
-        step_tail<FO>(J, st[td.x], s, ri.x, c, fsum[0][lane], fsum[1][lane], fsum[2][lane],
-                      FO ? fout : J.coord[(s + 1) & 1]);
+    if (w == 0 && lane < R && r < n)
+        step_tail<FO>(J, st[td.x], s, i, c, fsum[0][lane], fsum[1][lane], fsum[2][lane],
+                      FO ? fout : J.coord[(s + 1) & 1], FO ? nullptr : &pre);
 }
 
 // ---------------------------------------------------------------------------
@@ -687,6 +728,100 @@ __global__ __launch_bounds__(256) void k_possible(const DevJob* __restrict__ job
 // per chunk of candidates in scalar registers, and the accepted count is a
 // scalar; the candidates j come 64 per chunk from LDS in ascending order.
 // The same list in the same order as k_possible.
+// Possible-list rows r0 .. r1-1 of job J (r1 > r0) by one wavefront, from
+// the job's coordinates `cur` staged in LDS: each row scans the job's beads
+// in chunks of 64, ballot-compacts the accepted ones in ascending order, and
+// drops the bead's exclusions and itself.  Overflow sets bit 0 of *ovf.
+__device__ __forceinline__ void possible_rows(const DevJob& J, const float4* __restrict__ cur, int r0, int r1,
+                                              int* ovf) {
+    const int lane = threadIdx.x & 63;
+    const int n = J.n;
+    const float d2max = J.pp_d2;
+    const int cap = J.pp_cap, ecap = J.excl_cap;
+    const auto ecnt = gmem(J.excl_cnt);
+    const auto ej = gmem(J.excl_j);
+    const auto pp = gmem(static_cast<unsigned short*>(J.pp_j));
+    const auto pcnt = gmem(J.pp_cnt);
+    // bead i's exclusion count and its first 64 exclusions (lane k holds
+    // the k-th), loaded one bead ahead from clamped addresses
+    auto load_ex = [&](int i, int& ne, int& ev) {
+        ne = ecnt[i];
+        ev = ej[(size_t)min(lane, ecap - 1) * n + i];
+    };
+    int ne_n, ev_n;
+    load_ex(r0, ne_n, ev_n);
+    for (int i = r0; i < r1; ++i) {
+        const int ne = ne_n;
+        int ev = lane < ne ? ev_n : INT_MAX;
+        load_ex(min(i + 1, r1 - 1), ne_n, ev_n);
+        const float4 ci = cur[i];
+        const auto row = pp + (size_t)i * J.pp_ld;
+        int eg = 0, pe = 0;  // exclusion group (of 64) and index within it
+        int e_next = ne > 0 ? __builtin_amdgcn_readlane(ev, 0) : INT_MAX;
+        int self = i;  // the bead itself, until its chunk is passed
+        int cnt = 0;
+        // One chunk of 64 candidates from `base` with coordinates c; TAIL:
+        // the chunk may run past the job's end.  The exclusion mask is
+        // formed only for chunks holding an excluded partner or the bead.
+        auto chunk = [&](int base, const float4 c, bool tail) {
+            const int lim = base + 64;
+            const float x = c.x - ci.x, y = c.y - ci.y, z = c.z - ci.z;
+            bool acc = x * x + y * y + z * z <= d2max;
+            if (tail) acc = acc & (base + lane < n);
+            unsigned long long m = __builtin_amdgcn_ballot_w64(acc);
+            if (min(e_next, self) < lim) {
+                unsigned long long drop = 0ull;
+                while (e_next < lim) {
+                    drop |= 1ull << (e_next - base);
+                    if (++pe == 64) {
+                        eg += 64;
+                        pe = 0;
+                        ev = eg + lane < ne ? ej[(size_t)(eg + lane) * n + i] : INT_MAX;
+                    }
+                    e_next = eg + pe < ne ? __builtin_amdgcn_readlane(ev, pe) : INT_MAX;
+                }
+                if (self < lim) {
+                    drop |= 1ull << (self - base);
+                    self = INT_MAX;
+                }
+                m &= ~drop;
+                acc = acc && !((drop >> lane) & 1ull);
+            }
+            if (m) {
+                if (acc) {
+                    const unsigned pos = __builtin_amdgcn_mbcnt_hi(
+                        (unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, (unsigned)cnt));
+                    if ((int)pos < cap)
+                        *reinterpret_cast<__attribute__((address_space(1))) unsigned short*>(
+                            reinterpret_cast<__attribute__((address_space(1))) char*>(row) + (pos << 1)) =
+                            (unsigned short)(base + lane);
+                }
+                cnt += __popcll(m);
+            }
+        };
+        int b4 = 0;
+        for (; b4 + 256 <= n; b4 += 256) {  // four whole chunks, their LDS reads in flight together
+            float4 c[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) c[v] = cur[b4 + 64 * v + lane];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) chunk(b4 + 64 * v, c[v], false);
+        }
+        if (b4 < n) {  // the last one to four chunks
+            float4 c[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) c[v] = cur[min(b4 + 64 * v + lane, n - 1)];
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                if (b4 + 64 * v < n) chunk(b4 + 64 * v, c[v], true);
+        }
+        if (lane == 0) {
+            pcnt[i] = cnt < cap ? cnt : cap;
+            if (cnt > cap) atomicOr(ovf, 1);
+        }
+    }
+}
+
 __device__ __forceinline__ bool possible_lds_wanted(const DevJob& J, long long s, int force, int use_cells) {
     return J.lds_step && J.idx16 && !(use_cells && J.cell_cnt) && (force || due(J, s, J.pp_freq));
 }
@@ -695,7 +830,6 @@ __global__ __launch_bounds__(1024) void k_possible_lds(const DevJob* __restrict_
                                                        const int2* __restrict__ units, const int4* __restrict__ segs,
                                                        int nunits, int force, int use_cells) {
     extern __shared__ float4 lds_coord[];
-    const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
         const int2 ud = units[u];  // first segment, segments
@@ -724,95 +858,32 @@ __global__ __launch_bounds__(1024) void k_possible_lds(const DevJob* __restrict_
         if (q == ud.y) continue;
         const DevJob& J = jobs[sg.x];
         if (!possible_lds_wanted(J, st[sg.x].step, force, use_cells)) continue;
-        const int n = J.n;
-        const float4* __restrict__ cur = lds_coord + sg.w;
-        const float d2max = J.pp_d2;
-        const int cap = J.pp_cap, ecap = J.excl_cap;
-        const auto ecnt = gmem(J.excl_cnt);
-        const auto ej = gmem(J.excl_j);
-        const auto pp = gmem(static_cast<unsigned short*>(J.pp_j));
-        const auto pcnt = gmem(J.pp_cnt);
-        int* const ovf = &st[sg.x].overflow;
-        const int r0 = sg.y + t, r1 = sg.y + min(t + 64, sg.z);
-        // bead i's exclusion count and its first 64 exclusions (lane k holds
-        // the k-th), loaded one bead ahead from clamped addresses
-        auto load_ex = [&](int i, int& ne, int& ev) {
-            ne = ecnt[i];
-            ev = ej[(size_t)min(lane, ecap - 1) * n + i];
-        };
-        int ne_n, ev_n;
-        load_ex(r0, ne_n, ev_n);
-        for (int i = r0; i < r1; ++i) {
-            const int ne = ne_n;
-            int ev = lane < ne ? ev_n : INT_MAX;
-            load_ex(min(i + 1, r1 - 1), ne_n, ev_n);
-            const float4 ci = cur[i];
-            const auto row = pp + (size_t)i * J.pp_ld;
-            int eg = 0, pe = 0;  // exclusion group (of 64) and index within it
-            int e_next = ne > 0 ? __builtin_amdgcn_readlane(ev, 0) : INT_MAX;
-            int self = i;  // the bead itself, until its chunk is passed
-            int cnt = 0;
-            // One chunk of 64 candidates from `base` with coordinates c; TAIL:
-            // the chunk may run past the job's end.  The exclusion mask is
-            // formed only for chunks holding an excluded partner or the bead.
-            auto chunk = [&](int base, const float4 c, bool tail) {
-                const int lim = base + 64;
-                const float x = c.x - ci.x, y = c.y - ci.y, z = c.z - ci.z;
-                bool acc = x * x + y * y + z * z <= d2max;
-                if (tail) acc = acc & (base + lane < n);
-                unsigned long long m = __builtin_amdgcn_ballot_w64(acc);
-                if (min(e_next, self) < lim) {
-                    unsigned long long drop = 0ull;
-                    while (e_next < lim) {
-                        drop |= 1ull << (e_next - base);
-                        if (++pe == 64) {
-                            eg += 64;
-                            pe = 0;
-                            ev = eg + lane < ne ? ej[(size_t)(eg + lane) * n + i] : INT_MAX;
-                        }
-                        e_next = eg + pe < ne ? __builtin_amdgcn_readlane(ev, pe) : INT_MAX;
-                    }
-                    if (self < lim) {
-                        drop |= 1ull << (self - base);
-                        self = INT_MAX;
-                    }
-                    m &= ~drop;
-                    acc = acc && !((drop >> lane) & 1ull);
-                }
-                if (m) {
-                    if (acc) {
-                        const unsigned pos = __builtin_amdgcn_mbcnt_hi(
-                            (unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, (unsigned)cnt));
-                        if ((int)pos < cap)
-                            *reinterpret_cast<__attribute__((address_space(1))) unsigned short*>(
-                                reinterpret_cast<__attribute__((address_space(1))) char*>(row) + (pos << 1)) =
-                                (unsigned short)(base + lane);
-                    }
-                    cnt += __popcll(m);
-                }
-            };
-            int b4 = 0;
-            for (; b4 + 256 <= n; b4 += 256) {  // four whole chunks, their LDS reads in flight together
-                float4 c[4];
-#pragma unroll
-                for (int v = 0; v < 4; ++v) c[v] = cur[b4 + 64 * v + lane];
-#pragma unroll
-                for (int v = 0; v < 4; ++v) chunk(b4 + 64 * v, c[v], false);
-            }
-            if (b4 < n) {  // the last one to four chunks
-                float4 c[4];
-#pragma unroll
-                for (int v = 0; v < 4; ++v) c[v] = cur[min(b4 + 64 * v + lane, n - 1)];
-#pragma unroll
-                for (int v = 0; v < 4; ++v)
-                    if (b4 + 64 * v < n) chunk(b4 + 64 * v, c[v], true);
-            }
-            if (lane == 0) {
-                pcnt[i] = cnt < cap ? cnt : cap;
-                if (cnt > cap) atomicOr(ovf, 1);
-            }
-        }
+        possible_rows(J, lds_coord + sg.w, sg.y + t, sg.y + min(t + 64, sg.z), &st[sg.x].overflow);
     }
+}
+
+// The same lists spread for a few small jobs (the step's split_rows
+// shape): four 1024-thread workgroups per 64-row tile, one wavefront per
+// row, from the job's coordinates staged in LDS.  Grid: ntiles * 4.
+__global__ __launch_bounds__(1024) void k_possible_split(const DevJob* __restrict__ jobs, DevJobState* __restrict__ st,
+                                                         const int2* __restrict__ tiles, int ntiles, int force,
+                                                         int use_cells) {
+    extern __shared__ float4 lds_coord[];
+    const int tile = blockIdx.x >> 2, quarter = blockIdx.x & 3;
+    if (tile >= ntiles) return;
+    const int2 td = tiles[tile];
+    const DevJob& J = jobs[td.x];
+    const long long s = st[td.x].step;
+    if (!possible_lds_wanted(J, s, force, use_cells)) return;
+    const int n = J.n, row0 = td.y + 16 * quarter;
+    if (row0 >= n) return;  // every exit above is uniform over the workgroup
+    {
+        const float4* __restrict__ g = J.coord[s & 1];
+        for (int k = threadIdx.x; k < n; k += 1024) lds_coord[k] = g[k];
+    }
+    __syncthreads();
+    const int i = row0 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (i < n) possible_rows(J, lds_coord, i, i + 1, &st[td.x].overflow);
 }
 
 // ---------------------------------------------------------------------------
@@ -1210,7 +1281,7 @@ __device__ __forceinline__ void verlet_tile(const DevJob& J, DevJobState& S, con
     // b is done with its registers.  Unrolled, every prefetch has registers
     // of its own and the loads can be waited for one by one.
     if (b_lo >= b_hi) return;
-    constexpr int D = 2;
+    constexpr int D = RPW < 2 ? 1 : 2;
     int js[D][CH];
     float4 rs[D];
 #pragma unroll
@@ -1379,6 +1450,47 @@ __global__ __launch_bounds__(1024) void k_verlet_lds(const DevJob* __restrict__ 
                 __syncthreads();
             }
         }
+    }
+}
+
+// The Verlet build of the LDS-step jobs spread for a few small jobs (the
+// step's split_rows shape): four 1024-thread workgroups per 64-row tile,
+// each builds 16 rows, one wavefront per row (the same in-order ballot
+// compaction as k_verlet_lds, so the same lists), from the job's
+// coordinates staged in LDS.  Grid: ntiles * 4.
+__global__ __launch_bounds__(1024) void k_verlet_split(const DevJob* __restrict__ jobs, DevJobState* __restrict__ st,
+                                                       const int2* __restrict__ tiles, int ntiles, int force) {
+    constexpr int E = VerletLds<unsigned short>::entries, RS = VerletLds<unsigned short>::stride;
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+    unsigned short* rows = reinterpret_cast<unsigned short*>(lds_raw);
+    float4* coords = reinterpret_cast<float4*>(lds_raw + kVerletRowsBytes);
+    __shared__ int cnt_s[64];
+    const int tile = blockIdx.x >> 2, quarter = blockIdx.x & 3;
+    if (tile >= ntiles) return;
+    const int2 td = tiles[tile];
+    const DevJob& J = jobs[td.x];
+    if (!J.lds_step) return;  // built by k_verlet
+    DevJobState& S = st[td.x];
+    const long long s = S.step;
+    if (!force && !due(J, s, J.pairs_freq)) return;
+    const int n = J.n, i0 = td.y, nrow = min(64, n - i0);
+    if (16 * quarter >= nrow) return;  // every exit above is uniform over the workgroup
+    {
+        const float4* __restrict__ g = J.coord[s & 1];
+        for (int k = threadIdx.x; k < n; k += 1024) coords[k] = g[k];
+    }
+    __syncthreads();
+    const int w = threadIdx.x >> 6;
+    verlet_tile<unsigned short, 64, 8>(J, S, coords, i0, rows, cnt_s, 16 * quarter + w);
+    __syncthreads();
+    // this workgroup's 16 rows: thread (row b, entry lane kq)
+    const int b = 16 * quarter + (threadIdx.x & 15), kq = threadIdx.x >> 4;
+    if (b < nrow) {
+        const int c = cnt_s[b], cap = J.pair_cap;
+        if (kq == 0) J.pair_cnt[i0 + b] = c < cap ? c : cap;
+        const auto pj = gmem(static_cast<unsigned short*>(J.pair_j));
+        const int lim = min(c, min(E, cap));
+        for (int k = kq; k < lim; k += 64) pj[(size_t)k * J.pair_ld + i0 + b] = rows[b * RS + k];
     }
 }
 
@@ -1644,13 +1756,16 @@ hipError_t launch_forces(const DevJob* jobs, const DevJobState* st, const int2* 
 
 hipError_t launch_possible(const DevJob* jobs, DevJobState* st, const int2* tiles, int ntiles, int njobs,
                            int cell_words, int any_global, const int2* sunits, const int4* segs, int nsunits,
-                           int lds_beads, int force, hipStream_t stream) {
+                           int lds_beads, int force, hipStream_t stream, const StepShape& shape) {
     if (ntiles <= 0) return hipSuccess;
     const int use_cells = cell_words > 0;
     if (any_global || nsunits <= 0)
         hipLaunchKernelGGL(k_possible, dim3(grid_for(ntiles, 0)), dim3(256), 0, stream, jobs, st, tiles, ntiles,
                            force, use_cells, nsunits > 0);
-    if (nsunits > 0)
+    if (nsunits > 0 && shape.split_rows)
+        hipLaunchKernelGGL(k_possible_split, dim3(ntiles * 4), dim3(1024), (size_t)lds_beads * sizeof(float4), stream,
+                           jobs, st, tiles, ntiles, force, use_cells);
+    else if (nsunits > 0)
         hipLaunchKernelGGL(k_possible_lds, dim3(nsunits), dim3(kLdsUnit), (size_t)lds_beads * sizeof(float4), stream,
                            jobs, st, sunits, segs, nsunits, force, use_cells);
     if (use_cells) {
@@ -1682,7 +1797,10 @@ hipError_t launch_verlet(const DevJob* jobs, DevJobState* st, const int2* tiles,
     if (any_global || nsunits <= 0)
         hipLaunchKernelGGL(k_verlet, dim3(blocks), dim3(256), lds, stream, jobs, st, tiles, ntiles, force,
                            nsunits > 0);
-    if (nsunits > 0) {
+    if (nsunits > 0 && shape.split_rows) {
+        const size_t lds = (size_t)kVerletRowsBytes + (size_t)lds_beads * 16;
+        hipLaunchKernelGGL(k_verlet_split, dim3(ntiles * 4), dim3(1024), lds, stream, jobs, st, tiles, ntiles, force);
+    } else if (nsunits > 0) {
         const size_t lds = 2 * (size_t)kVerletRowsBytes + (size_t)lds_beads * 16;
         if (shape.verlet_chunk == 4)
             hipLaunchKernelGGL(k_verlet_lds<4>, dim3(nsunits), dim3(1024), lds, stream, jobs, st, sunits, segs, nsunits,

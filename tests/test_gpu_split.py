@@ -19,20 +19,28 @@ SPLITS = [4, 8, 16]
 @pytest.mark.parametrize("split", SPLITS)
 @pytest.mark.parametrize("n", [70, 1000, 5000])
 def test_split_forces_bit_exact(n, split):
-    """Forces-only pass: long Verlet rows (threshold 32 A: several
-    208-entry chunks per row) and a job whose last tile holds 6 rows."""
+    """Forces-only pass after list builds by the spread possible-list and
+    Verlet kernels: long Verlet rows (threshold 32 A: several 208-entry
+    chunks per row) and a job whose last tile holds 6 rows.  Forces equal
+    the oracle's, lists the one-bead-per-lane kernels'."""
     sysm = S.synth_chain(n, seed=900 + n)
     kw = c0_params(pairs_threshold=32.0)
-    with S.Engine(0, 2 << 30) as e:
-        e.set_shape(split_rows=split)
-        jid = e.add_job(S.make_params(**kw), sysm, 0)
-        e.rebuild(jid, 3)
-        print("case: split forces n=%d R=%d" % (n, split), flush=True)
-        f = e.forces(jid)
+    got = []
+    for sh in (split, 0):
+        with S.Engine(0, 2 << 30) as e:
+            e.set_shape(split_rows=sh)
+            jid = e.add_job(S.make_params(**kw), sysm, 0)
+            print("case: split lists and forces n=%d R=%d" % (n, sh), flush=True)
+            e.rebuild(jid, 3)
+            got.append((e.forces(jid), e.list(jid, 1), e.list(jid, 2)))
     ref = O.Sim(O.make_params(**kw), sysm)
     ref.build_possible()
     ref.build_verlet()
-    np.testing.assert_array_equal(f, ref.forces32())
+    np.testing.assert_array_equal(got[0][0], ref.forces32())
+    np.testing.assert_array_equal(got[1][0], ref.forces32())
+    for w in (1, 2):
+        np.testing.assert_array_equal(got[0][w][0], got[1][w][0])
+        np.testing.assert_array_equal(got[0][w][1], got[1][w][1])
 
 
 @pytest.mark.parametrize("split", SPLITS)
