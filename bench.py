@@ -487,6 +487,8 @@ def main():
     ap.add_argument("--no-c0", action="store_true")
     ap.add_argument("--c0-only", action="store_true",
                     help="run only the N = 1,000 single-chain phase (profiling)")
+    ap.add_argument("--capsid-only", action="store_true",
+                    help="run only the 100,000-bead capsid indentation phase (profiling)")
     ap.add_argument("--hi-only", action="store_true",
                     help="run only the exact-HI phase (Cholesky timing and integrator, no covariance check) "
                          "and print its JSON; for profiling the factorisation")
@@ -509,6 +511,9 @@ def main():
         return
     if args.c0_only:
         print(json.dumps({"c0_single_chain": c0_phase(S, c0_system(S), args.c0_steps)}))
+        return
+    if args.capsid_only:
+        print(json.dumps({"capsid_indentation": capsid_phase(S)}))
         return
 
     t_build = time.time()

@@ -177,6 +177,8 @@ typedef struct sop_tuning {
     int split_rows;   /* 4, 8 or 16: the jobs of at most 5120 beads are stepped with each bead's force terms
                          spread over a 1024-thread workgroup of that many beads (for a few small jobs; the
                          same results); 0: one bead per lane */
+    int split_global; /* the same for the jobs above 5120 beads (partners gathered from global memory): 4, 8
+                         or 16 beads per workgroup, 0: one bead per lane */
 } sop_tuning;
 int sop_engine_get_shape(sop_engine* e, sop_tuning* out);
 int sop_engine_set_shape(sop_engine* e, const sop_tuning* shape);

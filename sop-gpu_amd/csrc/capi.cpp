@@ -99,7 +99,7 @@ int sop_engine_set_cells(sop_engine* e, int min_beads, int enabled) {
 
 static sop_tuning to_c(const sop::Tuning& t) {
     return sop_tuning{t.max_blocks, t.unit_rows, t.shared_units, t.pair_batch, t.verlet_chunk, t.sort_every,
-                      t.split_rows};
+                      t.split_rows, t.split_global};
 }
 
 int sop_engine_get_shape(sop_engine* e, sop_tuning* out) {
@@ -113,7 +113,7 @@ int sop_engine_set_shape(sop_engine* e, const sop_tuning* t) {
     ENG(e);
     if (!t) return sop::fail(SOP_E_ARG, "null shape");
     return E.set_shape(sop::Tuning{t->max_blocks, t->unit_rows, t->shared_units, t->pair_batch, t->verlet_chunk,
-                                   t->sort_every, t->split_rows});
+                                   t->sort_every, t->split_rows, t->split_global});
 }
 
 int sop_engine_autotune(sop_engine* e, sop_tuning* best, double* ms_start, double* ms_best, int* trials) {

@@ -1583,14 +1583,15 @@ int Engine::list_stats(double* avg4, long long* beads) {
 
 Tuning Engine::tuning() const {
     return Tuning{max_blocks_,          shape_.unit_rows, shared_units_ ? 1 : 0, shape_.pair_batch,
-                  shape_.verlet_chunk, shape_.sort_every, shape_.split_rows};
+                  shape_.verlet_chunk, shape_.sort_every, shape_.split_rows, shape_.split_global};
 }
 
 int Engine::set_shape(const Tuning& t) {
     const bool ok = t.max_blocks >= 0 && t.unit_rows >= 64 && t.unit_rows <= 1024 && t.unit_rows % 64 == 0 &&
                     (t.pair_batch == 4 || t.pair_batch == 6 || t.pair_batch == 8) &&
                     (t.verlet_chunk == 4 || t.verlet_chunk == 8) && t.sort_every >= 1 && t.sort_every <= 1000 &&
-                    (t.split_rows == 0 || t.split_rows == 4 || t.split_rows == 8 || t.split_rows == 16);
+                    (t.split_rows == 0 || t.split_rows == 4 || t.split_rows == 8 || t.split_rows == 16) &&
+                    (t.split_global == 0 || t.split_global == 4 || t.split_global == 8 || t.split_global == 16);
     if (!ok) return fail(SOP_E_ARG, "launch shape out of range");
     max_blocks_ = t.max_blocks;
     shape_.unit_rows = t.unit_rows;
@@ -1598,6 +1599,7 @@ int Engine::set_shape(const Tuning& t) {
     shape_.verlet_chunk = t.verlet_chunk;
     shape_.sort_every = t.sort_every;
     shape_.split_rows = t.split_rows;
+    shape_.split_global = t.split_global;
     shared_units_ = t.shared_units != 0;
     tables_dirty_ = true;  // the unit plan depends on unit_rows and shared_units
     tiles1_n_ = -1;
@@ -1752,6 +1754,9 @@ int Engine::autotune(Tuning* best_out, double* ms_default, double* ms_best, int*
         for (int b : {256, 1024, 4096})
             if (b < full && b != best.max_blocks) { Tuning t = best; t.max_blocks = b; consider(t); }
         if (best.max_blocks != 0) { Tuning t = best; t.max_blocks = 0; consider(t); }
+        // each bead's terms spread over a workgroup: more wavefronts to hide the gathers
+        for (int v : {16, 8, 4})
+            if (v != best.split_global) { Tuning t = best; t.split_global = v; consider(t); }
     }
     // the jobs as they were, under the chosen shape
     const int rc2 = set_shape(rc ? start : best);

@@ -78,6 +78,7 @@ struct Tuning {
     int verlet_chunk;  // 64-entry chunks in flight per row in the LDS Verlet build
     int sort_every;    // Verlet rows re-sorted at every k-th Verlet rebuild
     int split_rows;    // 4, 8 or 16: LDS-step jobs stepped by k_step_split, that many rows per workgroup (0: off)
+    int split_global;  // the same for the jobs above the LDS limit (gathers from global memory)
 };
 
 class Engine {

@@ -12,13 +12,16 @@ namespace sop {
 // in flight per row in the Verlet build, and the row-sort cadence (every
 // k-th Verlet rebuild).  split_rows (4, 8 or 16; 0: off) steps the LDS-step
 // jobs by k_step_split instead, one workgroup per split_rows rows with each
-// row's force terms spread over the workgroup (for a few small jobs).
+// row's force terms spread over the workgroup (for a few small jobs), and
+// builds their lists one wavefront per row; split_global does the same for
+// the jobs above the LDS limit (gathers from global memory).
 struct StepShape {
     int unit_rows = kLdsUnit;
     int pair_batch = 6;
     int verlet_chunk = 8;
     int sort_every = kSortEvery;
     int split_rows = 0;
+    int split_global = 0;
 };
 // The autotuner tries split_rows only where the launch has at most this
 // many 64-row tiles (the spread pays where one bead per lane leaves CUs idle).

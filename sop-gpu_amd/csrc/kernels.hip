@@ -504,8 +504,9 @@ __device__ __forceinline__ SplitSrc split_fetch(const DevJob& J, int p, int r, i
     }
     return f;
 }
-__device__ __forceinline__ void split_eval(const DevJob& J, const SplitSrc& f, const float4* __restrict__ cin,
-                                           const float4 c, bool fast, float& tx, float& ty, float& tz) {
+template <typename CP>
+__device__ __forceinline__ void split_eval(const DevJob& J, const SplitSrc& f, CP cin, const float4 c, bool fast,
+                                           float& tx, float& ty, float& tz) {
     tx = -0.0f, ty = -0.0f, tz = -0.0f;
     if (f.type == 3) return;
     const float4 o = cin[f.j];
@@ -533,12 +534,13 @@ __device__ __forceinline__ void split_eval(const DevJob& J, const SplitSrc& f, c
 
 // Grid: ntiles * (64 / R) workgroups; workgroup b steps rows
 // 64 * tile + R * (b % (64 / R)) .. + R - 1 of tile b / (64 / R) when the
-// tile's job is an LDS-step job (the others are stepped by k_step).  Every
-// lane works for row lane % R: it reads that row's bead and counts itself,
-// and a worker reads its first two chunks' list entries, while the
-// coordinates are staged; wavefront 0 also reads the extra force record and
-// the state its tail needs.
-template <bool FO, int R>
+// tile's job is an LDS-step job (G: a job above the LDS limit, whose
+// partners are gathered from global memory, L2-resident; the other jobs are
+// stepped by the other kernels).  Every lane works for row lane % R: it
+// reads that row's bead and counts itself, and a worker reads its first two
+// chunks' list entries, while the coordinates are staged; wavefront 0 also
+// reads the extra force record and the state its tail needs.
+template <bool FO, int R, bool G>
 __global__ __launch_bounds__(1024) void k_step_split(const DevJob* __restrict__ jobs, const DevJobState* __restrict__ st,
                                                      const int2* __restrict__ tiles, int ntiles, int kstep,
                                                      float4* __restrict__ fout) {
@@ -550,17 +552,18 @@ __global__ __launch_bounds__(1024) void k_step_split(const DevJob* __restrict__ 
     if (tile >= ntiles) return;
     const int2 td = tiles[tile];
     const DevJob& J = jobs[td.x];
-    if (!J.lds_step) return;
+    if (G ? J.lds_step : !J.lds_step) return;
     const long long s = st[td.x].step + kstep;
     if (!FO && s >= J.end_step) return;
     const int n = J.n;
     const int row0 = td.y + R * (blockIdx.x % S);
     if (row0 >= n) return;  // every exit above is uniform over the workgroup
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-    {
+    if constexpr (!G) {
         const float4* __restrict__ g = J.coord[s & 1];
         for (int k = tid; k < n; k += 1024) lds_coord[k] = g[k];
     }
+    const float4* __restrict__ cin = G ? J.coord[s & 1] : lds_coord;
     const bool owner = w < kSplitOwners;
     // this lane's row (loads from a clamped row where the tile ends early)
     const int q = lane % R, r = row0 + q, rc = min(r, n - 1);
@@ -604,11 +607,11 @@ __global__ __launch_bounds__(1024) void k_step_split(const DevJob* __restrict__ 
             J.fextra[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
     const bool fast = div_fast_numerator(J.a2) && J.rc2 <= kDivFastMaxB;
-    const float4 c = lds_coord[i];
+    const float4 c = cin[i];
     for (int ch = 0; ch <= nch; ++ch) {
         if (!owner && ch < nch) {
             float tx, ty, tz;
-            split_eval(J, f0, lds_coord, c, fast, tx, ty, tz);
+            split_eval(J, f0, cin, c, fast, tx, ty, tz);
             f0 = f1;
             const int p = (ch + 2) * K + pos;
             if (ch + 2 < nch) f1 = split_fetch(J, p < len ? p : INT_MAX, r, i, nc, nn, np);
@@ -1073,9 +1076,20 @@ __global__ __launch_bounds__(256) void k_cells_fill(const DevJob* __restrict__ j
 
 // One 64-thread workgroup (one wavefront) per tile; dynamic LDS holds the
 // bitmap of ceil(n/64) words for the largest cell-binned job.
+// One wavefront per 64-bead tile, its beads one after another.  A bead's
+// candidates (the 3 x 3 rows of three x-adjacent cells, each row one
+// contiguous range of cell_beads) are read eight 64-entry chunks at a time,
+// all index loads and then all coordinate gathers in flight together; the
+// next bead's coordinates and first 64 exclusions are loaded while this one
+// is scanned.  Accepted j set bit j of the LDS bitmap bm and bit j / 64 of
+// bm2, so the read-back in index order visits only the 64-word blocks of bm
+// that hold an entry.
+constexpr int kCellChunks = 8;
 __global__ __launch_bounds__(64) void k_possible_cells(const DevJob* __restrict__ jobs, DevJobState* __restrict__ st,
-                                                       const int2* __restrict__ tiles, int ntiles, int force) {
+                                                       const int2* __restrict__ tiles, int ntiles, int force,
+                                                       int cell_words) {
     extern __shared__ unsigned long long bm[];
+    unsigned long long* const bm2 = bm + cell_words;  // one bit per word of bm
     const int lane = threadIdx.x;
     const int nb = gridDim.x, nxcd = nb >= 8 ? 8 : 1;
     const int slice = (ntiles + nxcd - 1) / nxcd;
@@ -1091,14 +1105,28 @@ __global__ __launch_bounds__(64) void k_possible_cells(const DevJob* __restrict_
         const int n = J.n;
         const int nwords = (n + 63) >> 6;
         for (int w = lane; w < nwords; w += 64) bm[w] = 0ull;
+        for (int w = lane; w < (nwords + 63) >> 6; w += 64) bm2[w] = 0ull;
         __syncthreads();
         const float4* __restrict__ cur = J.coord[s & 1];
+        const auto cbeads = gmem(J.cell_beads);
+        const auto cstart = gmem(J.cell_start);
+        const auto ecnt = gmem(J.excl_cnt);
+        const auto ej = gmem(J.excl_j);
         const float d2max = J.pp_d2;
-        const int cap = J.pp_cap;
+        const int cap = J.pp_cap, ecap = J.excl_cap;
         const int dx = S.cell_dim[0], dy = S.cell_dim[1], dz = S.cell_dim[2];
         const int iend = min(td.y + 64, n);
+        float4 r1n = cur[td.y];
+        int nen = ecnt[td.y];
+        int evn = ej[(size_t)min(lane, ecap - 1) * n + td.y];
         for (int i = td.y; i < iend; ++i) {
-            const float4 r1 = cur[i];
+            const float4 r1 = r1n;
+            const int ne = nen, ev = evn;
+            if (i + 1 < iend) {
+                r1n = cur[i + 1];
+                nen = ecnt[i + 1];
+                evn = ej[(size_t)min(lane, ecap - 1) * n + i + 1];
+            }
             const int cx = cell_axis(r1.x, S.cell_org[0], S.cell_inv, dx);
             const int cy = cell_axis(r1.y, S.cell_org[1], S.cell_inv, dy);
             const int cz = cell_axis(r1.z, S.cell_org[2], S.cell_inv, dz);
@@ -1107,24 +1135,34 @@ __global__ __launch_bounds__(64) void k_possible_cells(const DevJob* __restrict_
                 for (int y = max(cy - 1, 0); y <= min(cy + 1, dy - 1); ++y) {
                     const int row = (z * dy + y) * dx;
                     // the cells x-1..x+1 of one row are contiguous in cell_beads
-                    const int b = J.cell_start[row + max(cx - 1, 0)];
-                    const int e = J.cell_start[row + min(cx + 1, dx - 1) + 1];
-                    for (int k = b + lane; k < e; k += 64) {
-                        const int j = J.cell_beads[k];
-                        if (j == i) continue;
-                        const float4 r2 = cur[j];
-                        const float x = r2.x - r1.x, y2 = r2.y - r1.y, z2 = r2.z - r1.z;
-                        if (x * x + y2 * y2 + z2 * z2 <= d2max) {
-                            atomicOr(&bm[j >> 6], 1ull << (j & 63));
-                            jlo = min(jlo, j);
-                            jhi = max(jhi, j);
+                    const int b = cstart[row + max(cx - 1, 0)];
+                    const int e = cstart[row + min(cx + 1, dx - 1) + 1];
+                    for (int k0 = b; k0 < e; k0 += 64 * kCellChunks) {
+                        int jj[kCellChunks];
+#pragma unroll
+                        for (int u = 0; u < kCellChunks; ++u) {
+                            const int k = k0 + 64 * u + lane;
+                            jj[u] = k < e ? cbeads[k] : i;  // past the range: the bead itself, skipped
+                        }
+                        float4 r2[kCellChunks];
+#pragma unroll
+                        for (int u = 0; u < kCellChunks; ++u) r2[u] = cur[jj[u]];
+#pragma unroll
+                        for (int u = 0; u < kCellChunks; ++u) {
+                            const int j = jj[u];
+                            const float x = r2[u].x - r1.x, y2 = r2[u].y - r1.y, z2 = r2[u].z - r1.z;
+                            if (j != i && x * x + y2 * y2 + z2 * z2 <= d2max) {
+                                atomicOr(&bm[j >> 6], 1ull << (j & 63));
+                                atomicOr(&bm2[j >> 12], 1ull << ((j >> 6) & 63));
+                                jlo = min(jlo, j);
+                                jhi = max(jhi, j);
+                            }
                         }
                     }
                 }
             __syncthreads();
-            const int ne = J.excl_cnt[i];
             for (int k = lane; k < ne; k += 64) {
-                const int e = J.excl_j[(size_t)k * n + i];
+                const int e = k < 64 ? ev : ej[(size_t)k * n + i];
                 atomicAnd(&bm[e >> 6], ~(1ull << (e & 63)));
             }
             __syncthreads();
@@ -1134,10 +1172,12 @@ __global__ __launch_bounds__(64) void k_possible_cells(const DevJob* __restrict_
             const size_t rowp = (size_t)i * J.pp_ld;
             if (jhi >= 0) {
                 const int wlo = jlo >> 6, whi = jhi >> 6;
-                for (int wb = wlo; wb <= whi; wb += 64) {
+                for (int wb = wlo & ~63; wb <= whi; wb += 64) {
+                    const unsigned long long m2 = bm2[wb >> 6];
+                    if (m2 == 0ull) continue;  // no entry in these 64 words
                     const int w = wb + lane;
                     unsigned long long word = 0ull;
-                    if (w <= whi) {
+                    if ((m2 >> lane) & 1ull) {
                         word = bm[w];
                         bm[w] = 0ull;
                     }
@@ -1151,6 +1191,7 @@ __global__ __launch_bounds__(64) void k_possible_cells(const DevJob* __restrict_
                         word &= word - 1ull;
                     }
                     cnt += __shfl(incl, 63);
+                    if (lane == 0) bm2[wb >> 6] = 0ull;
                 }
             }
             __syncthreads();
@@ -1162,9 +1203,9 @@ __global__ __launch_bounds__(64) void k_possible_cells(const DevJob* __restrict_
     }
 }
 
-// Verlet list: one 256-thread workgroup per tile of 64 list rows (the
+// Verlet list: one 1024-thread workgroup per tile of 64 list rows (the
 // beads perm[row]).  Wave w filters the possible lists of the beads of rows
-// 16w..16w+15 in stored order with the ballot compaction; the accepted
+// 4w..4w+3 in stored order with the ballot compaction; the accepted
 // entries of the tile collect in LDS rows, and the workgroup then writes
 // them out slot-major (pair_j[k*n + row]) so each store covers the 64 rows
 // of one slot.  A bead's row is read eight 64-entry
@@ -1377,8 +1418,11 @@ __global__ __launch_bounds__(1024) void k_sort_rows(const DevJob* __restrict__ j
     }
 }
 
-__global__ __launch_bounds__(256) void k_verlet(const DevJob* __restrict__ jobs, DevJobState* __restrict__ st,
-                                                const int2* __restrict__ tiles, int ntiles, int force, int skip_lds) {
+// Wavefronts per 64-row tile of k_verlet (four rows each): sixteen, so that
+// the latency of the L2 gathers is hidden by the other wavefronts of the CU.
+constexpr int kVerletWaves = 16;
+__global__ __launch_bounds__(1024) void k_verlet(const DevJob* __restrict__ jobs, DevJobState* __restrict__ st,
+                                                 const int2* __restrict__ tiles, int ntiles, int force, int skip_lds) {
     extern __shared__ __attribute__((aligned(16))) int lds_rows[];  // 64 rows of uint16 or int32 entries
     __shared__ int cnt_s[64];
     const int nb = gridDim.x, nxcd = nb >= 8 ? 8 : 1;
@@ -1393,17 +1437,18 @@ __global__ __launch_bounds__(256) void k_verlet(const DevJob* __restrict__ jobs,
         if (!force && !due(J, s, J.pairs_freq)) continue;
         if (skip_lds && J.lds_step) continue;  // built by k_verlet_lds
         const float4* __restrict__ cur = J.coord[s & 1];
+        const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
         if (J.idx16)
-            verlet_tile<unsigned short, 4>(J, S, cur, td.y, reinterpret_cast<unsigned short*>(lds_rows), cnt_s,
-                                           wave_id());
+            verlet_tile<unsigned short, kVerletWaves>(J, S, cur, td.y, reinterpret_cast<unsigned short*>(lds_rows),
+                                                      cnt_s, w);
         else
-            verlet_tile<int, 4>(J, S, cur, td.y, lds_rows, cnt_s, wave_id());
+            verlet_tile<int, kVerletWaves>(J, S, cur, td.y, lds_rows, cnt_s, w);
         __syncthreads();
         if (J.idx16)
-            verlet_store<unsigned short, 4>(J, td.y, reinterpret_cast<const unsigned short*>(lds_rows), cnt_s,
-                                            threadIdx.x);
+            verlet_store<unsigned short, kVerletWaves>(J, td.y, reinterpret_cast<const unsigned short*>(lds_rows),
+                                                       cnt_s, threadIdx.x);
         else
-            verlet_store<int, 4>(J, td.y, static_cast<const int*>(lds_rows), cnt_s, threadIdx.x);
+            verlet_store<int, kVerletWaves>(J, td.y, static_cast<const int*>(lds_rows), cnt_s, threadIdx.x);
         __syncthreads();
     }
 }
@@ -1498,28 +1543,49 @@ __global__ __launch_bounds__(1024) void k_verlet_split(const DevJob* __restrict_
 // per-job updaters at stride boundaries
 // ---------------------------------------------------------------------------
 // One wavefront per job.  The reaction on the tip is summed in bead order,
-// as the host loop of the reference sums it: the wavefront loads 64 beads'
-// forces at a time (coalesced, and clears them), and lane 0 adds them one
-// by one from the other lanes' registers.
+// as the host loop of the reference sums it: the wavefront loads the
+// beads' forces kTipChunk at a time (coalesced, all in flight together, and
+// clears them), stages their components in LDS, and lane c < 3 adds
+// component c bead by bead; the components' sums are then gathered to lane 0.
+constexpr int kTipChunk = 512;
 __device__ void tip_update(const DevJob& J, DevJobState& S, long long s) {
+    __shared__ __align__(16) float comp[3][kTipChunk];
     const int lane = threadIdx.x & 63;
     if (s == J.retraction && lane == 0) S.V = -S.V;
     if (s % J.nav != 0) return;
-    float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f;
-    for (int base = 0; base < J.n; base += 64) {
-        const int i = base + lane;
-        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (i < J.n) {
-            v = J.tipF[i];
-            J.tipF[i] = make_float4(0.0f, 0.0f, 0.0f, v.w);
+    float acc = 0.0f;  // lane c < 3: component c
+    for (int base = 0; base < J.n; base += kTipChunk) {
+        float4 v[kTipChunk / 64];
+#pragma unroll
+        for (int u = 0; u < kTipChunk / 64; ++u) {
+            const int i = base + 64 * u + lane;
+            v[u] = i < J.n ? J.tipF[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
-        const int m = min(64, J.n - base);
-        for (int b = 0; b < m; ++b) {
-            f0 += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.x), b));
-            f1 += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.y), b));
-            f2 += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.z), b));
+#pragma unroll
+        for (int u = 0; u < kTipChunk / 64; ++u) {
+            const int i = base + 64 * u + lane;
+            if (i < J.n) J.tipF[i] = make_float4(0.0f, 0.0f, 0.0f, v[u].w);
+            comp[0][64 * u + lane] = v[u].x;
+            comp[1][64 * u + lane] = v[u].y;
+            comp[2][64 * u + lane] = v[u].z;
         }
+        __syncthreads();  // the workgroup is this one wavefront
+        const int m = min(kTipChunk, J.n - base);
+        if (lane < 3) {
+            const float* __restrict__ c = comp[lane];
+            int b = 0;
+            for (; b + 4 <= m; b += 4) {
+                const float4 q = *reinterpret_cast<const float4*>(c + b);
+                acc += q.x;
+                acc += q.y;
+                acc += q.z;
+                acc += q.w;
+            }
+            for (; b < m; ++b) acc += c[b];
+        }
+        __syncthreads();  // the sums' reads are done before the next chunk's writes
     }
+    float f0 = __shfl(acc, 0), f1 = __shfl(acc, 1), f2 = __shfl(acc, 2);
     if (lane != 0) return;
     const float fn = (float)J.nav;
     f0 /= fn;
@@ -1701,31 +1767,34 @@ static int grid_for(int ntiles, int max_blocks) {
     return b < 1 ? 1 : b;
 }
 
-// k_step_split over every tile (its workgroups of other jobs exit at once).
-template <bool FO>
+// k_step_split over every tile (its workgroups of other jobs exit at once);
+// G: the jobs above the LDS limit, gathering from global memory.
+template <bool FO, bool G>
 static void launch_split(int R, const DevJob* jobs, const DevJobState* st, const int2* tiles, int ntiles,
                          int lds_beads, int kstep, float4* out, hipStream_t stream) {
-    const size_t lds = (size_t)lds_beads * sizeof(float4);
+    const size_t lds = G ? 0 : (size_t)lds_beads * sizeof(float4);
     if (R == 4)
-        hipLaunchKernelGGL((k_step_split<FO, 4>), dim3(ntiles * 16), dim3(1024), lds, stream, jobs, st, tiles, ntiles,
-                           kstep, out);
+        hipLaunchKernelGGL((k_step_split<FO, 4, G>), dim3(ntiles * 16), dim3(1024), lds, stream, jobs, st, tiles,
+                           ntiles, kstep, out);
     else if (R == 8)
-        hipLaunchKernelGGL((k_step_split<FO, 8>), dim3(ntiles * 8), dim3(1024), lds, stream, jobs, st, tiles, ntiles,
-                           kstep, out);
+        hipLaunchKernelGGL((k_step_split<FO, 8, G>), dim3(ntiles * 8), dim3(1024), lds, stream, jobs, st, tiles,
+                           ntiles, kstep, out);
     else
-        hipLaunchKernelGGL((k_step_split<FO, 16>), dim3(ntiles * 4), dim3(1024), lds, stream, jobs, st, tiles, ntiles,
-                           kstep, out);
+        hipLaunchKernelGGL((k_step_split<FO, 16, G>), dim3(ntiles * 4), dim3(1024), lds, stream, jobs, st, tiles,
+                           ntiles, kstep, out);
 }
 
 hipError_t launch_step(const DevJob* jobs, const DevJobState* st, const int2* tiles, int ntiles, int any_global,
                        const int2* units, const int4* segs, int nunits, int lds_beads, int kstep, int max_blocks,
                        hipStream_t stream, const StepShape& shape) {
     if (ntiles <= 0) return hipSuccess;
-    if (any_global)
+    if (any_global && shape.split_global)
+        launch_split<false, true>(shape.split_global, jobs, st, tiles, ntiles, 0, kstep, nullptr, stream);
+    else if (any_global)
         hipLaunchKernelGGL(k_step<false>, dim3(grid_for(ntiles, max_blocks)), dim3(256), 0, stream, jobs, st, tiles,
                            ntiles, kstep, nullptr);
     if (nunits > 0 && shape.split_rows) {
-        launch_split<false>(shape.split_rows, jobs, st, tiles, ntiles, lds_beads, kstep, nullptr, stream);
+        launch_split<false, false>(shape.split_rows, jobs, st, tiles, ntiles, lds_beads, kstep, nullptr, stream);
     } else if (nunits > 0) {
         const dim3 g(nunits), b(shape.unit_rows);
         const size_t lds = (size_t)lds_beads * sizeof(float4);
@@ -1743,11 +1812,13 @@ hipError_t launch_forces(const DevJob* jobs, const DevJobState* st, const int2* 
                          const int2* units, const int4* segs, int nunits, int lds_beads, float4* out,
                          hipStream_t stream, const StepShape& shape) {
     if (ntiles <= 0) return hipSuccess;
-    if (any_global)
+    if (any_global && shape.split_global)
+        launch_split<true, true>(shape.split_global, jobs, st, tiles, ntiles, 0, 0, out, stream);
+    else if (any_global)
         hipLaunchKernelGGL(k_step<true>, dim3(grid_for(ntiles, 0)), dim3(256), 0, stream, jobs, st, tiles, ntiles, 0,
                            out);
     if (nunits > 0 && shape.split_rows)
-        launch_split<true>(shape.split_rows, jobs, st, tiles, ntiles, lds_beads, 0, out, stream);
+        launch_split<true, false>(shape.split_rows, jobs, st, tiles, ntiles, lds_beads, 0, out, stream);
     else if (nunits > 0)
         hipLaunchKernelGGL((k_step_lds<true, 6>), dim3(nunits), dim3(kLdsUnit), (size_t)lds_beads * sizeof(float4),
                            stream, jobs, st, units, segs, nunits, 0, out);
@@ -1777,8 +1848,8 @@ hipError_t launch_possible(const DevJob* jobs, DevJobState* st, const int2* tile
                            force);
         int blocks = ntiles;
         if (blocks >= 8) blocks = (blocks + 7) / 8 * 8;
-        hipLaunchKernelGGL(k_possible_cells, dim3(blocks), dim3(64), (size_t)cell_words * 8, stream, jobs, st, tiles,
-                           ntiles, force);
+        hipLaunchKernelGGL(k_possible_cells, dim3(blocks), dim3(64), (size_t)(cell_words + (cell_words + 63) / 64) * 8,
+                           stream, jobs, st, tiles, ntiles, force, cell_words);
     }
     return hipGetLastError();
 }
@@ -1795,8 +1866,8 @@ hipError_t launch_verlet(const DevJob* jobs, DevJobState* st, const int2* tiles,
     const size_t lds = any_idx32 ? 64 * VerletLds<int>::stride * sizeof(int)
                                  : 64 * VerletLds<unsigned short>::stride * sizeof(unsigned short);
     if (any_global || nsunits <= 0)
-        hipLaunchKernelGGL(k_verlet, dim3(blocks), dim3(256), lds, stream, jobs, st, tiles, ntiles, force,
-                           nsunits > 0);
+        hipLaunchKernelGGL(k_verlet, dim3(blocks), dim3(64 * kVerletWaves), lds, stream, jobs, st, tiles, ntiles,
+                           force, nsunits > 0);
     if (nsunits > 0 && shape.split_rows) {
         const size_t lds = (size_t)kVerletRowsBytes + (size_t)lds_beads * 16;
         hipLaunchKernelGGL(k_verlet_split, dim3(ntiles * 4), dim3(1024), lds, stream, jobs, st, tiles, ntiles, force);

@@ -266,7 +266,7 @@ int sop_server_stats(sop_server* s, double* bead_steps, double* busy_seconds, in
 int sop_server_set_shape(sop_server* s, const sop_tuning* t) {
     if (!s || !t) return sop::fail(SOP_E_ARG, "null argument");
     return s->eng->set_shape(sop::Tuning{t->max_blocks, t->unit_rows, t->shared_units, t->pair_batch,
-                                         t->verlet_chunk, t->sort_every, t->split_rows});
+                                         t->verlet_chunk, t->sort_every, t->split_rows, t->split_global});
 }
 
 int sop_server_autotune(sop_server* s, sop_tuning* best, double* ms_start, double* ms_best, int* trials) {
@@ -276,8 +276,8 @@ int sop_server_autotune(sop_server* s, sop_tuning* best, double* ms_start, doubl
     int n = 0;
     if (int rc = s->eng->autotune(&t, &a, &b, &n)) return rc;
     if (best)
-        *best = sop_tuning{t.max_blocks, t.unit_rows,  t.shared_units, t.pair_batch,
-                           t.verlet_chunk, t.sort_every, t.split_rows};
+        *best = sop_tuning{t.max_blocks,   t.unit_rows,  t.shared_units, t.pair_batch,
+                           t.verlet_chunk, t.sort_every, t.split_rows,   t.split_global};
     if (ms_start) *ms_start = a;
     if (ms_best) *ms_best = b;
     if (trials) *trials = n;

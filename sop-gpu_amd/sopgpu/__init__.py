@@ -205,7 +205,7 @@ _sig("sop_engine_set_lds_step", C.c_int, VP, C.c_int)
 class Tuning(C.Structure):
     _fields_ = [("max_blocks", C.c_int), ("unit_rows", C.c_int), ("shared_units", C.c_int),
                 ("pair_batch", C.c_int), ("verlet_chunk", C.c_int), ("sort_every", C.c_int),
-                ("split_rows", C.c_int)]
+                ("split_rows", C.c_int), ("split_global", C.c_int)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -279,9 +279,12 @@ def test_batched_jobs_with_different_list_cadences_match_oracle():
     np.testing.assert_array_equal(alone, got[0])
 
 
-def test_indentation_bit_exact(eng):
-    n = 300
-    sysm = numpy_chain(n, seed=50)
+@pytest.mark.parametrize("n", [300, 1500])
+def test_indentation_bit_exact(eng, n):
+    """The tip and substrate walls and the tip's Langevin update; the
+    reaction on the tip is summed in bead order in chunks of 512 beads
+    (1,500: two whole chunks and a partial one)."""
+    sysm = numpy_chain(n, seed=50) if n <= 400 else S.synth_chain(n, seed=50)
     c = sysm.coords[:, :3]
     top = float(c[:, 2].max())
     kw = c0_params(temperature=0.6, indentation=1, indTip=(0, 0, top + 53.0), indChip=(0, 0, top + 53.0),

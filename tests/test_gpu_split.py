@@ -69,12 +69,13 @@ def test_split_trajectory_bit_exact(split):
 @pytest.mark.parametrize("split", SPLITS)
 def test_split_batched_equals_one_lane_step(split):
     """Five jobs either side of the LDS limit (the 6,000-bead job is
-    stepped by the global kernel beside the split kernel), batched, then
-    one job alone: the same bits as the one-bead-per-lane step."""
+    stepped from global memory beside the LDS jobs, spread as well),
+    batched, then one job alone: the same bits as the one-bead-per-lane
+    step."""
     runs = []
     for sh in (0, split):
         with S.Engine(0, 4 << 30) as e:
-            e.set_shape(split_rows=sh)
+            e.set_shape(split_rows=sh, split_global=sh)
             ids = []
             for k, n in enumerate([1000, 130, 2049, 700, 6000]):
                 sysm = S.synth_chain(n, seed=920 + k)
@@ -141,8 +142,60 @@ def test_split_indentation_bit_exact():
     np.testing.assert_array_equal(ind, ref.indentation())
 
 
+@pytest.mark.parametrize("split", SPLITS)
+def test_split_global_forces_bit_exact(split):
+    """Jobs above the LDS limit, partners gathered from global memory:
+    forces of a 6,000-bead chain equal the oracle's."""
+    n = 6000
+    sysm = S.synth_chain(n, seed=970)
+    kw = c0_params(pairs_threshold=32.0)
+    with S.Engine(0, 2 << 30) as e:
+        e.set_shape(split_global=split)
+        jid = e.add_job(S.make_params(**kw), sysm, 0)
+        print("case: split_global forces n=%d R=%d" % (n, split), flush=True)
+        e.rebuild(jid, 3)
+        f = e.forces(jid)
+    ref = O.Sim(O.make_params(**kw), sysm)
+    ref.build_possible()
+    ref.build_verlet()
+    np.testing.assert_array_equal(f, ref.forces32())
+
+
+@pytest.mark.parametrize("split", [4, 16])
+def test_split_global_trajectories_equal_one_lane_step(split):
+    """A 70,000-bead chain (32-bit list entries) pulled at T > 0, and a
+    6,000-bead chain under an indentation tip, each with and without the
+    spread step: the same bits."""
+    sysm = S.synth_chain(70000, seed=980)
+    sysm.fixed = np.array([0], np.int32)
+    sysm.pulled = np.array([69999], np.int32)
+    kw = c0_params(seed=981, temperature=0.6, pulling=1, fixedEnd=0, pulledEnd=69999, deltax=0.02, pullFreq=50,
+                   nav=50, outputtiming=50, dcdfreq=50)
+    ind = S.synth_chain(6000, seed=982)
+    c = ind.coords[:, :3]
+    top = float(c[:, 2].max())
+    kwi = c0_params(temperature=0.6, indentation=1, indTip=(0, 0, top + 53.0), indChip=(0, 0, top + 53.0),
+                    indDirection=(0, 0, -1), indSurfR0=(0, 0, float(c[:, 2].min()) - 5.0), indSurfN=(0, 0, 1),
+                    indTipR=50.0, indKs=0.07, indDeltaX=0.5, indTipZeta=5000.0, indSurfA=1.0, indSurfB=-2.0,
+                    indSurfSigma=2.0, indSurfEl=0.2, nav=50, outputtiming=50, dcdfreq=50)
+    runs = []
+    for sh in (0, split):
+        with S.Engine(0, 8 << 30) as e:
+            e.set_shape(split_global=sh)
+            a = e.add_job(S.make_params(**kw), sysm, 90)
+            b = e.add_job(S.make_params(**kwi), ind, 190)
+            print("case: split_global trajectories R=%d" % sh, flush=True)
+            e.run_job(a)
+            e.run_job(b)
+            runs.append([e.coords(a), e.rng(a), e.temperature(a), e.coords(b), e.indentation(b)[:9]])
+    for x, y in zip(*runs):
+        np.testing.assert_array_equal(x, y)
+
+
 def test_split_rows_values_are_checked():
     with S.Engine(0, 1 << 26) as e:
         for bad in (2, 32, 64, -4):
             with pytest.raises(S.SopError):
                 e.set_shape(split_rows=bad)
+            with pytest.raises(S.SopError):
+                e.set_shape(split_global=bad)
