@@ -234,7 +234,8 @@ int sop_engine_list_stats(sop_engine* e, double* avg4, long long* beads);
 
 /* Hydrodynamics (src/Integrators/bdhitea.cu).  The exact path's trailing
  * update runs as split-BF16 three-product MFMA (mode 0, default) or FP32
- * MFMA (mode 1); diagonal blocks are always factorised in FP64. */
+ * MFMA (mode 1); the 64x64 diagonal blocks are factorised, and inverted for
+ * the panel solves, in FP32 in mode 0 and in FP64 in mode 1. */
 int sop_engine_set_chol_mode(sop_engine* e, int mode);
 /* epsilon and beta' of the job's last TEA update (TeaUpdater::update,
  * bdhitea.cu:133-176) */

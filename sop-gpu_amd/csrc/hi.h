@@ -13,9 +13,10 @@ enum { SOP_CHOL_BF16X3 = 0, SOP_CHOL_F32 = 1 };
 // alternate: a panel's trailing update, and the next panel's steps, still
 // read its copy while the next panel is factored).
 constexpr int kCholPanel = 512;
-// Step buffers of the 64-column steps: the FP64 inverses of two consecutive
-// 64x64 diagonal blocks, then two FP32 64x64 blocks that carry a diagonal
-// block, updated by all but its last 64 columns, from one step to the next.
+// Step buffers of the 64-column steps: the inverses of two consecutive 64x64
+// diagonal blocks (FP64, or FP32 in the first half of the slot in the
+// split-BF16 mode), then two FP32 64x64 blocks that carry a diagonal block,
+// updated by all but its last 64 columns, from one step to the next.
 constexpr size_t kCholLinvBytes = (size_t)2 * 64 * 64 * sizeof(double) + (size_t)2 * 64 * 64 * sizeof(float);
 // Column blocks left for the next step (two consecutive steps): mp x 64 FP32 each.
 inline size_t chol_bbuf_bytes(size_t mp) { return (size_t)2 * mp * 64 * sizeof(float); }
@@ -24,7 +25,7 @@ inline size_t chol_panel_bytes(size_t mp) { return (size_t)2 * mp * kCholPanel *
 
 // Device workspace of one blocked factorisation.
 struct CholWork {
-    double* linv;          // kCholLinvBytes: two FP64 inverses, then two FP32 diagonal blocks
+    double* linv;          // kCholLinvBytes: two inverse slots (FP64-sized), then two FP32 diagonal blocks
     float* bbuf;           // chol_bbuf_bytes(mp)
     unsigned short* phi;   // chol_panel_bytes(mp): split-BF16 hi of the outer panels
     unsigned short* plo;   // chol_panel_bytes(mp): lo

@@ -32,6 +32,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <type_traits>
 #include <mutex>
 #include <string>
 
@@ -563,21 +564,52 @@ __device__ __forceinline__ unsigned short f2bf(float x) {  // round to nearest e
 }
 __device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float(((unsigned int)h) << 16); }
 
-// A lane's double from another lane of the wavefront (uniform lane index).
-__device__ __forceinline__ double readlane_d(double v, int lane) {
+// The diagonal machinery below (factor and inverse of a 64x64 block, the
+// panel solve with the inverse) runs in the scalar type R: FP64 for the FP32
+// factorisation, FP32 for the split-BF16 one (whose trailing updates carry
+// errors near 1e-6 relative; an FP32 diagonal adds ~1e-7, and halves the
+// cost of the serial pivot chain and of the inverse's MFMA work).
+template <int MODE>
+using CholReal = typename std::conditional<MODE == SOP_CHOL_BF16X3, float, double>::type;
+template <typename R>
+using R4 = typename std::conditional<std::is_same<R, double>::value, f64x4, f32x4>::type;
+
+// A lane's value from another lane of the wavefront (uniform lane index).
+__device__ __forceinline__ double readlane_r(double v, int lane) {
     const long long b = __double_as_longlong(v);
     const int lo = __builtin_amdgcn_readlane((int)b, lane);
     const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
+__device__ __forceinline__ float readlane_r(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
 
-// 1/sqrt(p) in FP64: the hardware estimate (relative error below 2^-22)
-// refined by one Newton step (y <- y (3 - p y^2) / 2), which leaves a
-// relative error near 2^-44 -- far below the FP32 rounding of the factor
-// it feeds, while the pivot chain of the diagonal factorisation stays short.
-__device__ __forceinline__ double rsqrt_d(double p) {
+// 1/sqrt(p): the hardware estimate refined by one Newton step (y <- y (3 -
+// p y^2) / 2).  FP64: estimate error below 2^-22, after the step near 2^-44
+// -- far below the FP32 rounding of the factor it feeds; FP32: within an
+// ulp or two of the FP32 value.
+__device__ __forceinline__ double rsqrt_r(double p) {
     const double y = __builtin_amdgcn_rsq(p);
     return y * fma(-0.5 * p * y, y, 1.5);
+}
+__device__ __forceinline__ float rsqrt_r(float p) {
+    const float y = __builtin_amdgcn_rsqf(p);
+    return y * fmaf(-0.5f * p * y, y, 1.5f);
+}
+
+// acc += A B on the 16x16x4 MFMA of R.  Lane layout of both: A[i=l%16][k=l/16],
+// B[k=l/16][j=l%16]; D row drow(l/16, v), column l%16 in register v: FP64
+// interleaves the rows (l/16 + 4v), FP32 blocks them (4(l/16) + v).
+__device__ __forceinline__ f64x4 mfma16(double a, double b, f64x4 acc) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+}
+__device__ __forceinline__ f32x4 mfma16(float a, float b, f32x4 acc) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+}
+template <typename R>
+__device__ __forceinline__ int drow(int lk, int v) {
+    return std::is_same<R, double>::value ? lk + 4 * v : 4 * lk + v;
 }
 
 // 16-column steps of the diagonal factorisation, run by one wavefront on
@@ -589,35 +621,39 @@ __device__ __forceinline__ double rsqrt_d(double p) {
 // forward substitution of the panel solve, so both run as one pass.  Lanes
 // 0..15 accumulate scratch above their diagonal, which is never written
 // back.  y[J] receives 1/L[J][J] (uniform).
-template <int J>
-__device__ __forceinline__ void chol16_step(double (&v)[16], double (&y)[16]) {
-    y[J] = rsqrt_d(readlane_d(v[J], J));
+template <int J, typename R>
+__device__ __forceinline__ void chol16_step(R (&v)[16], R (&y)[16]) {
+    y[J] = rsqrt_r(readlane_r(v[J], J));
     v[J] *= y[J];  // lane J: L[J][J] = p / sqrt(p); below: a[r][J] / L[J][J]
     if constexpr (J + 1 < 16) {
 #pragma unroll
-        for (int c = J + 1; c < 16; ++c) v[c] = fma(-v[J], readlane_d(v[J], c), v[c]);
+        for (int c = J + 1; c < 16; ++c) v[c] = fma(-v[J], readlane_r(v[J], c), v[c]);
         chol16_step<J + 1>(v, y);
     }
 }
 
-static constexpr int LDS_ROW = NB + 1;  // row stride of the FP64 block being factored
-static constexpr int LDT = NB + 2;      // row stride of the transposed FP64 blocks (16-byte rows)
+static constexpr int LDS_ROW = NB + 1;  // row stride of the block being factored
+// row stride of the transposed blocks (Li^T): 16-byte rows, and for FP32 the
+// four row groups of an MFMA operand read in distinct bank ranges
+template <typename R>
+constexpr int ldt() { return std::is_same<R, double>::value ? NB + 2 : NB + 16; }
 
-// Column block K of the 64x64 FP64 block s (rows 16K..63) factored by
+// Column block K of the 64x64 block s (rows 16K..63) factored by
 // wavefront 0 in registers: lane r holds row 16K + r (lanes 0..15 the
 // diagonal sub-block, the others the rows below, forward-substituted in
 // the same pass).  Lanes past row 63 read a clamped row and keep zeros;
 // lanes 0..15 write their scratch above the diagonal back, where nothing
 // reads it.  dinv receives the reciprocals of the sub-block's diagonal.
-__device__ __forceinline__ void chol_pass(double (*s)[LDS_ROW], double* __restrict__ dinv, int K) {
+template <typename R>
+__device__ __forceinline__ void chol_pass(R (*s)[LDS_ROW], R* __restrict__ dinv, int K) {
     const int r = threadIdx.x & 63, q0 = 16 * K, row = q0 + r;
     const bool live = row < NB;
     const int rr = live ? row : NB - 1;
-    double v[16], y[16];
+    R v[16], y[16];
 #pragma unroll
     for (int c = 0; c < 16; ++c) {
-        const double e = s[rr][q0 + c];
-        v[c] = live && (r >= 16 || c <= r) ? e : 0.0;
+        const R e = s[rr][q0 + c];
+        v[c] = live && (r >= 16 || c <= r) ? e : R(0);
     }
     chol16_step<0>(v, y);
     if (live) {
@@ -630,36 +666,36 @@ __device__ __forceinline__ void chol_pass(double (*s)[LDS_ROW], double* __restri
     }
 }
 
-// C_IJ -= L_IK L_JK^T for the 16x16 tile (I, J) of s on FP64 MFMA, one
-// wavefront.  v_mfma_f64_16x16x4f64 lane layout: A[i=l%16][k=l/16],
-// B[k=l/16][j=l%16], D[l/16+4v][l%16] in register v.
-__device__ __forceinline__ void rank16_tile(double (*s)[LDS_ROW], int I, int J, int K) {
+// C_IJ -= L_IK L_JK^T for the 16x16 tile (I, J) of s on MFMA, one wavefront.
+template <typename R>
+__device__ __forceinline__ void rank16_tile(R (*s)[LDS_ROW], int I, int J, int K) {
     const int l = threadIdx.x & 63, lr = l & 15, lk = l >> 4;
-    f64x4 acc;
+    R4<R> acc;
 #pragma unroll
-    for (int v = 0; v < 4; ++v) acc[v] = s[16 * I + lk + 4 * v][16 * J + lr];
+    for (int v = 0; v < 4; ++v) acc[v] = s[16 * I + drow<R>(lk, v)][16 * J + lr];
 #pragma unroll
     for (int k0 = 0; k0 < 16; k0 += 4)
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-s[16 * I + lr][16 * K + k0 + lk], s[16 * J + lr][16 * K + k0 + lk],
-                                                   acc, 0, 0, 0);
+        acc = mfma16(-s[16 * I + lr][16 * K + k0 + lk], s[16 * J + lr][16 * K + k0 + lk], acc);
 #pragma unroll
-    for (int v = 0; v < 4; ++v) s[16 * I + lk + 4 * v][16 * J + lr] = acc[v];
+    for (int v = 0; v < 4; ++v) s[16 * I + drow<R>(lk, v)][16 * J + lr] = acc[v];
 }
 
 // Column c (lane c < 16) of inv(L_KK) by forward substitution, y_t =
 // (e_c - sum_{u<t} L_tu y_u) / L_tt with the sums accumulated right-looking
 // so each y_t waits for one product; written transposed to LiT (LiT[a*LDT
 // + b] = Li[b][a]).
-__device__ __forceinline__ void inv_diag16(const double (*s)[LDS_ROW], double* __restrict__ LiT,
-                                           const double* __restrict__ dinv, int K) {
+template <typename R>
+__device__ __forceinline__ void inv_diag16(const R (*s)[LDS_ROW], R* __restrict__ LiT, const R* __restrict__ dinv,
+                                           int K) {
+    constexpr int LDT = ldt<R>();
     const int c = threadIdx.x & 63, q0 = 16 * K;
     if (c < 16) {
-        double acc[16], y[16];
+        R acc[16], y[16];
 #pragma unroll
-        for (int r = 0; r < 16; ++r) acc[r] = 0.0;
+        for (int r = 0; r < 16; ++r) acc[r] = R(0);
 #pragma unroll
         for (int t = 0; t < 16; ++t) {
-            y[t] = t < c ? 0.0 : (t == c ? dinv[q0 + t] : -acc[t] * dinv[q0 + t]);
+            y[t] = t < c ? R(0) : (t == c ? dinv[q0 + t] : -acc[t] * dinv[q0 + t]);
 #pragma unroll
             for (int r = t + 1; r < 16; ++r) acc[r] = fma(s[q0 + r][q0 + t], y[t], acc[r]);
         }
@@ -668,49 +704,51 @@ __device__ __forceinline__ void inv_diag16(const double (*s)[LDS_ROW], double* _
     }
 }
 
-// Off-diagonal 16x16 block (I, J) of the inverse, J < I, on FP64 MFMA by one
+// Off-diagonal 16x16 block (I, J) of the inverse, J < I, on MFMA by one
 // wavefront, in two halves: T = sum_{J<=K<I} L_IK Li_KJ into the
 // wavefront's LDS tile (needs the rows J..I-1 of the inverse), then
 // Li_IJ = -Li_II T (needs Li_II).
-__device__ __forceinline__ void inv_sum16(const double (*s)[LDS_ROW], const double* __restrict__ LiT,
-                                          double (*T)[17], int I, int J) {
+template <typename R>
+__device__ __forceinline__ void inv_sum16(const R (*s)[LDS_ROW], const R* __restrict__ LiT, R (*T)[17], int I,
+                                          int J) {
+    constexpr int LDT = ldt<R>();
     const int l = threadIdx.x & 63, lr = l & 15, lk = l >> 4;
-    f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+    R4<R> acc = {R(0), R(0), R(0), R(0)};
     for (int k0 = 0; k0 < 16 * (I - J); k0 += 4)
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(s[16 * I + lr][16 * J + k0 + lk],
-                                                   LiT[(16 * J + lr) * LDT + 16 * J + k0 + lk], acc, 0, 0, 0);
+        acc = mfma16(s[16 * I + lr][16 * J + k0 + lk], LiT[(16 * J + lr) * LDT + 16 * J + k0 + lk], acc);
 #pragma unroll
-    for (int v = 0; v < 4; ++v) T[lk + 4 * v][lr] = acc[v];
+    for (int v = 0; v < 4; ++v) T[drow<R>(lk, v)][lr] = acc[v];
 }
-__device__ __forceinline__ void inv_apply16(double* __restrict__ LiT, const double (*T)[17], int I, int J) {
+template <typename R>
+__device__ __forceinline__ void inv_apply16(R* __restrict__ LiT, const R (*T)[17], int I, int J) {
+    constexpr int LDT = ldt<R>();
     const int l = threadIdx.x & 63, lr = l & 15, lk = l >> 4;
-    f64x4 li = {0.0, 0.0, 0.0, 0.0};
+    R4<R> li = {R(0), R(0), R(0), R(0)};
 #pragma unroll
-    for (int k0 = 0; k0 < 16; k0 += 4)
-        li = __builtin_amdgcn_mfma_f64_16x16x4f64(-LiT[(16 * I + k0 + lk) * LDT + 16 * I + lr], T[k0 + lk][lr], li, 0,
-                                                  0, 0);
+    for (int k0 = 0; k0 < 16; k0 += 4) li = mfma16(-LiT[(16 * I + k0 + lk) * LDT + 16 * I + lr], T[k0 + lk][lr], li);
 #pragma unroll
-    for (int v = 0; v < 4; ++v) LiT[(16 * J + lr) * LDT + 16 * I + lk + 4 * v] = li[v];
+    for (int v = 0; v < 4; ++v) LiT[(16 * J + lr) * LDT + 16 * I + drow<R>(lk, v)] = li[v];
 }
 
-// Factor the 64x64 FP64 block s (lower triangle, row-major) in place and
-// form the inverse of the factor, transposed, in LiT (zero above its
-// diagonal), with 256 threads.  The factor runs in four rounds, one per
-// 16-column block K: wavefront 0 applies the previous block's rank-16
-// update to block K's tiles, then factors block K in registers; meanwhile
-// wavefronts 2 and 3 apply that update to the tiles right of block K, and
-// wavefront 1 forms row K-1 of the inverse (its diagonal block by
-// substitution, the blocks left of it on FP64 MFMA) -- so the inverse trails
-// the factor by one round instead of following it.  The inverse's last row
-// is formed after the last round.  dinv receives the reciprocals of the
-// factor's diagonal.
-__device__ __forceinline__ void potrf_inv64(double (*s)[LDS_ROW], double* __restrict__ LiT, double (*T)[16][17],
-                                            double* __restrict__ dinv) {
+// Factor the 64x64 block s (lower triangle, row-major) in place and form
+// the inverse of the factor, transposed, in LiT (zero above its diagonal),
+// with 256 threads.  The factor runs in four rounds, one per 16-column
+// block K: wavefront 0 applies the previous block's rank-16 update to block
+// K's tiles, then factors block K in registers; meanwhile wavefronts 2 and
+// 3 apply that update to the tiles right of block K, and wavefront 1 forms
+// row K-1 of the inverse (its diagonal block by substitution, the blocks
+// left of it on MFMA) -- so the inverse trails the factor by one round
+// instead of following it.  The inverse's last row is formed after the
+// last round.  dinv receives the reciprocals of the factor's diagonal.
+template <typename R>
+__device__ __forceinline__ void potrf_inv64(R (*s)[LDS_ROW], R* __restrict__ LiT, R (*T)[16][17],
+                                            R* __restrict__ dinv) {
+    constexpr int LDT = ldt<R>();
     const int tid = threadIdx.x, w = wave_of();
     if (w == 0)
         chol_pass(s, dinv, 0);
     else
-        for (int e = tid - 64; e < NB * LDT; e += 192) LiT[e] = 0.0;
+        for (int e = tid - 64; e < NB * LDT; e += 192) LiT[e] = R(0);
     __syncthreads();
 #pragma unroll 1
     for (int K = 1; K < NB / 16; ++K) {
@@ -735,13 +773,13 @@ __device__ __forceinline__ void potrf_inv64(double (*s)[LDS_ROW], double* __rest
     }
     // row 3 of the inverse: its diagonal block (wavefront 0) beside the sums
     // of the blocks left of it (wavefronts 1..3), then the products
-    constexpr int R = NB / 16 - 1;
+    constexpr int RL = NB / 16 - 1;
     if (w == 0)
-        inv_diag16(s, LiT, dinv, R);
+        inv_diag16(s, LiT, dinv, RL);
     else
-        inv_sum16(s, LiT, T[w - 1], R, w - 1);
+        inv_sum16(s, LiT, T[w - 1], RL, w - 1);
     __syncthreads();
-    if (w != 0) inv_apply16(LiT, T[w - 1], R, w - 1);
+    if (w != 0) inv_apply16(LiT, T[w - 1], RL, w - 1);
     __syncthreads();
 }
 
@@ -1122,8 +1160,8 @@ struct CholStep {
     int c0, kw;                      // this outer panel
     int pc, pkw;                     // the previous outer panel (pkw = 0: none)
     int j;                           // step 0 .. nb
-    const double* li_in;             // Li^T of column block j-1 (64 x 64, row a = column a of Li)
-    double* li_out;                  // Li^T of column block j
+    const void* li_in;               // Li^T of column block j-1 (64 x 64 CholReal, row a = column a of Li)
+    void* li_out;                    // Li^T of column block j
     const float* b_in;               // B of column block j-1, mp x 64 by row
     float* b_out;                    // B of column block j
     const float* p_in;               // the diagonal block's P formed by the previous step (j >= 1)
@@ -1142,37 +1180,35 @@ __device__ __forceinline__ void load_solve_rows(const float* __restrict__ b, siz
     for (int q = 0; q < 16; ++q) bv[q] = row[4 * q];
 }
 
-// X = B Li^T for one 64x64 block on FP64 MFMA (B from load_solve_rows,
-// LiT[t*LDT + c] = Li[c][t], zero for t > c), to Xs (LDS, row stride XS)
-// and, when `out`, to M (rows r0.., columns col0..) and the panel copy
-// (columns pcol0..).  Wave w forms rows 16w..16w+15; output block jb
-// (columns 16jb..) sums t < 16(jb+1) only, where Li^T is not zero.
-// v_mfma_f64_16x16x4f64 lane layout: A[i=l%16][k=l/16], B[k=l/16][j=l%16],
-// D[l/16+4v][l%16] in register v (rows interleaved, unlike the FP32 form).
+// X = B Li^T for one 64x64 block on MFMA in CholReal (B from
+// load_solve_rows, LiT[t*LDT + c] = Li[c][t], zero for t > c), to Xs (LDS,
+// row stride XS) and, when `out`, to M (rows r0.., columns col0..) and the
+// panel copy (columns pcol0..).  Wave w forms rows 16w..16w+15; output
+// block jb (columns 16jb..) sums t < 16(jb+1) only, where Li^T is not zero.
 template <int MODE>
-__device__ __forceinline__ void solve_block(const float (&bv)[16], const double* __restrict__ LiT,
+__device__ __forceinline__ void solve_block(const float (&bv)[16], const CholReal<MODE>* __restrict__ LiT,
                                             float* __restrict__ Xs, bool out, float* __restrict__ M, size_t ld,
                                             size_t r0, size_t col0, unsigned short* __restrict__ phi,
                                             unsigned short* __restrict__ plo, int pcol0) {
+    using R = CholReal<MODE>;
+    constexpr int LDT = ldt<R>();
     const int l = threadIdx.x & 63, w = wave_of(), lr = l & 15, lk = l >> 4;
-    f64x4 acc[4];
+    R4<R> acc[4];
 #pragma unroll
-    for (int jb = 0; jb < 4; ++jb) acc[jb] = f64x4{0.0, 0.0, 0.0, 0.0};
+    for (int jb = 0; jb < 4; ++jb) acc[jb] = R4<R>{R(0), R(0), R(0), R(0)};
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
         const int t0 = 4 * q;
-        const double a = (double)bv[q];
+        const R a = (R)bv[q];
 #pragma unroll
         for (int jb = 0; jb < 4; ++jb)
-            if (t0 < 16 * (jb + 1))
-                acc[jb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, LiT[(t0 + lk) * LDT + 16 * jb + lr], acc[jb], 0, 0,
-                                                                0);
+            if (t0 < 16 * (jb + 1)) acc[jb] = mfma16(a, LiT[(t0 + lk) * LDT + 16 * jb + lr], acc[jb]);
     }
 #pragma unroll
     for (int jb = 0; jb < 4; ++jb)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const int row = 16 * w + lk + 4 * v, c = 16 * jb + lr;
+            const int row = 16 * w + drow<R>(lk, v), c = 16 * jb + lr;
             const float x = (float)acc[jb][v];
             Xs[row * XS + c] = x;
             if (!out) continue;
@@ -1346,12 +1382,21 @@ __device__ __forceinline__ void mfma_lds(f32x4 (&acc)[4], const float* __restric
 
 template <int MODE>
 __global__ __launch_bounds__(256) void k_chol_step(CholStep a) {
-    __shared__ __attribute__((aligned(16))) double SA[NB * LDT];   // B_g^T for a solve, then the block factored
-    __shared__ __attribute__((aligned(16))) double LiT[NB * LDT];  // Li^T in, then the new inverse
+    using R = CholReal<MODE>;
+    constexpr int LDT = ldt<R>();
+    // SA: the block factored (R, row stride LDS_ROW), and the partial sums of
+    // the update (FP32); LiT: Li^T in, then the new inverse (R, stride LDT),
+    // and partial sums; both sized for the larger use
+    constexpr int kSA = NB * (NB + 2);  // doubles
+    __shared__ __attribute__((aligned(16))) double SA[kSA];
+    __shared__ __attribute__((aligned(16))) double LiTraw[kSA];
     __shared__ __attribute__((aligned(16))) float Xg[NB * XS];
     __shared__ __attribute__((aligned(16))) float Xd[NB * XS];
-    __shared__ double T[3][16][17];
-    __shared__ double dinv[NB];
+    __shared__ R T[3][16][17];
+    __shared__ R dinv[NB];
+    R* const LiT = reinterpret_cast<R*>(LiTraw);
+    static_assert(NB * ldt<R>() * sizeof(R) <= sizeof(LiTraw), "Li^T fits its buffer");
+    static_assert(NB * LDS_ROW * sizeof(R) <= sizeof(SA), "the factored block fits its buffer");
     const int tid = threadIdx.x;
     const int nb = a.kw / NB, j = a.j;
     // workgroup 0: the diagonal block; 1 (when a.look): the lookahead, which
@@ -1379,15 +1424,17 @@ __global__ __launch_bounds__(256) void k_chol_step(CholStep a) {
     if (j >= 1) {
         load_solve_rows(a.b_in, rg, bg);
         if (update && x != 0 && !look) load_solve_rows(a.b_in, rd, bd);
-        {  // Li^T: eight 16-byte loads per thread, all in flight before the LDS stores
-            const double2* __restrict__ src = reinterpret_cast<const double2*>(a.li_in);
-            double2 lv[NB * NB / 512];
+        {  // Li^T: 16-byte loads, all in flight before the LDS stores
+            using V = typename std::conditional<std::is_same<R, double>::value, double2, float4>::type;
+            constexpr int EPV = 16 / sizeof(R), NV = NB * NB / (256 * EPV);
+            const V* __restrict__ src = reinterpret_cast<const V*>(a.li_in);
+            V lv[NV];
 #pragma unroll
-            for (int k = 0; k < NB * NB / 512; ++k) lv[k] = src[tid + 256 * k];
+            for (int k = 0; k < NV; ++k) lv[k] = src[tid + 256 * k];
 #pragma unroll
-            for (int k = 0; k < NB * NB / 512; ++k) {
-                const int e = 2 * (tid + 256 * k);  // e, e+1 in the same row
-                *reinterpret_cast<double2*>(LiT + (e >> 6) * LDT + (e & 63)) = lv[k];
+            for (int k = 0; k < NV; ++k) {
+                const int e = EPV * (tid + 256 * k);  // e .. e+EPV-1 in the same row
+                *reinterpret_cast<V*>(LiT + (e >> 6) * LDT + (e & 63)) = lv[k];
             }
         }
         __syncthreads();
@@ -1416,7 +1463,7 @@ __global__ __launch_bounds__(256) void k_chol_step(CholStep a) {
         else
             mfma_split<MODE, false>(part, a, rg, rd);
         constexpr int PS = NB + 1;
-        float* const P = (w < 2 ? reinterpret_cast<float*>(SA) : reinterpret_cast<float*>(LiT)) + (w & 1) * NB * PS;
+        float* const P = (w < 2 ? reinterpret_cast<float*>(SA) : reinterpret_cast<float*>(LiTraw)) + (w & 1) * NB * PS;
 #pragma unroll
         for (int bi = 0; bi < 4; ++bi)
 #pragma unroll
@@ -1425,7 +1472,7 @@ __global__ __launch_bounds__(256) void k_chol_step(CholStep a) {
                 for (int q = 0; q < 4; ++q) P[(16 * bi + 4 * (l >> 4) + q) * PS + 16 * bj + (l & 15)] = part[bi][bj][q];
         __syncthreads();
         const float* const P0 = reinterpret_cast<const float*>(SA);
-        const float* const P2 = reinterpret_cast<const float*>(LiT);
+        const float* const P2 = reinterpret_cast<const float*>(LiTraw);
 #pragma unroll
         for (int t = 0; t < 4; ++t)
 #pragma unroll
@@ -1436,7 +1483,7 @@ __global__ __launch_bounds__(256) void k_chol_step(CholStep a) {
         __syncthreads();  // SA and LiT are reused below
     }
     if (j >= 1) mfma_lds<MODE>(acc, Xg, x == 0 || look ? Xg : Xd);
-    double(*S)[LDS_ROW] = reinterpret_cast<double(*)[LDS_ROW]>(SA);
+    R(*S)[LDS_ROW] = reinterpret_cast<R(*)[LDS_ROW]>(SA);
 #pragma unroll
     for (int t = 0; t < 4; ++t)
 #pragma unroll
@@ -1446,7 +1493,7 @@ __global__ __launch_bounds__(256) void k_chol_step(CholStep a) {
             if (look)
                 a.p_out[row * NB + c] = b;
             else if (x == 0)
-                S[row][c] = (double)b;
+                S[row][c] = (R)b;
             else
                 a.b_out[(rg + row) * NB + c] = b;
         }
@@ -1458,16 +1505,18 @@ __global__ __launch_bounds__(256) void k_chol_step(CholStep a) {
         const int e = tid + 256 * k, r = e / NB, c = e % NB;
         if (c <= r) a.M[(rd + r) * a.ld + rd + c] = (float)S[r][c];
     }
-    {  // eight 16-byte LDS reads per thread, then the stores
-        double2 lv[NB * NB / 512];
+    {  // 16-byte LDS reads, then the stores
+        using V = typename std::conditional<std::is_same<R, double>::value, double2, float4>::type;
+        constexpr int EPV = 16 / sizeof(R), NV = NB * NB / (256 * EPV);
+        V lv[NV];
 #pragma unroll
-        for (int k = 0; k < NB * NB / 512; ++k) {
-            const int e = 2 * (tid + 256 * k);
-            lv[k] = *reinterpret_cast<const double2*>(LiT + (e >> 6) * LDT + (e & 63));
+        for (int k = 0; k < NV; ++k) {
+            const int e = EPV * (tid + 256 * k);
+            lv[k] = *reinterpret_cast<const V*>(LiT + (e >> 6) * LDT + (e & 63));
         }
-        double2* __restrict__ dst = reinterpret_cast<double2*>(a.li_out);
+        V* __restrict__ dst = reinterpret_cast<V*>(a.li_out);
 #pragma unroll
-        for (int k = 0; k < NB * NB / 512; ++k) dst[tid + 256 * k] = lv[k];
+        for (int k = 0; k < NV; ++k) dst[tid + 256 * k] = lv[k];
     }
 }
 
