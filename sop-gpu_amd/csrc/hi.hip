@@ -56,11 +56,26 @@ struct F6 {
 // D_ij / D_ii for a normalised separation with its length in w.
 __device__ __forceinline__ F6 rpy(const float4 dr, float a) {
     F6 d;
-    const float ra = dr.w / a;
+    // the correctly rounded quotients of the reference's expressions; where
+    // the operands lie in div_rn_fast's range (ra in (2, 2^59]: every
+    // numerator below is then in [2^-60, 1) and every divisor in (2, 2^59]),
+    // by the short division, else by the IEEE one -- the same bits
+    float ra = div_rn_fast(dr.w, a);
+    if (!(div_fast_numerator(dr.w) && a >= kDivFastMinB && a <= kDivFastMaxB)) ra = div_ieee(dr.w, a);
     float crr, cii;
     if (ra > 2.f) {
-        crr = 0.75f / ra * (1.f - 2.f / ra / ra);
-        cii = 0.75f / ra * (1.f + 2.f / 3.f / ra / ra);
+        constexpr float two_thirds = 2.f / 3.f;
+        float q0 = div_rn_fast(0.75f, ra), q1 = div_rn_fast(2.f, ra), q3 = div_rn_fast(two_thirds, ra);
+        float q2 = div_rn_fast(q1, ra), q4 = div_rn_fast(q3, ra);
+        if (!(ra <= 0x1p59f)) {
+            q0 = div_ieee(0.75f, ra);
+            q1 = div_ieee(2.f, ra);
+            q2 = div_ieee(q1, ra);
+            q3 = div_ieee(two_thirds, ra);
+            q4 = div_ieee(q3, ra);
+        }
+        crr = q0 * (1.f - q2);
+        cii = q0 * (1.f + q4);
     } else {
         crr = 3.f * ra / 32.f;
         cii = 1.f - 9.f * ra / 32.f;
@@ -81,9 +96,20 @@ __device__ __forceinline__ float4 unit_sep(const float4 a, const float4 b) {
     dr.y -= a.y;
     dr.z -= a.z;
     dr.w = sqrtf(dr.x * dr.x + dr.y * dr.y + dr.z * dr.z);
-    dr.x /= dr.w;
-    dr.y /= dr.w;
-    dr.z /= dr.w;
+    // correctly rounded quotients: the short division where w and the
+    // components lie in its range (w in [2^-60, 2^60], each component 0 or
+    // of magnitude at least 2^-60; at most w), else the IEEE one
+    const float x = dr.x, y = dr.y, z = dr.z, w = dr.w;
+    dr.x = div_rn_fast(x, w);
+    dr.y = div_rn_fast(y, w);
+    dr.z = div_rn_fast(z, w);
+    const bool fast = w >= kDivFastMinA && w <= kDivFastMaxA && div_fast_numerator(fabsf(x)) &&
+                      div_fast_numerator(fabsf(y)) && div_fast_numerator(fabsf(z));
+    if (!fast) {
+        dr.x = div_ieee(x, w);
+        dr.y = div_ieee(y, w);
+        dr.z = div_ieee(z, w);
+    }
     return dr;
 }
 
