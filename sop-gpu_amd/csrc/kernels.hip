@@ -1076,7 +1076,7 @@ __global__ __launch_bounds__(256) void k_cells_fill(const DevJob* __restrict__ j
 
 // One 64-thread workgroup (one wavefront) per tile; dynamic LDS holds the
 // bitmap of ceil(n/64) words for the largest cell-binned job.
-// One wavefront per 64-bead tile, its beads one after another.  A bead's
+// One wavefront per half tile (32 beads), its beads one after another.  A bead's
 // candidates (the 3 x 3 rows of three x-adjacent cells, each row one
 // contiguous range of cell_beads) are read eight 64-entry chunks at a time,
 // all index loads and then all coordinate gathers in flight together; the
@@ -1092,17 +1092,20 @@ __global__ __launch_bounds__(64) void k_possible_cells(const DevJob* __restrict_
     unsigned long long* const bm2 = bm + cell_words;  // one bit per word of bm
     const int lane = threadIdx.x;
     const int nb = gridDim.x, nxcd = nb >= 8 ? 8 : 1;
-    const int slice = (ntiles + nxcd - 1) / nxcd;
+    const int nhalf = 2 * ntiles;  // work units: the two halves of every tile
+    const int slice = (nhalf + nxcd - 1) / nxcd;
     const int xcd = blockIdx.x % nxcd;
-    const int tend = min((xcd + 1) * slice, ntiles);
-    for (int t = xcd * slice + blockIdx.x / nxcd; t < tend; t += nb / nxcd) {
-        const int2 td = tiles[t];
+    const int hend = min((xcd + 1) * slice, nhalf);
+    for (int h = xcd * slice + blockIdx.x / nxcd; h < hend; h += nb / nxcd) {
+        const int2 td = tiles[h >> 1];
         const DevJob& J = jobs[td.x];
         if (!J.cell_cnt) continue;
         const DevJobState& S = st[td.x];
         const long long s = S.step;
         if (!force && !due(J, s, J.pp_freq)) continue;
         const int n = J.n;
+        const int i0 = td.y + 32 * (h & 1);
+        if (i0 >= n) continue;
         const int nwords = (n + 63) >> 6;
         for (int w = lane; w < nwords; w += 64) bm[w] = 0ull;
         for (int w = lane; w < (nwords + 63) >> 6; w += 64) bm2[w] = 0ull;
@@ -1115,11 +1118,11 @@ __global__ __launch_bounds__(64) void k_possible_cells(const DevJob* __restrict_
         const float d2max = J.pp_d2;
         const int cap = J.pp_cap, ecap = J.excl_cap;
         const int dx = S.cell_dim[0], dy = S.cell_dim[1], dz = S.cell_dim[2];
-        const int iend = min(td.y + 64, n);
-        float4 r1n = cur[td.y];
-        int nen = ecnt[td.y];
-        int evn = ej[(size_t)min(lane, ecap - 1) * n + td.y];
-        for (int i = td.y; i < iend; ++i) {
+        const int iend = min(i0 + 32, n);
+        float4 r1n = cur[i0];
+        int nen = ecnt[i0];
+        int evn = ej[(size_t)min(lane, ecap - 1) * n + i0];
+        for (int i = i0; i < iend; ++i) {
             const float4 r1 = r1n;
             const int ne = nen, ev = evn;
             if (i + 1 < iend) {
@@ -1846,7 +1849,7 @@ hipError_t launch_possible(const DevJob* jobs, DevJobState* st, const int2* tile
         hipLaunchKernelGGL(k_cells_scan, dim3(njobs), dim3(1024), 0, stream, jobs, st, njobs, force);
         hipLaunchKernelGGL(k_cells_fill, dim3(grid_for(ntiles, 0)), dim3(256), 0, stream, jobs, st, tiles, ntiles,
                            force);
-        int blocks = ntiles;
+        int blocks = 2 * ntiles;  // one wavefront per half tile
         if (blocks >= 8) blocks = (blocks + 7) / 8 * 8;
         hipLaunchKernelGGL(k_possible_cells, dim3(blocks), dim3(64), (size_t)(cell_words + (cell_words + 63) / 64) * 8,
                            stream, jobs, st, tiles, ntiles, force, cell_words);
