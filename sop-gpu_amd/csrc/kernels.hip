@@ -76,6 +76,30 @@ __device__ __forceinline__ __attribute__((address_space(1))) T* gmem(T* p) {
     return (__attribute__((address_space(1))) T*)p;
 }
 
+// Coordinates gathered from global memory with global-address-space loads
+// (cin[j]), for the kernels that gather from L2: through a generic pointer
+// the gathers are flat loads, which count with the scalar loads too, so a
+// wait for a descriptor field also drained every gather in flight.
+typedef float f32x4_g __attribute__((ext_vector_type(4)));
+typedef float f32x2_g __attribute__((ext_vector_type(2)));
+// p[k] of a float2 array in global memory, by a global-address-space load.
+__device__ __forceinline__ float2 ldg2(const float2* p, size_t k) {
+    const f32x2_g v = ((const __attribute__((address_space(1))) f32x2_g*)p)[k];
+    return make_float2(v.x, v.y);
+}
+struct CoordGlobal {
+    const float4* p;
+    __device__ __forceinline__ float4 operator[](int j) const {
+        const f32x4_g v = ((const __attribute__((address_space(1))) f32x4_g*)p)[j];
+        return make_float4(v.x, v.y, v.z, v.w);
+    }
+};
+// The global-gather kernels also read the per-bead counts and the native
+// parameters by global-address-space loads; the LDS kernels keep the
+// generic loads (the global form costs them registers).
+template <typename CP>
+constexpr bool kGlobalCoords = std::is_same<CP, CoordGlobal>::value;
+
 // ---------------------------------------------------------------------------
 // fused step
 // ---------------------------------------------------------------------------
@@ -111,8 +135,11 @@ __device__ __forceinline__ float native_q(float x, float y, float z, float2 par)
 // indices of batch b+1 and the coordinates of batch b are in flight.  PB is
 // 8 for gathers from L2 and 6 for gathers from LDS (which keeps the LDS
 // kernel within the 64 VGPRs of two workgroups per CU).
-template <int PB, typename IdxT>
-__device__ __forceinline__ void pair_loop(const __attribute__((address_space(1))) IdxT* __restrict__ pj, int np, int n, const float4* __restrict__ cin,
+// TAILB: the last np % PB entries are loaded as one predicated batch
+// instead of one after another (gathers from L2, few wavefronts to hide
+// the latency of each).
+template <int PB, typename IdxT, bool TAILB = false, typename CP = const float4*>
+__device__ __forceinline__ void pair_loop(const __attribute__((address_space(1))) IdxT* __restrict__ pj, int np, int n, CP cin,
                                           const float4 c, float rc2, float a2, float m6, float& fx, float& fy,
                                           float& fz) {
     // every d2 below rc2 and above kDivFastMinB takes the short division
@@ -145,9 +172,24 @@ __device__ __forceinline__ void pair_loop(const __attribute__((address_space(1))
             for (int u = 0; u < PB; ++u) SOP_PAIR(o[u]);
         }
     }
-    for (; k < np; ++k) {
-        const float4 o = cin[(int)__builtin_nontemporal_load(pj + (size_t)k * n)];
-        SOP_PAIR(o);
+    if constexpr (TAILB) {
+        if (k < np) {
+            int jt[PB];
+#pragma unroll
+            for (int u = 0; u < PB; ++u)
+                jt[u] = k + u < np ? (int)__builtin_nontemporal_load(pj + (size_t)(k + u) * n) : 0;
+            float4 ot[PB];
+#pragma unroll
+            for (int u = 0; u < PB; ++u) ot[u] = cin[jt[u]];
+#pragma unroll
+            for (int u = 0; u < PB; ++u)
+                if (k + u < np) SOP_PAIR(ot[u]);
+        }
+    } else {
+        for (; k < np; ++k) {
+            const float4 o = cin[(int)__builtin_nontemporal_load(pj + (size_t)k * n)];
+            SOP_PAIR(o);
+        }
     }
 #undef SOP_PAIR
 }
@@ -171,9 +213,9 @@ __device__ __forceinline__ void ext_rules(const DevJob& J, int i, float& fx, flo
 // of one bead: nc covalent and nn native partners from slot-major lists of
 // slot stride n whose columns for this bead start at cj, cr, nj, npar
 // (32- or 16-bit indices; bead order, or row order for sorted rows).
-template <typename CJ, typename CR, typename NJ>
+template <typename CJ, typename CR, typename NJ, typename CP>
 __device__ __forceinline__ void bonded_terms(const DevJob& J, int nc, int nn, int n, CJ cj, CR cr, NJ nj,
-                                             const float2* __restrict__ npar, const float4* __restrict__ cin,
+                                             const float2* __restrict__ npar, CP cin,
                                              const float4 c, float& fx, float& fy, float& fz) {
     // FENE (covalent_kernel.cu:36-48)
     {
@@ -205,7 +247,7 @@ __device__ __forceinline__ void bonded_terms(const DevJob& J, int nc, int nn, in
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 jj[u] = nj[(size_t)(k + u) * n];
-                pp[u] = npar[(size_t)(k + u) * n];
+                pp[u] = kGlobalCoords<CP> ? ldg2(npar, (size_t)(k + u) * n) : npar[(size_t)(k + u) * n];
             }
             float4 o[4];
 #pragma unroll
@@ -214,7 +256,7 @@ __device__ __forceinline__ void bonded_terms(const DevJob& J, int nc, int nn, in
             for (int u = 0; u < 4; ++u) SOP_NAT(o[u], pp[u]);
         }
         for (; k < nn; ++k) {
-            const float2 par = npar[(size_t)k * n];
+            const float2 par = kGlobalCoords<CP> ? ldg2(npar, (size_t)k * n) : npar[(size_t)k * n];
             const float4 o = cin[nj[(size_t)k * n]];
             SOP_NAT(o, par);
         }
@@ -227,9 +269,9 @@ __device__ __forceinline__ void bonded_terms(const DevJob& J, int nc, int nn, in
 // row-ordered copies -- a wavefront's 64 rows read 64 consecutive words per
 // slot -- else in bead order, partner indices 16-bit where the job has a
 // 16-bit copy).  i: bead, r: its Verlet row; c = cin[i].
-template <bool FO>
-__device__ __forceinline__ void step_head(const DevJob& J, int i, int r, int n, const float4* __restrict__ cin,
-                                          const float4 c, float& fx, float& fy, float& fz) {
+template <bool FO, typename CP>
+__device__ __forceinline__ void step_head(const DevJob& J, int i, int r, int n, CP cin, const float4 c, float& fx,
+                                          float& fy, float& fz) {
     fx = 0.0f, fy = 0.0f, fz = 0.0f;
     if (J.flags & JF_EXTRA_FORCE) {
         const float4 e = J.fextra[i];
@@ -241,11 +283,13 @@ __device__ __forceinline__ void step_head(const DevJob& J, int i, int r, int n, 
         bonded_terms(J, cnt & 0xffff, cnt >> 16, J.pair_ld, gmem(J.rb_cov_j) + r, gmem(J.rb_cov_r0) + r,
                      gmem(J.rb_nat_j) + r, J.rb_nat_par + r, cin, c, fx, fy, fz);
     } else if (J.nat_j16) {
-        bonded_terms(J, J.cov_cnt[i], J.nat_cnt[i], n, gmem(J.cov_j16) + i, gmem(J.cov_r0) + i, gmem(J.nat_j16) + i,
-                     J.nat_par + i, cin, c, fx, fy, fz);
+        bonded_terms(J, kGlobalCoords<CP> ? gmem(J.cov_cnt)[i] : J.cov_cnt[i],
+                     kGlobalCoords<CP> ? gmem(J.nat_cnt)[i] : J.nat_cnt[i], n, gmem(J.cov_j16) + i, gmem(J.cov_r0) + i,
+                     gmem(J.nat_j16) + i, J.nat_par + i, cin, c, fx, fy, fz);
     } else {
-        bonded_terms(J, J.cov_cnt[i], J.nat_cnt[i], n, gmem(J.cov_j) + i, gmem(J.cov_r0) + i, gmem(J.nat_j) + i,
-                     J.nat_par + i, cin, c, fx, fy, fz);
+        bonded_terms(J, kGlobalCoords<CP> ? gmem(J.cov_cnt)[i] : J.cov_cnt[i],
+                     kGlobalCoords<CP> ? gmem(J.nat_cnt)[i] : J.nat_cnt[i], n, gmem(J.cov_j) + i, gmem(J.cov_r0) + i,
+                     gmem(J.nat_j) + i, J.nat_par + i, cin, c, fx, fy, fz);
     }
 }
 
@@ -351,20 +395,20 @@ __device__ __forceinline__ void step_tail(const DevJob& J, const DevJobState& S,
 // FO (forces only): the same force terms, written to `cout` (indexed by
 // bead) before the noise, with no state changed -- the parity hook of the
 // force gather runs the kernel the step runs.
-template <int PB, bool FO = false>
+template <int PB, bool FO = false, bool TAILB = false, typename CP = const float4*>
 __device__ __forceinline__ void step_bead(const DevJob& J, const DevJobState& S, long long s, int i, int r, int n,
-                                          const float4* __restrict__ cin, float4* __restrict__ cout) {
+                                          CP cin, float4* __restrict__ cout) {
     const float4 c = cin[i];
     float fx, fy, fz;
     step_head<FO>(J, i, r, n, cin, c, fx, fy, fz);
     // repulsive LJ over the Verlet list (pairs_kernel.cu:41-56)
     const int np = gmem(J.pair_cnt)[r];
     if (J.idx16)
-        pair_loop<PB>(gmem(static_cast<const unsigned short*>(J.pair_j)) + r, np, J.pair_ld, cin, c, J.rc2, J.a2, J.m6,
-                      fx, fy, fz);
+        pair_loop<PB, unsigned short, TAILB>(gmem(static_cast<const unsigned short*>(J.pair_j)) + r, np, J.pair_ld,
+                                             cin, c, J.rc2, J.a2, J.m6, fx, fy, fz);
     else
-        pair_loop<PB>(gmem(static_cast<const int*>(J.pair_j)) + r, np, J.pair_ld, cin, c, J.rc2, J.a2, J.m6, fx, fy,
-                      fz);
+        pair_loop<PB, int, TAILB>(gmem(static_cast<const int*>(J.pair_j)) + r, np, J.pair_ld, cin, c, J.rc2, J.a2,
+                                  J.m6, fx, fy, fz);
     step_tail<FO>(J, S, s, i, c, fx, fy, fz, cout);
 }
 
@@ -385,7 +429,8 @@ __global__ __launch_bounds__(256) void k_step(const DevJob* __restrict__ jobs, c
         const int r = td.y + lane;  // Verlet row; the lane steps bead perm[r]
         const int n = J.n;
         if (r >= n) continue;
-        step_bead<8, FO>(J, st[td.x], s, J.perm[r], r, n, J.coord[s & 1], FO ? fout : J.coord[(s + 1) & 1]);
+        step_bead<8, FO, true>(J, st[td.x], s, gmem(J.perm)[r], r, n, CoordGlobal{J.coord[s & 1]},
+                               FO ? fout : J.coord[(s + 1) & 1]);
     }
 }
 
