@@ -719,7 +719,7 @@ __global__ __launch_bounds__(256) void k_possible(const DevJob* __restrict__ job
         const long long s = st[td.x].step;
         if (!force && !due(J, s, J.pp_freq)) continue;
         const int n = J.n;
-        const float4* __restrict__ cur = J.coord[s & 1];
+        const CoordGlobal cur{J.coord[s & 1]};
         const float d2max = J.pp_d2;
         const int cap = J.pp_cap;
         const int i0 = td.y, nrow = min(64, n - i0);
@@ -1155,7 +1155,7 @@ __global__ __launch_bounds__(64) void k_possible_cells(const DevJob* __restrict_
         for (int w = lane; w < nwords; w += 64) bm[w] = 0ull;
         for (int w = lane; w < (nwords + 63) >> 6; w += 64) bm2[w] = 0ull;
         __syncthreads();
-        const float4* __restrict__ cur = J.coord[s & 1];
+        const CoordGlobal cur{J.coord[s & 1]};
         const auto cbeads = gmem(J.cell_beads);
         const auto cstart = gmem(J.cell_start);
         const auto ecnt = gmem(J.excl_cnt);
@@ -1273,8 +1273,9 @@ struct VerletLds<int> {
 
 // w: this wavefront's index among the WAVES wavefronts that build the tile.
 // CH: chunks of 64 possible-list entries whose loads are in flight together.
-template <typename IdxT, int WAVES, int CH = 8>
-__device__ __forceinline__ void verlet_tile(const DevJob& J, DevJobState& S, const float4* __restrict__ cur, int i0,
+// cur: the job's coordinates (staged in LDS, or a CoordGlobal).
+template <typename IdxT, int WAVES, int CH = 8, typename CP = const float4*>
+__device__ __forceinline__ void verlet_tile(const DevJob& J, DevJobState& S, CP cur, int i0,
                                             IdxT* __restrict__ rows, int* __restrict__ cnt_s, int w) {
     constexpr int E = VerletLds<IdxT>::entries, RS = VerletLds<IdxT>::stride;
     constexpr int RPW = 64 / WAVES;  // rows per wave
@@ -1484,7 +1485,7 @@ __global__ __launch_bounds__(1024) void k_verlet(const DevJob* __restrict__ jobs
         const long long s = S.step;
         if (!force && !due(J, s, J.pairs_freq)) continue;
         if (skip_lds && J.lds_step) continue;  // built by k_verlet_lds
-        const float4* __restrict__ cur = J.coord[s & 1];
+        const CoordGlobal cur{J.coord[s & 1]};
         const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
         if (J.idx16)
             verlet_tile<unsigned short, kVerletWaves>(J, S, cur, td.y, reinterpret_cast<unsigned short*>(lds_rows),
