@@ -271,16 +271,22 @@ __global__ __launch_bounds__(64) void k_hi_beta(const DevJob* __restrict__ jobs,
     const long long s = S.step;
     if (!(s < J.end_step && s % J.stride == 0 && s % J.hi_eps_freq == 0)) return;
     const int N = J.n;
-    __shared__ __attribute__((aligned(16))) float chunk[256];
+    // the values staged kBetaChunk at a time, all of a chunk's loads in flight
+    // together, then added in bead order
+    constexpr int kBetaChunk = 4096;
+    __shared__ __attribute__((aligned(16))) float chunk[kBetaChunk];
     double epsilon = 0.0;
-    for (int base = 0; base < N; base += 256) {
+    for (int base = 0; base < N; base += kBetaChunk) {
+        float v[kBetaChunk / 64];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kBetaChunk / 64; ++u) {
             const int i = base + 64 * u + lane;
-            chunk[64 * u + lane] = i < N ? J.eps[i] : 0.f;
+            v[u] = i < N ? J.eps[i] : 0.f;
         }
+#pragma unroll
+        for (int u = 0; u < kBetaChunk / 64; ++u) chunk[64 * u + lane] = v[u];
         __syncthreads();
-        const int m = min(256, N - base);
+        const int m = min(kBetaChunk, N - base);
         const float4* __restrict__ c4 = reinterpret_cast<const float4*>(chunk);
         int k = 0;
         for (; k + 4 <= m; k += 4) {
